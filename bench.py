@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the MinimumLBVH GPU path on one MI355X.
+
+Workload (the scale configuration): procedural displaced icosphere, level 9
+(5,242,880 triangles, de-indexed to 36-byte Triangle records), BVH built by
+the GPU builder, then ambient occlusion at 1920x1080 with 1 primary + 16 AO
+rays per hit pixel (ray length 0.2, Owen-scrambled Sobol seed 0).
+
+A step is one AO frame (primary + AO rays) over the resident BVH; `value` is
+rays traced per second over the K timed steps.  The same run also times the
+full BVH build of that mesh (Mtris/s) and the standalone sort of 2^25 uniform
+u64 keys, each against the 8 TB/s HBM roofline, and the CPU oracle (build +
+a band of the same AO frame) on this host's cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "AO MRays/s @4M tris 1080p; build Mtris/s; batched & fwd+bwd vs plain"
+HBM_PEAK_GBS = 8000.0
+BUILD_BYTES_PER_TRI = 306  # scene AABB 36 + keys 44 + 4-pass sort 72 + deltas 9 + init 4 + climb 141
+NODE_BYTES, TRI_BYTES = 64, 36
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in evs:
+        s.record()
+        fn()
+        e.record()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    ev_ms = sum(s.elapsed_time(e) for s, e in evs) / steps
+    return wall_ms, ev_ms
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--level", type=int, default=9, help="icosphere subdivision level (9 = 5,242,880 tris)")
+    ap.add_argument("--sort-log2", type=int, default=25, help="standalone sort size = 2^k u64 keys")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this path has no multi-GPU "
+              "component, see DESIGN.md for how it would shard", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from minimumlbvh_amd import lbvh, scenes
+
+    dev = torch.device("cuda:0")
+    W, H, SPP, RADIUS, SEED = 1920, 1080, 16, 0.2, 0
+    log(f"[bench] generating icosphere level {args.level}")
+    verts, faces = scenes.displaced_icosphere(args.level, seed=0)
+    tris_np = scenes.deindex(verts, faces)
+    n = tris_np.shape[0]
+    tris = torch.from_numpy(tris_np).to(dev)
+    stream = torch.cuda.current_stream()
+
+    sorter = lbvh.OnesweepSort()
+    builder = lbvh.BVHGPUBuilder()
+    log(f"[bench] build n={n}")
+
+    def build_step():
+        builder.build(tris, n, 0, sorter, stream)
+
+    build_wall, build_ev = timed(build_step, max(3, args.steps), 2)
+    build_mtris = n / (build_ev / 1e3) / 1e6
+    build_gbs = BUILD_BYTES_PER_TRI * n / (build_ev / 1e3) / 1e9
+
+    eye, ctr, up, fov = scenes.CAMERA_SPHERE
+    cam = lbvh.RayGenerator().lookat(eye, ctr, up, math.radians(fov), W, H)
+    px = torch.zeros(W * H, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(3, dtype=torch.int64, device=dev)
+    lbvh.render_ao(px, W, H, cam, builder.m_rootNode, builder.m_internals, tris, 1, SPP, RADIUS, SEED,
+                   counters=cnt)
+    torch.cuda.synchronize()
+    rays, fetches, tests = (int(x) for x in cnt.cpu().numpy().view(np.uint64))
+    ao_bytes = NODE_BYTES * fetches + TRI_BYTES * tests
+
+    def ao_step():
+        lbvh.render_ao(px, W, H, cam, builder.m_rootNode, builder.m_internals, tris, 1, SPP, RADIUS, SEED)
+
+    log(f"[bench] AO frame: {rays} rays, {fetches} node fetches, {tests} triangle tests")
+    ao_wall, ao_ev = timed(ao_step, args.steps, args.warmup)
+    mrays = rays / (ao_wall / 1e3) / 1e6
+    ao_gbs = ao_bytes / (ao_ev / 1e3) / 1e9
+
+    # standalone sort of 2^k uniform u64 keys, 8 passes of 8-bit digits
+    N = 1 << args.sort_log2
+    log(f"[bench] sort N={N}")
+    g = torch.Generator(device=dev).manual_seed(0)
+    keys0 = torch.randint(-(2**63), 2**63 - 1, (N,), dtype=torch.int64, device=dev, generator=g)
+    keys = torch.empty_like(keys0)
+    tmp = torch.empty_like(keys0)
+
+    sort_ms = []
+    for i in range(args.warmup + args.steps):
+        keys.copy_(keys0)
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        sorter.sort(keys, None, tmp, None, N, 0, 64, stream)
+        e.record()
+        torch.cuda.synchronize()
+        if i >= args.warmup:
+            sort_ms.append(s.elapsed_time(e))
+    sort_avg = sum(sort_ms) / len(sort_ms)
+    P = 8
+    sort_bytes = (2 * P + 1) * 8 * N
+    sort_gbs = sort_bytes / (sort_avg / 1e3) / 1e9
+    del keys0, keys, tmp
+
+    out = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "MRays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ao_wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"ao_icosphere_l{args.level}_{W}x{H}_{SPP}spp",
+            "triangles": n,
+            "width": W,
+            "height": H,
+            "ao_spp": SPP,
+            "ao_radius": RADIUS,
+            "sampler": "owen-scrambled sobol, seed 0",
+            "rays_per_step": rays,
+        },
+        "roofline": {
+            "kernel": "k_render_ao",
+            "bound": "hbm",
+            "achieved": round(ao_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ao_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": ao_bytes,
+            "avg_launch_ms": round(ao_ev, 4),
+            "note": "algorithmic bytes = 64 B x node fetches + 36 B x triangle tests of the frame",
+        },
+        "build": {
+            "triangles": n,
+            "ms": round(build_ev, 4),
+            "mtris_per_s": round(build_mtris, 1),
+            "algorithmic_bytes": BUILD_BYTES_PER_TRI * n,
+            "achieved_gbs": round(build_gbs, 1),
+            "frac": round(build_gbs / HBM_PEAK_GBS, 4),
+        },
+        "sort": {
+            "keys": N,
+            "key_bits": 64,
+            "passes": P,
+            "ms": round(sort_avg, 4),
+            "gkeys_per_s": round(N / (sort_avg / 1e3) / 1e9, 2),
+            "algorithmic_bytes": sort_bytes,
+            "achieved_gbs": round(sort_gbs, 1),
+            "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
+        },
+    }
+
+    if not args.no_cpu:
+        from oracle import pyoracle as orc
+
+        threads = orc.max_threads()
+        log(f"[bench] CPU oracle baseline on {threads} threads")
+        t0 = orc.now()
+        nodes, root, _, _ = orc.build(tris_np)
+        cpu_build_s = orc.now() - t0
+        og = orc.raygen_lookat(eye, ctr, up, math.radians(fov), W, H)
+        r0, r1 = H // 2 - 135, H // 2 + 135
+        t0 = orc.now()
+        _, _, ccnt = orc.render_ao(og, W, H, root, nodes, tris_np, 1, SPP, RADIUS, SEED, row0=r0, row1=r1)
+        cpu_ao_s = orc.now() - t0
+        out["cpu_baseline"] = {
+            "value": round(int(ccnt[0]) / cpu_ao_s / 1e6, 3),
+            "unit": "MRays/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"rows {r0}-{r1} of the same AO frame ({int(ccnt[0])} rays, {cpu_ao_s:.2f} s), oracle "
+                      f"stack-free traversal, OpenMP rows; plus one full oracle build of the {n}-tri mesh",
+            "build_mtris_per_s": round(n / cpu_build_s / 1e6, 2),
+            "build_s": round(cpu_build_s, 3),
+            "cpu_model": cpu_model(),
+            "nproc": os.cpu_count(),
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

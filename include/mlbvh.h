@@ -1,0 +1,150 @@
+/*
+ * mlbvh.h -- C ABI of the MI355X (gfx950) MinimumLBVH library, libmlbvh.so.
+ *
+ * Plain C: pointers, sizes and status codes only.  Device pointers are HIP
+ * device allocations; `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Every entry point is asynchronous on `stream` unless noted
+ * and returns MLBVH_OK or a negative status; mlbvh_last_error() then holds a
+ * message (thread-local).
+ *
+ * Each entry point names the reference interface of Ushio/MinimumLBVH it
+ * replaces (file:line).  The C++ facade in include/minimum_lbvh.h and
+ * include/tinyhiponesweep.h keeps the reference's class/method names on top
+ * of these functions.
+ *
+ * Binary layouts (little-endian, identical to the reference's structs):
+ *   Triangle      36 B  float v[3][3]                       minimum_lbvh.h:207
+ *   InternalNode  64 B  u32 children[2] @0 (bit31 = leaf, 0x7FFFFFFF = invalid),
+ *                       AABB aabbs[2] @8 (lower xyz, upper xyz),
+ *                       u32 parent @56, u32 context @60     minimum_lbvh.h:254-282
+ *   NodeIndex      4 B  u32, bits 0..30 index, bit 31 leaf
+ *   Hit           32 B  f32 t @0, f32 u,v @8, f32 ng[3] @16, u32 triangleIndex @28
+ *                                                            minimum_lbvh.h:890-896
+ *   IndexedMorton  8 B  u32 morton @0, u32 index @4 (a u64 with the key in
+ *                       the low word)                        minimum_lbvh.h:502-506
+ *   RayGenerator  36 B  f32 origin[3], right[3], up[3]       camera.h:4-8
+ *   TriangleAttrib 60 B f32 shadingNormals[3][3], reflectance[3], emissive[3]
+ *                                                            main_gpu.cu:10-15
+ *   Ray (trace)   32 B  f32 ox,oy,oz,tmax, dx,dy,dz, pad (this library's own
+ *                       batch format; tmax seeds Hit.t like main_gpu.cu:167)
+ */
+#ifndef MLBVH_H
+#define MLBVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#define MLBVH_API __attribute__((visibility("default")))
+#else
+#define MLBVH_API
+#endif
+
+#define MLBVH_ABI_VERSION 1
+
+#define MLBVH_OK 0
+#define MLBVH_E_INVALID_ARG (-1) /* bad size / pointer / bit range */
+#define MLBVH_E_HIP (-2)         /* a HIP runtime call failed */
+#define MLBVH_E_NO_DEVICE (-3)   /* no usable gfx950 device */
+
+/* build options, minimum_lbvh.h:42-46 */
+#define MLBVH_BUILD_OPTION_USE_NORMAL 1u
+#define MLBVH_BUILD_OPTION_CPU_PARALLEL 2u /* accepted and ignored on the GPU */
+
+/* ray queries, minimum_lbvh.h:975-979 */
+#define MLBVH_RAY_QUERY_CLOSEST 0
+#define MLBVH_RAY_QUERY_ANY 1
+
+typedef struct mlbvh_ctx mlbvh_ctx;
+
+typedef struct mlbvh_raygen {
+    float origin[3];
+    float right[3];
+    float up[3];
+} mlbvh_raygen;
+
+/* Library identity. */
+MLBVH_API int mlbvh_abi_version(void);
+MLBVH_API const char* mlbvh_last_error(void);
+
+/* Per-device context owning the sorter's and builder's scratch (look-back
+ * table, digit histograms, records, deltas).  Replaces the resources created
+ * by OnesweepSort(oroDevice) (tinyhiponesweep.h:575-579) and
+ * BVHGPUBuilder(kernel, includeDir) (minimum_lbvh.h:729-775); kernels are
+ * compiled ahead of time for gfx950, so nothing is compiled at run time.
+ * Not re-entrant: calls sharing one context must be ordered on one stream. */
+MLBVH_API int mlbvh_ctx_create(int device, mlbvh_ctx** out);
+MLBVH_API int mlbvh_ctx_destroy(mlbvh_ctx* ctx);
+
+/* Stable LSD radix sort, OnesweepSort::sort(KeyValueSoA32|64, tmp, n,
+ * startBit, endBit, stream) (tinyhiponesweep.h:592-599).  Sorts ascending by
+ * the key bits [startBit, startBit + 8*ceil((endBit-startBit)/8)) (bits past
+ * the key width read as 0); values == NULL means key-only.  The result ends in
+ * keys/values; keys_tmp/values_tmp are clobbered.  n may be 0. */
+MLBVH_API int mlbvh_sort_u32(mlbvh_ctx* ctx, uint32_t* keys, uint32_t* values, uint32_t* keys_tmp, uint32_t* values_tmp,
+                   uint32_t n, int startBit, int endBit, void* stream);
+MLBVH_API int mlbvh_sort_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* values, uint64_t* keys_tmp, uint32_t* values_tmp,
+                   uint32_t n, int startBit, int endBit, void* stream);
+
+/* LBVH build, BVHGPUBuilder::build(triangles, n, buildOption, sorter, stream)
+ * (minimum_lbvh.h:783-871): scene AABB -> 32-bit Morton keys -> stable sort of
+ * (morton, index) -> deltas -> agglomerative build.  Writes n-1 InternalNode
+ * to d_nodes and the root NodeIndex to d_root.  2 <= n <= 2^31-1.
+ * No host synchronisation and no timing printf (the reference syncs at :869). */
+MLBVH_API int mlbvh_build(mlbvh_ctx* ctx, const void* d_triangles, int n, uint32_t buildOption, void* d_nodes, void* d_root,
+                void* stream);
+
+/* mlbvh_build plus optional copies of the sorted IndexedMorton records
+ * (n x u64) and the uint8 deltas (n-1), which mlbvh_refit consumes.
+ * No reference counterpart beyond the locals of minimum_lbvh.h:790-796. */
+MLBVH_API int mlbvh_build_ex(mlbvh_ctx* ctx, const void* d_triangles, int n, uint32_t buildOption, void* d_nodes, void* d_root,
+                   uint64_t* d_sorted_records, uint8_t* d_deltas, void* stream);
+
+/* Refit: recompute every node AABB for moved vertices with the topology
+ * fixed by an earlier build's sorted records and deltas (re-runs the
+ * agglomerative climb of minimum_lbvh.h:284-361 on unchanged deltas).  No
+ * reference counterpart (the reference always rebuilds). */
+MLBVH_API int mlbvh_refit(mlbvh_ctx* ctx, const void* d_triangles, int n, const uint64_t* d_sorted_records,
+                const uint8_t* d_deltas, void* d_nodes, void* d_root, void* stream);
+
+/* Batch trace with intersect_stackfree (minimum_lbvh.h:981-1061).
+ * d_rays: nRays x 32 B Ray; d_hits: nRays x 32 B Hit.  d_counters (nullable):
+ * 2 x u64 accumulated [internal-node fetches, triangle tests]. */
+MLBVH_API int mlbvh_trace(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles, const void* d_root,
+                const float* d_rays, void* d_hits, int nRays, int rayQuery, unsigned long long* d_counters,
+                void* stream);
+
+/* Host-side RayGenerator::lookat (camera.h:10-22); fovy in radians. */
+MLBVH_API int mlbvh_raygen_lookat(mlbvh_raygen* out, const float eye[3], const float center[3], const float up[3], float fovy,
+                        int width, int height);
+
+/* Render kernels of main_gpu.cu.  `camera` is a host pointer read at launch. */
+/* normal (main_gpu.cu:84-109) */
+MLBVH_API int mlbvh_render_normal(mlbvh_ctx* ctx, uint32_t* d_pixels, int width, int height, const mlbvh_raygen* camera,
+                        const void* d_root, const void* d_nodes, const void* d_triangles, void* stream);
+/* ao (main_gpu.cu:111-177), with the sample count, ray length and Sobol seed
+ * as parameters (the reference hard-codes 16, 1.0f and 12345).  d_ao
+ * (nullable) receives the occlusion fraction per pixel, -1 on a primary miss.
+ * d_counters (nullable): 3 x u64 accumulated [rays traced, node fetches,
+ * triangle tests]. */
+MLBVH_API int mlbvh_render_ao(mlbvh_ctx* ctx, uint32_t* d_pixels, float* d_ao, int width, int height,
+                    const mlbvh_raygen* camera, const void* d_root, const void* d_nodes, const void* d_triangles,
+                    int useSobol, int nSample, float radius, uint32_t seed, unsigned long long* d_counters,
+                    void* stream);
+/* pt (main_gpu.cu:179-271): one sample per pixel per call, accumulated in
+ * d_accumulators (float4 per pixel, .w = samples so far); maxDepth bounds the
+ * bounce loop (the reference uses 1024). */
+MLBVH_API int mlbvh_render_pt(mlbvh_ctx* ctx, float* d_accumulators, int width, int height, const mlbvh_raygen* camera,
+                    const void* d_root, const void* d_nodes, const void* d_triangles, const void* d_attribs,
+                    int useSobol, int maxSPP, int maxDepth, void* stream);
+/* pack (main_gpu.cu:273-286) */
+MLBVH_API int mlbvh_pack(mlbvh_ctx* ctx, uint32_t* d_pixels, const float* d_accumulators, int n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

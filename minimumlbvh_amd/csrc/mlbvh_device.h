@@ -1,0 +1,144 @@
+// Device-side types and IEEE-exact vector math shared by the gfx950 kernels.
+//
+// Layouts are the ABI of the reference's minimum_lbvh.h (Triangle 36 B,
+// AABB 24 B, NodeIndex 4 B, InternalNode 64 B, Hit 32 B; minimum_lbvh.h:207-282,
+// 890-896).  Arithmetic is written operation by operation in the order the
+// reference's helper_math.h evaluates it, and every file is compiled with
+// -ffp-contract=off and HIP's default correctly-rounded fp32 divide/sqrt, so
+// results are bit-identical to the reference's host arithmetic for finite
+// inputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mlbvh.h"
+
+namespace mlbvh {
+
+constexpr float kFltMax = 3.402823466e+38F;  // MINIMUM_LBVH_FLT_MAX
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kInvalidNode = 0x7FFFFFFFu;  // NodeIndex::invalid()
+constexpr uint32_t kMortonMax = 0x1FFFFFu;      // MORTON_MAX_VALUE_3D
+
+struct f3 {
+    float x, y, z;
+};
+struct Tri {
+    f3 v[3];
+};
+struct Box {
+    f3 lo, hi;
+};
+struct Node {
+    uint32_t child[2];
+    Box box[2];
+    uint32_t parent;
+    uint32_t context;
+};
+struct HitRec {
+    float t;
+    uint32_t pad;
+    float u, v;
+    float ngx, ngy, ngz;
+    uint32_t tri;
+};
+static_assert(sizeof(Tri) == 36, "Triangle is 36 bytes");
+static_assert(sizeof(Box) == 24, "AABB is 24 bytes");
+static_assert(sizeof(Node) == 64, "InternalNode is 64 bytes");
+static_assert(sizeof(HitRec) == 32, "Hit is 32 bytes");
+
+__host__ __device__ inline f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ inline f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline f3 operator*(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ inline f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ inline f3 operator/(f3 a, f3 b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
+__device__ inline f3 operator/(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ inline f3 operator-(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline f3 cross(f3 a, f3 b)
+{
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// normalize with 1/sqrtf (the reference's host rsqrtf), not the approximate rsq.
+__device__ inline f3 normalize(f3 v)
+{
+    float inv = 1.0f / sqrtf(dot(v, v));
+    return v * inv;
+}
+// Host helper_math semantics of fminf/fmaxf (a<b?a:b, a>b?a:b): used where the
+// chosen operand is stored (AABB unions), so even the sign of zero matches.
+__device__ inline float hmin(float a, float b) { return a < b ? a : b; }
+__device__ inline float hmax(float a, float b) { return a > b ? a : b; }
+__device__ inline f3 hmin3(f3 a, f3 b) { return mk(hmin(a.x, b.x), hmin(a.y, b.y), hmin(a.z, b.z)); }
+__device__ inline f3 hmax3(f3 a, f3 b) { return mk(hmax(a.x, b.x), hmax(a.y, b.y), hmax(a.z, b.z)); }
+
+// float -> int truncation with x86-64 cvttss2si semantics (out of range and
+// NaN give INT_MIN), the behaviour of the reference's host make_int3(float3).
+__device__ inline int f2i_trunc(float f)
+{
+    return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : (int)0x80000000u;
+}
+__device__ inline int iclamp(int f, int a, int b)
+{
+    int m = f < b ? f : b;
+    return a > m ? a : m;
+}
+
+// Spread the low 21 bits to every third bit (bit k -> 3k).
+__device__ inline uint64_t splat3(uint32_t a)
+{
+    uint64_t x = a & 0x1fffffu;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+// 63-bit Morton code of p inside box (x -> bit 0, y -> bit 1, z -> bit 2).
+__device__ inline uint64_t morton_in_box(f3 lo, f3 hi, f3 p)
+{
+    f3 f = ((p - lo) / (hi - lo)) * (float)(kMortonMax + 1);
+    uint32_t cx = (uint32_t)iclamp(f2i_trunc(f.x), 0, (int)kMortonMax);
+    uint32_t cy = (uint32_t)iclamp(f2i_trunc(f.y), 0, (int)kMortonMax);
+    uint32_t cz = (uint32_t)iclamp(f2i_trunc(f.z), 0, (int)kMortonMax);
+    return splat3(cx) | splat3(cy) << 1 | splat3(cz) << 2;
+}
+
+// Order-preserving float <-> uint map for atomic min/max of floats.
+__device__ inline uint32_t f2ord(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ inline float ord2f(uint32_t k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// Pinhole camera of the reference's camera.h (origin, right, up; rd is not
+// normalised).
+__device__ inline void shoot(const mlbvh_raygen& g, f3* ro, f3* rd, float u, float v)
+{
+    f3 origin = mk(g.origin[0], g.origin[1], g.origin[2]);
+    f3 right = mk(g.right[0], g.right[1], g.right[2]);
+    f3 up = mk(g.up[0], g.up[1], g.up[2]);
+    f3 forward = normalize(cross(up, right));
+    // lerp(a, b, t) = a + t * (b - a)
+    f3 nr = -right;
+    f3 lr = nr + (right - nr) * u;
+    f3 nu = -up;
+    f3 lu = up + (nu - up) * v;
+    f3 to = ((origin + forward) + lr) + lu;
+    *ro = origin;
+    *rd = to - origin;
+}
+
+// Error plumbing shared by the entry points.
+void set_error(const char* fmt, ...);
+int hip_status(hipError_t e, const char* what);
+
+}  // namespace mlbvh

@@ -1,0 +1,587 @@
+// Stack-free parent-pointer traversal (intersect_stackfree,
+// minimum_lbvh.h:981-1061) and the kernels it drives: batch trace and the
+// normal / AO / path-tracing / pack kernels of main_gpu.cu.
+//
+// One ray per lane.  The image kernels give each wave64 an 8x8 pixel tile
+// (a 256-thread block covers 16x16 pixels) so neighbouring lanes walk the
+// same part of the tree.  A node visit is four 16-byte loads of the 64-byte
+// InternalNode; the slab test uses the hardware v_min/v_max (identical to the
+// reference's host min/max for the NaN-free values a finite ray produces).
+#include "mlbvh_ctx.h"
+
+namespace mlbvh {
+
+constexpr int kTraceBlock = 256;
+constexpr uint32_t kMaxTraversalSteps = 1u << 26;
+
+struct NodeView {
+    uint32_t c0, c1, parent;
+    f3 lo0, hi0, lo1, hi1;
+};
+
+__device__ inline NodeView load_node(const Node* nodes, uint32_t i)
+{
+    const uint4* p = reinterpret_cast<const uint4*>(nodes + i);
+    uint4 a = p[0], b = p[1], c = p[2], d = p[3];
+    NodeView v;
+    v.c0 = a.x;
+    v.c1 = a.y;
+    v.lo0 = mk(__uint_as_float(a.z), __uint_as_float(a.w), __uint_as_float(b.x));
+    v.hi0 = mk(__uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w));
+    v.lo1 = mk(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z));
+    v.hi1 = mk(__uint_as_float(c.w), __uint_as_float(d.x), __uint_as_float(d.y));
+    v.parent = d.z;
+    return v;
+}
+
+// slabs(), minimum_lbvh.h:82-96
+__device__ inline float2 slabs(f3 ro, f3 inv, f3 lo, f3 hi, float tHit)
+{
+    f3 t0 = (lo - ro) * inv;
+    f3 t1 = (hi - ro) * inv;
+    float tminx = fminf(t0.x, t1.x), tminy = fminf(t0.y, t1.y), tminz = fminf(t0.z, t1.z);
+    float tmaxx = fmaxf(t0.x, t1.x), tmaxy = fmaxf(t0.y, t1.y), tmaxz = fmaxf(t0.z, t1.z);
+    float rmin = fmaxf(fmaxf(tminx, tminy), tminz);
+    float rmax = fminf(fminf(tmaxx, tmaxy), tmaxz) * 1.00000024f;
+    rmin = fmaxf(rmin, 0.0f);
+    rmax = fminf(rmax, tHit);
+    return make_float2(rmin, rmax);
+}
+
+// invRd(), minimum_lbvh.h:898-901
+__device__ inline f3 inv_rd(f3 rd)
+{
+    f3 r = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    r.x = fmaxf(-kFltMax, fminf(r.x, kFltMax));
+    r.y = fmaxf(-kFltMax, fminf(r.y, kFltMax));
+    r.z = fmaxf(-kFltMax, fminf(r.z, kFltMax));
+    return r;
+}
+
+// intersectRayTriangle(), minimum_lbvh.h:363-413
+__device__ inline bool isect_tri(float* tOut, float* uOut, float* vOut, f3* ngOut, float tmin, float tmax, f3 ro,
+                                 f3 rd, f3 v0, f3 v1, f3 v2)
+{
+    f3 e0 = v1 - v0;
+    f3 e1 = v2 - v1;
+    f3 ng = cross(e0, e1);
+    float t = dot(v0 - ro, ng) / dot(ng, rd);
+    if (tmin <= t && t <= tmax) {
+        f3 e2 = v0 - v2;
+        f3 ro2 = ro * 2.0f;
+        float u = dot(cross(e2, (v0 + v2) - ro2), rd);
+        float v = dot(cross(e0, (v1 + v0) - ro2), rd);
+        float w = dot(cross(e1, (v2 + v1) - ro2), rd);
+        if (fminf(fminf(u, v), w) < 0.0f && 0.0f < fmaxf(fmaxf(u, v), w)) return false;
+        float vol = (u + v) + w;
+        *tOut = t;
+        *uOut = u / vol;
+        *vOut = v / vol;
+        *ngOut = ng;
+        return true;
+    }
+    return false;
+}
+
+struct Hit {
+    float t, u, v;
+    f3 ng;
+    uint32_t tri;
+};
+
+template <bool kAny, bool kCount>
+__device__ inline void traverse(const Node* __restrict__ nodes, const Tri* __restrict__ tris, uint32_t root, f3 ro,
+                                f3 rd, f3 inv, Hit& hit, uint32_t& nFetch, uint32_t& nTest)
+{
+    uint32_t curr = root;
+    uint32_t prev = kInvalidNode;
+    // A well-formed tree visits each node at most three times; the cap only
+    // stops a malformed node array from looping forever.
+    uint32_t guard = kMaxTraversalSteps;
+    while (curr != kInvalidNode && guard-- != 0) {
+        if (curr & kLeafBit) {
+            const Tri tri = tris[curr & ~kLeafBit];
+            if (kCount) nTest++;
+            float t, u, v;
+            f3 ng;
+            if (isect_tri(&t, &u, &v, &ng, 0.0f, hit.t, ro, rd, tri.v[0], tri.v[1], tri.v[2])) {
+                hit.t = t;
+                hit.u = u;
+                hit.v = v;
+                hit.ng = ng;
+                hit.tri = curr & ~kLeafBit;
+                if (kAny) break;
+            }
+            uint32_t tmp = curr;
+            curr = prev;
+            prev = tmp;
+            continue;
+        }
+        const NodeView nd = load_node(nodes, curr);
+        if (kCount) nFetch++;
+        float2 rl = slabs(ro, inv, nd.lo0, nd.hi0, hit.t);
+        float2 rr = slabs(ro, inv, nd.lo1, nd.hi1, hit.t);
+        bool hitL = rl.x <= rl.y;
+        bool hitR = rr.x <= rr.y;
+        uint32_t nearN = nd.c0, farN = nd.c1;
+        int nHits = 0;
+        if (hitL && hitR) {
+            if (rr.x < rl.x) {
+                nearN = nd.c1;
+                farN = nd.c0;
+            }
+            nHits = 2;
+        } else if (hitL || hitR) {
+            nHits = 1;
+            nearN = hitR ? farN : nearN;
+        }
+        uint32_t next;
+        if (prev == nd.parent) next = 0 < nHits ? nearN : nd.parent;
+        else if (prev == nearN) next = nHits == 2 ? farN : nd.parent;
+        else next = nd.parent;
+        prev = curr;
+        curr = next;
+    }
+}
+
+__device__ inline void wave_add_u64(unsigned long long* dst, uint64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+template <bool kAny, bool kCount>
+__global__ void __launch_bounds__(kTraceBlock)
+    k_trace(const Node* __restrict__ nodes, const Tri* __restrict__ tris, const uint32_t* __restrict__ rootPtr,
+            const float4* __restrict__ rays, HitRec* __restrict__ hits, int nRays, unsigned long long* counters)
+{
+    int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    uint32_t nFetch = 0, nTest = 0;
+    if (i < nRays) {
+        float4 a = rays[2 * i], b = rays[2 * i + 1];
+        f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
+        Hit h;
+        h.t = a.w;
+        h.u = h.v = 0.0f;
+        h.ng = mk(0, 0, 0);
+        h.tri = 0xFFFFFFFFu;
+        traverse<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
+        HitRec r;
+        r.t = h.t;
+        r.pad = 0;
+        r.u = h.u;
+        r.v = h.v;
+        r.ngx = h.ng.x;
+        r.ngy = h.ng.y;
+        r.ngz = h.ng.z;
+        r.tri = h.tri;
+        hits[i] = r;
+    }
+    if (kCount) {
+        wave_add_u64(&counters[0], nFetch);
+        wave_add_u64(&counters[1], nTest);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Shading helpers of main_gpu.cu:17-82
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t pack_rgba(float r, float g, float b)
+{
+    int x = iclamp(f2i_trunc(r * 255.0f + 0.5f), 0, 255);
+    int y = iclamp(f2i_trunc(g * 255.0f + 0.5f), 0, 255);
+    int z = iclamp(f2i_trunc(b * 255.0f + 0.5f), 0, 255);
+    return ((uint32_t)z << 16) | ((uint32_t)y << 8) | (uint32_t)x;
+}
+
+__device__ inline f3 hemisphere_cos(float xi0, float xi1)
+{
+    const float kPi = 3.14159265358979323846f;
+    float phi = (xi0 * 2.0f) * kPi;
+    float r = sqrtf(xi1);
+    float x = cosf(phi) * r;
+    float z = sinf(phi) * r;
+    float y = sqrtf(fmaxf(1.0f - r * r, 0.0f));
+    return mk(x, y, z);
+}
+
+// Orthonormal basis of Duff et al. 2017.
+__device__ inline void onb(f3 z, f3* xaxis, f3* yaxis)
+{
+    const float sign = copysignf(1.0f, z.z);
+    const float a = -1.0f / (sign + z.z);
+    const float b = (z.x * z.y) * a;
+    *xaxis = mk(1.0f + ((sign * z.x) * z.x) * a, sign * b, -sign * z.x);
+    *yaxis = mk(b, sign + (z.y * z.y) * a, -z.y);
+}
+
+__device__ inline uint32_t hash_pcg(uint32_t v)
+{
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28) + 4)) ^ state) * 277803737u;
+    return (word >> 22) ^ word;
+}
+
+struct Pcg {
+    uint64_t state, inc;
+    __device__ Pcg(uint64_t seed, uint64_t seq)
+    {
+        state = 0u;
+        inc = (seq << 1u) | 1u;
+        next();
+        state += seed;
+        next();
+    }
+    __device__ uint32_t next()
+    {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31));
+    }
+    __device__ float uniformf() { return __uint_as_float((next() >> 9) | 0x3f800000u) - 1.0f; }
+};
+
+// Owen-scrambled, shuffled 2D Sobol (Burley 2020) as in sobol.h:110-129.
+__device__ inline uint32_t lk_perm(uint32_t x, uint32_t seed)
+{
+    x += seed;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return x;
+}
+__device__ inline uint32_t pascal(uint32_t v)
+{
+    v ^= v << 16;
+    v ^= (v & 0x00FF00FFu) << 8;
+    v ^= (v & 0x0F0F0F0Fu) << 4;
+    v ^= (v & 0x33333333u) << 2;
+    v ^= (v & 0x55555555u) << 1;
+    return v;
+}
+__device__ inline float u24(uint32_t u) { return (float)(u >> 8) / 16777216.0f; }
+__device__ inline void sobol2d(float* x, float* y, uint32_t index, uint32_t p0, uint32_t p1, uint32_t p2)
+{
+    uint32_t vx = p0 * 1664525u + 1013904223u;
+    uint32_t vy = p1 * 1664525u + 1013904223u;
+    uint32_t vz = p2 * 1664525u + 1013904223u;
+    vx += vy * vz;
+    vy += vz * vx;
+    vz += vx * vy;
+    vx ^= vx >> 16u;
+    vy ^= vy >> 16u;
+    vz ^= vz >> 16u;
+    vx += vy * vz;
+    vy += vz * vx;
+    vz += vx * vy;
+    uint32_t v = lk_perm(__brev(index), vx);
+    *x = u24(__brev(lk_perm(__brev(v), vy)));
+    *y = u24(__brev(lk_perm(__brev(pascal(v)), vz)));
+}
+
+__device__ inline void tile_pixel(int* xi, int* yi)
+{
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    *xi = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    *yi = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+}
+
+__global__ void __launch_bounds__(kTraceBlock)
+    k_render_normal(uint32_t* pixels, int w, int h, mlbvh_raygen cam, const uint32_t* __restrict__ rootPtr,
+                    const Node* __restrict__ nodes, const Tri* __restrict__ tris)
+{
+    int xi, yi;
+    tile_pixel(&xi, &yi);
+    if (xi >= w || yi >= h) return;
+    f3 ro, rd;
+    shoot(cam, &ro, &rd, (float)xi / (float)w, (float)yi / (float)h);
+    Hit hit;
+    hit.t = kFltMax;
+    hit.ng = mk(0, 0, 0);
+    hit.tri = 0xFFFFFFFFu;
+    uint32_t a = 0, b = 0;
+    traverse<false, false>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), hit, a, b);
+    uint32_t px;
+    if (hit.t != kFltMax) {
+        f3 n = normalize(hit.ng);
+        f3 c = (n + mk(1.0f, 1.0f, 1.0f)) * 0.5f;
+        px = pack_rgba(c.x, c.y, c.z);
+    } else {
+        px = pack_rgba(0, 0, 0);
+    }
+    pixels[xi + yi * w] = px;
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kTraceBlock)
+    k_render_ao(uint32_t* pixels, float* aoOut, int w, int h, mlbvh_raygen cam, const uint32_t* __restrict__ rootPtr,
+                const Node* __restrict__ nodes, const Tri* __restrict__ tris, int useSobol, int nSample, float radius,
+                uint32_t seed, unsigned long long* counters)
+{
+    int xi, yi;
+    tile_pixel(&xi, &yi);
+    uint32_t nRays = 0, nFetch = 0, nTest = 0;
+    if (xi < w && yi < h) {
+        const uint32_t root = *rootPtr;
+        int pixel = xi + yi * w;
+        f3 ro, rd;
+        shoot(cam, &ro, &rd, (float)xi / (float)w, (float)yi / (float)h);
+        Hit hit;
+        hit.t = kFltMax;
+        hit.ng = mk(0, 0, 0);
+        hit.tri = 0xFFFFFFFFu;
+        traverse<false, kCount>(nodes, tris, root, ro, rd, inv_rd(rd), hit, nFetch, nTest);
+        nRays++;
+        if (hit.t == kFltMax) {
+            pixels[pixel] = pack_rgba(0, 0, 0);
+            if (aoOut) aoOut[pixel] = -1.0f;
+        } else {
+            f3 n = normalize(hit.ng);
+            if (0.0f < dot(n, rd)) n = -n;
+            f3 xaxis, zaxis;
+            onb(n, &xaxis, &zaxis);
+            f3 org = (ro + rd * hit.t) + n * 0.001f;
+            Pcg rng(hash_pcg((uint32_t)pixel), 0);
+            float ao = 0.0f;
+            const float step = 1.0f / (float)nSample;
+            for (int i = 0; i < nSample; i++) {
+                float r0, r1;
+                if (useSobol) {
+                    sobol2d(&r0, &r1, (uint32_t)i, (uint32_t)xi, (uint32_t)yi, seed);
+                } else {
+                    r0 = rng.uniformf();
+                    r1 = rng.uniformf();
+                }
+                f3 dl = hemisphere_cos(r0, r1);
+                f3 dir = ((xaxis * dl.x) + (zaxis * dl.z)) + (n * dl.y);
+                Hit h2;
+                h2.t = radius;
+                h2.ng = mk(0, 0, 0);
+                h2.tri = 0xFFFFFFFFu;
+                traverse<true, kCount>(nodes, tris, root, org, dir, inv_rd(dir), h2, nFetch, nTest);
+                nRays++;
+                if (h2.t != radius) ao += step;
+            }
+            float brightness = powf(1.0f - ao, 1.0f / 2.2f);
+            pixels[pixel] = pack_rgba(brightness, brightness, brightness);
+            if (aoOut) aoOut[pixel] = ao;
+        }
+    }
+    if (kCount) {
+        wave_add_u64(&counters[0], nRays);
+        wave_add_u64(&counters[1], nFetch);
+        wave_add_u64(&counters[2], nTest);
+    }
+}
+
+struct Attrib {
+    f3 sn[3];
+    f3 reflectance;
+    f3 emissive;
+};
+static_assert(sizeof(Attrib) == 60, "TriangleAttrib is 60 bytes");
+
+__global__ void __launch_bounds__(kTraceBlock)
+    k_render_pt(float4* acc, int w, int h, mlbvh_raygen cam, const uint32_t* __restrict__ rootPtr,
+                const Node* __restrict__ nodes, const Tri* __restrict__ tris, const Attrib* __restrict__ attribs,
+                int useSobol, int maxSPP, int maxDepth)
+{
+    int xi, yi;
+    tile_pixel(&xi, &yi);
+    if (xi >= w || yi >= h) return;
+    const uint32_t root = *rootPtr;
+    int pixel = xi + yi * w;
+    float4 a = acc[pixel];
+    int iteration = (int)a.w;
+    Pcg rng(hash_pcg(hash_pcg((uint32_t)pixel) + (uint32_t)iteration), 0);
+    uint32_t dim = 0;
+    auto next2 = [&](float* r0, float* r1) {
+        if (useSobol) {
+            sobol2d(r0, r1, (uint32_t)iteration, (uint32_t)xi, (uint32_t)yi, dim++);
+        } else {
+            *r0 = rng.uniformf();
+            *r1 = rng.uniformf();
+        }
+    };
+    float r0, r1;
+    next2(&r0, &r1);
+    f3 ro, rd;
+    shoot(cam, &ro, &rd, ((float)xi + r0) / (float)w, ((float)yi + r1) / (float)h);
+    f3 color = mk(0, 0, 0);
+    f3 tp = mk(1.0f, 1.0f, 1.0f);
+    Hit hit;
+    hit.t = kFltMax;
+    hit.ng = mk(0, 0, 0);
+    hit.tri = 0xFFFFFFFFu;
+    uint32_t c0 = 0, c1 = 0;
+    traverse<false, false>(nodes, tris, root, ro, rd, inv_rd(rd), hit, c0, c1);
+    if (hit.t != kFltMax) {
+        for (int depth = 0; depth < maxDepth; depth++) {
+            next2(&r0, &r1);
+            f3 hp = ro + rd * hit.t;
+            f3 hn = normalize(hit.ng);
+            if (0.0f < dot(hn, rd)) hn = -hn;
+            const Attrib at = attribs[hit.tri];
+            color = color + tp * at.emissive;
+            f3 xaxis, zaxis;
+            onb(hn, &xaxis, &zaxis);
+            f3 dl = hemisphere_cos(r0, r1);
+            f3 nro = hp + hn * 0.0001f;
+            f3 nrd = ((xaxis * dl.x) + (zaxis * dl.z)) + (hn * dl.y);
+            tp = tp * at.reflectance;
+            float maxComp = fmaxf(fmaxf(tp.x, tp.y), tp.z);
+            float p = fminf(maxComp / 0.25f, 1.0f);
+            float q0, q1;
+            next2(&q0, &q1);
+            if (p < q0) break;
+            tp = tp / p;
+            hit.t = kFltMax;
+            hit.u = hit.v = 0.0f;
+            hit.ng = mk(0, 0, 0);
+            hit.tri = 0xFFFFFFFFu;
+            traverse<false, false>(nodes, tris, root, nro, nrd, inv_rd(nrd), hit, c0, c1);
+            if (hit.t == kFltMax) break;
+            ro = nro;
+            rd = nrd;
+        }
+    }
+    if (iteration < maxSPP) {
+        a.x += color.x;
+        a.y += color.y;
+        a.z += color.z;
+        a.w += 1.0f;
+        acc[pixel] = a;
+    }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_pack(uint32_t* pixels, const float4* __restrict__ acc, int n)
+{
+    int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    float4 a = acc[i];
+    pixels[i] = pack_rgba(powf(a.x / a.w, 1.0f / 2.2f), powf(a.y / a.w, 1.0f / 2.2f), powf(a.z / a.w, 1.0f / 2.2f));
+}
+
+static int check_scene(mlbvh_ctx* ctx, const void* root, const void* nodes, const void* tris, const char* who)
+{
+    if (!ctx || !root || !nodes || !tris) {
+        set_error("%s: null argument", who);
+        return MLBVH_E_INVALID_ARG;
+    }
+    return set_device(ctx);
+}
+
+static dim3 image_grid(int w, int h) { return dim3((unsigned)((w + 15) / 16), (unsigned)((h + 15) / 16), 1); }
+
+}  // namespace mlbvh
+
+using namespace mlbvh;
+
+extern "C" int mlbvh_trace(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles, const void* d_root,
+                           const float* d_rays, void* d_hits, int nRays, int rayQuery, unsigned long long* d_counters,
+                           void* stream)
+{
+    int rc = check_scene(ctx, d_root, d_nodes, d_triangles, "trace");
+    if (rc) return rc;
+    if (nRays < 0 || (nRays > 0 && (!d_rays || !d_hits)) ||
+        (rayQuery != MLBVH_RAY_QUERY_CLOSEST && rayQuery != MLBVH_RAY_QUERY_ANY)) {
+        set_error("trace: invalid rays/hits/query");
+        return MLBVH_E_INVALID_ARG;
+    }
+    if (nRays == 0) return MLBVH_OK;
+    hipStream_t s = (hipStream_t)stream;
+    unsigned g = (unsigned)((nRays + kTraceBlock - 1) / kTraceBlock);
+    const Node* nodes = (const Node*)d_nodes;
+    const Tri* tris = (const Tri*)d_triangles;
+    const uint32_t* root = (const uint32_t*)d_root;
+    const float4* rays = (const float4*)d_rays;
+    HitRec* hits = (HitRec*)d_hits;
+    bool any = rayQuery == MLBVH_RAY_QUERY_ANY;
+    if (d_counters) {
+        if (any) k_trace<true, true><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
+        else k_trace<false, true><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
+    } else {
+        if (any) k_trace<true, false><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
+        else k_trace<false, false><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, "trace launch");
+}
+
+extern "C" int mlbvh_render_normal(mlbvh_ctx* ctx, uint32_t* d_pixels, int width, int height,
+                                   const mlbvh_raygen* camera, const void* d_root, const void* d_nodes,
+                                   const void* d_triangles, void* stream)
+{
+    int rc = check_scene(ctx, d_root, d_nodes, d_triangles, "render_normal");
+    if (rc) return rc;
+    if (!d_pixels || !camera || width <= 0 || height <= 0) {
+        set_error("render_normal: invalid image arguments");
+        return MLBVH_E_INVALID_ARG;
+    }
+    k_render_normal<<<image_grid(width, height), kTraceBlock, 0, (hipStream_t)stream>>>(
+        d_pixels, width, height, *camera, (const uint32_t*)d_root, (const Node*)d_nodes, (const Tri*)d_triangles);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, "render_normal launch");
+}
+
+extern "C" int mlbvh_render_ao(mlbvh_ctx* ctx, uint32_t* d_pixels, float* d_ao, int width, int height,
+                               const mlbvh_raygen* camera, const void* d_root, const void* d_nodes,
+                               const void* d_triangles, int useSobol, int nSample, float radius, uint32_t seed,
+                               unsigned long long* d_counters, void* stream)
+{
+    int rc = check_scene(ctx, d_root, d_nodes, d_triangles, "render_ao");
+    if (rc) return rc;
+    if (!d_pixels || !camera || width <= 0 || height <= 0 || nSample < 0) {
+        set_error("render_ao: invalid image arguments");
+        return MLBVH_E_INVALID_ARG;
+    }
+    dim3 g = image_grid(width, height);
+    hipStream_t s = (hipStream_t)stream;
+    if (d_counters)
+        k_render_ao<true><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, (const uint32_t*)d_root,
+                                                   (const Node*)d_nodes, (const Tri*)d_triangles, useSobol, nSample,
+                                                   radius, seed, d_counters);
+    else
+        k_render_ao<false><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, (const uint32_t*)d_root,
+                                                    (const Node*)d_nodes, (const Tri*)d_triangles, useSobol, nSample,
+                                                    radius, seed, nullptr);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, "render_ao launch");
+}
+
+extern "C" int mlbvh_render_pt(mlbvh_ctx* ctx, float* d_accumulators, int width, int height,
+                               const mlbvh_raygen* camera, const void* d_root, const void* d_nodes,
+                               const void* d_triangles, const void* d_attribs, int useSobol, int maxSPP, int maxDepth,
+                               void* stream)
+{
+    int rc = check_scene(ctx, d_root, d_nodes, d_triangles, "render_pt");
+    if (rc) return rc;
+    if (!d_accumulators || !d_attribs || !camera || width <= 0 || height <= 0 || maxDepth < 0) {
+        set_error("render_pt: invalid arguments");
+        return MLBVH_E_INVALID_ARG;
+    }
+    k_render_pt<<<image_grid(width, height), kTraceBlock, 0, (hipStream_t)stream>>>(
+        (float4*)d_accumulators, width, height, *camera, (const uint32_t*)d_root, (const Node*)d_nodes,
+        (const Tri*)d_triangles, (const Attrib*)d_attribs, useSobol, maxSPP, maxDepth);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, "render_pt launch");
+}
+
+extern "C" int mlbvh_pack(mlbvh_ctx* ctx, uint32_t* d_pixels, const float* d_accumulators, int n, void* stream)
+{
+    if (!ctx || n < 0 || (n > 0 && (!d_pixels || !d_accumulators))) {
+        set_error("pack: invalid arguments");
+        return MLBVH_E_INVALID_ARG;
+    }
+    if (n == 0) return MLBVH_OK;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    k_pack<<<(unsigned)((n + kTraceBlock - 1) / kTraceBlock), kTraceBlock, 0, (hipStream_t)stream>>>(
+        d_pixels, (const float4*)d_accumulators, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, "pack launch");
+}

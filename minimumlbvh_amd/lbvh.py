@@ -1,0 +1,208 @@
+"""Python host API over libmlbvh, mirroring the reference's C++ classes.
+
+Names follow Ushio/MinimumLBVH: ``OnesweepSort`` (tinyhiponesweep.h:572-748),
+``BVHGPUBuilder`` (minimum_lbvh.h:726-885) with public ``m_internals`` /
+``m_rootNode`` device buffers, ``RayGenerator`` (camera.h), and the render
+kernels of main_gpu.cu.  Device memory and streams come from PyTorch-ROCm
+tensors; all compute runs in the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Optional
+
+import torch
+
+from . import _capi
+from ._capi import RAY_QUERY_ANY, RAY_QUERY_CLOSEST, RayGen, check
+
+NODE_BYTES = 64
+TRI_BYTES = 36
+HIT_BYTES = 32
+INVALID_NODE = 0x7FFFFFFF
+LEAF_BIT = 0x80000000
+
+_contexts: dict[int, "Context"] = {}
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream) -> ctypes.c_void_p:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    if isinstance(stream, torch.cuda.Stream):
+        return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+class Context:
+    """Per-device library context (sorter + builder scratch)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _capi.load()
+        self.device = device
+        h = ctypes.c_void_p()
+        check(self.lib.mlbvh_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            check(self.lib.mlbvh_ctx_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def context(device: int = 0) -> Context:
+    if device not in _contexts:
+        _contexts[device] = Context(device)
+    return _contexts[device]
+
+
+class OnesweepSort:
+    """tinyhiponesweep::OnesweepSort: stable LSD radix sort on the GPU."""
+
+    def __init__(self, device: int = 0):
+        self.ctx = context(device)
+
+    def sort(self, keys: torch.Tensor, values: Optional[torch.Tensor], keys_tmp: torch.Tensor,
+             values_tmp: Optional[torch.Tensor], n: int, startBit: int, endBit: int, stream=None) -> None:
+        lib = self.ctx.lib
+        es = keys.element_size()
+        fn = {4: lib.mlbvh_sort_u32, 8: lib.mlbvh_sort_u64}.get(es)
+        if fn is None:
+            raise ValueError("keys must be 32- or 64-bit")
+        if keys_tmp.element_size() != es or keys.numel() < n or keys_tmp.numel() < n:
+            raise ValueError("keys / keys_tmp size mismatch")
+        if values is not None and (values.element_size() != 4 or values.numel() < n or values_tmp is None
+                                   or values_tmp.numel() < n):
+            raise ValueError("values must be 32-bit with a temp of the same size")
+        check(fn(self.ctx.handle, _ptr(keys), _ptr(values), _ptr(keys_tmp), _ptr(values_tmp), n, startBit, endBit,
+                 _stream(stream)))
+
+
+class BVHGPUBuilder:
+    """minimum_lbvh::BVHGPUBuilder.  ``kernel``/``includeDir`` are accepted for
+    signature compatibility; the kernels are compiled ahead of time."""
+
+    def __init__(self, kernel: Optional[str] = None, includeDir: Optional[str] = None, device: int = 0):
+        self.ctx = context(device)
+        self.device = device
+        self.m_internals: Optional[torch.Tensor] = None  # (n-1, 16) int32 = InternalNode[n-1]
+        self.m_rootNode = torch.zeros(1, dtype=torch.int32, device=f"cuda:{device}")
+        self.m_nTriangles = 0
+        self.m_sortedRecords: Optional[torch.Tensor] = None
+        self.m_deltas: Optional[torch.Tensor] = None
+
+    def build(self, triangles: torch.Tensor, nTriangles: int, buildOption: int = 0,
+              sorter: Optional[OnesweepSort] = None, stream=None, keep_topology: bool = False) -> None:
+        if nTriangles < 2:
+            raise ValueError("at least 2 triangles are required")
+        if triangles.dtype != torch.float32 or triangles.numel() < 9 * nTriangles:
+            raise ValueError("triangles must be float32 with 9 values per triangle")
+        dev = triangles.device
+        if self.m_internals is None or self.m_internals.shape[0] != nTriangles - 1:
+            self.m_internals = torch.empty((nTriangles - 1, 16), dtype=torch.int32, device=dev)
+        self.m_nTriangles = nTriangles
+        lib = self.ctx.lib
+        if keep_topology:
+            if self.m_sortedRecords is None or self.m_sortedRecords.numel() != nTriangles:
+                self.m_sortedRecords = torch.empty(nTriangles, dtype=torch.int64, device=dev)
+                self.m_deltas = torch.empty(nTriangles - 1, dtype=torch.uint8, device=dev)
+            check(lib.mlbvh_build_ex(self.ctx.handle, _ptr(triangles), nTriangles, buildOption,
+                                     _ptr(self.m_internals), _ptr(self.m_rootNode), _ptr(self.m_sortedRecords),
+                                     _ptr(self.m_deltas), _stream(stream)))
+        else:
+            check(lib.mlbvh_build(self.ctx.handle, _ptr(triangles), nTriangles, buildOption, _ptr(self.m_internals),
+                                  _ptr(self.m_rootNode), _stream(stream)))
+
+    def refit(self, triangles: torch.Tensor, stream=None) -> None:
+        if self.m_sortedRecords is None:
+            raise RuntimeError("refit needs a build(..., keep_topology=True) first")
+        check(self.ctx.lib.mlbvh_refit(self.ctx.handle, _ptr(triangles), self.m_nTriangles,
+                                       _ptr(self.m_sortedRecords), _ptr(self.m_deltas), _ptr(self.m_internals),
+                                       _ptr(self.m_rootNode), _stream(stream)))
+
+    def empty(self) -> bool:
+        return self.m_nTriangles == 0
+
+
+class RayGenerator:
+    """camera.h RayGenerator; lookat is evaluated by the library on the host."""
+
+    def __init__(self):
+        self.g = RayGen((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+
+    def lookat(self, eye, center, up, fovy: float, imageWidth: int, imageHeight: int) -> "RayGenerator":
+        f3 = ctypes.c_float * 3
+        check(_capi.load().mlbvh_raygen_lookat(ctypes.byref(self.g), f3(*eye), f3(*center), f3(*up), fovy,
+                                               imageWidth, imageHeight))
+        return self
+
+    def as_tuple(self):
+        return tuple(self.g.origin), tuple(self.g.right), tuple(self.g.up)
+
+
+def trace(builder_or_nodes, triangles: torch.Tensor, rays: torch.Tensor, query: int = RAY_QUERY_CLOSEST,
+          root: Optional[torch.Tensor] = None, counters: Optional[torch.Tensor] = None, stream=None,
+          device: int = 0) -> torch.Tensor:
+    """Trace rays (n, 8) float32 [ox,oy,oz,tmax,dx,dy,dz,pad]; returns hits as an
+    (n, 8) int32 tensor holding the 32-byte Hit records."""
+    if isinstance(builder_or_nodes, BVHGPUBuilder):
+        nodes, root = builder_or_nodes.m_internals, builder_or_nodes.m_rootNode
+    else:
+        nodes = builder_or_nodes
+    n = rays.shape[0]
+    hits = torch.empty((n, 8), dtype=torch.int32, device=rays.device)
+    ctx = context(device)
+    check(ctx.lib.mlbvh_trace(ctx.handle, _ptr(nodes), _ptr(triangles), _ptr(root), _ptr(rays), _ptr(hits), n, query,
+                              _ptr(counters), _stream(stream)))
+    return hits
+
+
+def render_normal(pixels: torch.Tensor, width: int, height: int, rayGenerator: RayGenerator, rootNode, internals,
+                  triangles, stream=None, device: int = 0) -> None:
+    ctx = context(device)
+    check(ctx.lib.mlbvh_render_normal(ctx.handle, _ptr(pixels), width, height, ctypes.byref(rayGenerator.g),
+                                      _ptr(rootNode), _ptr(internals), _ptr(triangles), _stream(stream)))
+
+
+def render_ao(pixels: torch.Tensor, width: int, height: int, rayGenerator: RayGenerator, rootNode, internals,
+              triangles, useSobol: int = 1, nSample: int = 16, radius: float = 1.0, seed: int = 12345,
+              ao: Optional[torch.Tensor] = None, counters: Optional[torch.Tensor] = None, stream=None,
+              device: int = 0) -> None:
+    ctx = context(device)
+    check(ctx.lib.mlbvh_render_ao(ctx.handle, _ptr(pixels), _ptr(ao), width, height, ctypes.byref(rayGenerator.g),
+                                  _ptr(rootNode), _ptr(internals), _ptr(triangles), useSobol, nSample, radius,
+                                  seed & 0xFFFFFFFF, _ptr(counters), _stream(stream)))
+
+
+def render_pt(accumulators: torch.Tensor, width: int, height: int, rayGenerator: RayGenerator, rootNode, internals,
+              triangles, attribs, useSobol: int = 1, maxSPP: int = 1024, maxDepth: int = 1024, stream=None,
+              device: int = 0) -> None:
+    ctx = context(device)
+    check(ctx.lib.mlbvh_render_pt(ctx.handle, _ptr(accumulators), width, height, ctypes.byref(rayGenerator.g),
+                                  _ptr(rootNode), _ptr(internals), _ptr(triangles), _ptr(attribs), useSobol, maxSPP,
+                                  maxDepth, _stream(stream)))
+
+
+def pack(pixels: torch.Tensor, accumulators: torch.Tensor, n: int, stream=None, device: int = 0) -> None:
+    ctx = context(device)
+    check(ctx.lib.mlbvh_pack(ctx.handle, _ptr(pixels), _ptr(accumulators), n, _stream(stream)))
+
+
+def deg2rad(d: float) -> float:
+    return d * math.pi / 180.0

@@ -1,0 +1,90 @@
+"""GPU LBVH build (mlbvh_build) vs the oracle: node arrays and root
+bit-identical, plus the Cartesian-tree characterisation at full size."""
+import numpy as np
+import pytest
+import torch
+
+from minimumlbvh_amd import scenes
+from tests import lbvh_checks as C
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_build(gpu, tris, opt=0, keep=False):
+    t = torch.from_numpy(np.ascontiguousarray(tris, np.float32)).cuda()
+    b = gpu.BVHGPUBuilder()
+    b.build(t, tris.shape[0], opt, gpu.OnesweepSort(), keep_topology=keep)
+    torch.cuda.synchronize()
+    nodes = b.m_internals.cpu().numpy().view(np.uint32)
+    root = int(b.m_rootNode.cpu().numpy().view(np.uint32)[0])
+    return nodes, root, b, t
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 100, 257, 4096, 65536, 65537, 300_001])
+def test_build_bit_exact(gpu, orc, n):
+    print(f"build n={n}", flush=True)
+    tris = scenes.random_triangles(n, 1000 + n)
+    gn, groot, _, _ = gpu_build(gpu, tris)
+    on, oroot, _, _ = orc.build(tris)
+    assert groot == oroot
+    assert np.array_equal(gn, on)
+
+
+def test_build_use_normal_bit_exact(gpu, orc):
+    tris = scenes.random_triangles(50_000, 5)
+    gn, groot, _, _ = gpu_build(gpu, tris, opt=1)
+    on, oroot, _, _ = orc.build(tris, opt=1)
+    assert groot == oroot and np.array_equal(gn, on)
+
+
+@pytest.mark.parametrize("case", ["all_equal", "flat_axis", "duplicates", "tiny_extent"])
+def test_build_degenerate(gpu, orc, case):
+    if case == "all_equal":
+        tris = np.tile(scenes.random_triangles(1, 1), (5000, 1, 1))
+    elif case == "flat_axis":
+        tris = scenes.random_triangles(20000, 2)
+        tris[:, :, 1] = 0.5
+    elif case == "duplicates":
+        tris = scenes.random_triangles(20000, 3)
+        tris[::2] = tris[7]
+    else:
+        tris = (scenes.random_triangles(20000, 4) * np.float32(1e-30)).astype(np.float32)
+    gn, groot, _, _ = gpu_build(gpu, tris)
+    on, oroot, _, _ = orc.build(tris)
+    assert groot == oroot and np.array_equal(gn, on)
+
+
+def test_build_deterministic_over_repeats(gpu):
+    tris = scenes.random_triangles(200_000, 9)
+    first, r0, _, _ = gpu_build(gpu, tris)
+    for _ in range(5):
+        again, r1, _, _ = gpu_build(gpu, tris)
+        assert r1 == r0 and np.array_equal(again, first)
+
+
+def test_build_icosphere_level9_full_size(gpu, orc):
+    """5,242,880 triangles (the scale workload): bit-exact vs the oracle and the
+    independent characterisation."""
+    v, f = scenes.displaced_icosphere(9, seed=0)
+    tris = scenes.deindex(v, f)
+    gn, groot, _, _ = gpu_build(gpu, tris)
+    on, oroot, rec, d = orc.build(tris)
+    assert groot == oroot
+    assert np.array_equal(gn, on)
+    C.structural_checks(gn, groot, tris.shape[0])
+    assert orc.validate(groot, gn, d)
+
+
+def test_refit_matches_climb_on_moved_vertices(gpu, orc):
+    tris = scenes.random_triangles(30_000, 21)
+    _, _, b, t = gpu_build(gpu, tris, keep=True)
+    order = (b.m_sortedRecords.cpu().numpy().view(np.uint64) >> np.uint64(32)).astype(np.uint32)
+    deltas = b.m_deltas.cpu().numpy()
+    moved = tris + np.random.default_rng(2).normal(0, 0.003, tris.shape).astype(np.float32)
+    t.copy_(torch.from_numpy(moved).cuda())
+    b.refit(t)
+    torch.cuda.synchronize()
+    gn = b.m_internals.cpu().numpy().view(np.uint32)
+    groot = int(b.m_rootNode.cpu().numpy().view(np.uint32)[0])
+    en, eroot = orc.build_from_deltas(deltas, tri_index=order, tris=moved)
+    assert groot == eroot and np.array_equal(gn, en)

@@ -1,0 +1,100 @@
+"""GPU Onesweep sort (mlbvh_sort_u32/u64) vs the oracle's stable LSD radix
+sort: bit-exact keys and values."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    a = np.ascontiguousarray(a)
+    view = {np.dtype(np.uint64): np.int64, np.dtype(np.uint32): np.int32}.get(a.dtype, a.dtype)
+    return torch.from_numpy(a.view(view)).cuda()
+
+
+def _host(t, dtype):
+    return t.cpu().numpy().view(dtype)
+
+
+def _run(gpu, keys, vals, start, end):
+    n = keys.shape[0]
+    k = _dev(keys)
+    kt = torch.empty_like(k)
+    v = _dev(vals) if vals is not None else None
+    vt = torch.empty_like(v) if v is not None else None
+    gpu.OnesweepSort().sort(k, v, kt, vt, n, start, end)
+    torch.cuda.synchronize()
+    return _host(k, keys.dtype), (_host(v, np.uint32) if v is not None else None)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 100, 4095, 4096, 4097, 65537, (1 << 20) + 3])
+@pytest.mark.parametrize("dtype,bits", [(np.uint32, 32), (np.uint64, 64)])
+def test_sort_full_range(gpu, orc, n, dtype, bits):
+    print(f"sort n={n} bits={bits}", flush=True)
+    r = np.random.default_rng(n + bits)
+    keys = r.integers(0, np.iinfo(dtype).max, size=n, dtype=dtype, endpoint=True)
+    vals = np.arange(n, dtype=np.uint32)
+    gk, gv = _run(gpu, keys, vals, 0, bits)
+    ok, ov, _ = orc.radix_sort(keys, vals, 0, bits)
+    assert np.array_equal(gk, ok)
+    assert np.array_equal(gv, ov)
+
+
+@pytest.mark.parametrize("rng_", [(0, 30), (8, 40), (4, 12), (24, 32), (13, 14), (0, 8)])
+@pytest.mark.parametrize("dtype,bits", [(np.uint32, 32), (np.uint64, 64)])
+@pytest.mark.parametrize("with_values", [False, True])
+def test_sort_bit_ranges(gpu, orc, rng_, dtype, bits, with_values):
+    start, end = rng_
+    print(f"sort range {start}-{end} bits={bits} values={with_values}", flush=True)
+    n = 300_007
+    r = np.random.default_rng(start * 1000 + end)
+    keys = r.integers(0, np.iinfo(dtype).max, size=n, dtype=dtype, endpoint=True)
+    keys[::5] = keys[1]  # ties must keep input order
+    vals = r.integers(0, 2**32, size=n, dtype=np.uint32) if with_values else None
+    gk, gv = _run(gpu, keys, vals, start, end)
+    ok, ov, _ = orc.radix_sort(keys, vals, start, end)
+    assert np.array_equal(gk, ok)
+    if with_values:
+        assert np.array_equal(gv, ov)
+
+
+@pytest.mark.parametrize("pattern", ["equal", "sorted", "reversed", "two"])
+def test_sort_patterns(gpu, orc, pattern):
+    n = 200_000
+    if pattern == "equal":
+        keys = np.full(n, 0xDEADBEEFCAFE, np.uint64)
+    elif pattern == "sorted":
+        keys = np.arange(n, dtype=np.uint64) * 977
+    elif pattern == "reversed":
+        keys = (np.arange(n, dtype=np.uint64) * 977)[::-1].copy()
+    else:
+        keys = (np.arange(n, dtype=np.uint64) % 2) << np.uint64(40)
+    vals = np.arange(n, dtype=np.uint32)
+    gk, gv = _run(gpu, keys, vals, 0, 64)
+    ok, ov, _ = orc.radix_sort(keys, vals, 0, 64)
+    assert np.array_equal(gk, ok) and np.array_equal(gv, ov)
+
+
+def test_sort_repeated_calls_and_shrinking_sizes(gpu, orc):
+    """Look-back words are tagged per pass; back-to-back sorts of different
+    sizes on one context must not see each other's state."""
+    for n in [500_000, 70_000, 4097, 1_000_003, 3]:
+        keys = np.random.default_rng(n).integers(0, 2**64 - 1, size=n, dtype=np.uint64)
+        gk, _ = _run(gpu, keys, None, 0, 64)
+        assert np.array_equal(gk, np.sort(keys))
+
+
+def test_sort_32m_u64_properties(gpu):
+    """Full-size standalone sort (2^25 uniform u64 keys): sorted, and a
+    permutation of the input (checksums of the multiset)."""
+    n = 1 << 25
+    g = torch.Generator(device="cuda").manual_seed(0)
+    k = torch.randint(-(2**63), 2**63 - 1, (n,), dtype=torch.int64, device="cuda", generator=g)
+    before = k.cpu().numpy().view(np.uint64).copy()
+    kt = torch.empty_like(k)
+    gpu.OnesweepSort().sort(k, None, kt, None, n, 0, 64)
+    torch.cuda.synchronize()
+    after = k.cpu().numpy().view(np.uint64)
+    assert np.all(after[1:] >= after[:-1])
+    assert np.array_equal(after, np.sort(before))

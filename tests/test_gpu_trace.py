@@ -1,0 +1,167 @@
+"""GPU stack-free traversal and render kernels vs the oracle.
+
+Bar: Hit records (t, uv, ng, triangleIndex) bit-identical to the oracle;
+node-fetch / triangle-test counters identical (the traversal order is the
+same).  Normal images bit-identical.  AO: primary hit/miss identical on every
+pixel; the occlusion fraction may differ only where an AO direction, which
+goes through the device cosf/sinf instead of the host libm ones, grazes an
+edge -- at most 0.5% of hit pixels and by at most 1/N each.  PT: mean radiance
+within 2% and 98% of pixels within 5% (paths fork where a bounce direction
+differs in the last ulp).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from minimumlbvh_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def random_rays(n, seed, tmax=None, lo=-0.5, hi=1.5):
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = rng.uniform(lo, hi, (n, 3))
+    r[:, 4:7] = rng.uniform(0.0, 1.0, (n, 3)) - r[:, 0:3]
+    r[:, 3] = np.float32(3.402823466e38) if tmax is None else tmax
+    r[:16, 4:7] = 0
+    r[:16, 4 + (np.arange(16) % 3)] = np.where(np.arange(16) % 2, 1.0, -1.0)
+    return r
+
+
+def upload_bvh(nodes, root):
+    return (torch.from_numpy(np.ascontiguousarray(nodes).view(np.int32)).cuda(),
+            torch.tensor([root], dtype=torch.int64).to(torch.int32).cuda() if root < 2**31 else
+            torch.from_numpy(np.array([root], np.uint32).view(np.int32)).cuda())
+
+
+def gpu_trace(gpu, nodes_t, root_t, tris_t, rays, query):
+    r = torch.from_numpy(rays).cuda()
+    cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+    hits = gpu.trace(nodes_t, tris_t, r, query=query, root=root_t, counters=cnt)
+    torch.cuda.synchronize()
+    return hits.cpu().numpy().view(np.uint32).reshape(-1, 8), cnt.cpu().numpy().view(np.uint64)
+
+
+@pytest.mark.parametrize("query", [0, 1])
+@pytest.mark.parametrize("n", [2, 1000, 65536])
+def test_trace_bit_exact_on_oracle_bvh(gpu, orc, n, query):
+    print(f"trace n={n} query={query}", flush=True)
+    tris = scenes.random_triangles(n, 7 + n, sigma=0.02)
+    nodes, root, _, _ = orc.build(tris)
+    rays = random_rays(50_000, n, tmax=None if query == 0 else 0.25)
+    oh, oc = orc.trace(nodes, tris, root, rays, query=query)
+    nt, rt = upload_bvh(nodes, root)
+    gh, gc = gpu_trace(gpu, nt, rt, torch.from_numpy(tris).cuda(), rays, query)
+    assert np.array_equal(gh, oh.view(np.uint32).reshape(-1, 8))
+    assert np.array_equal(gc, oc)
+
+
+def test_trace_on_gpu_built_bvh(gpu, orc):
+    tris = scenes.random_triangles(100_000, 3, sigma=0.01)
+    t = torch.from_numpy(tris).cuda()
+    b = gpu.BVHGPUBuilder()
+    b.build(t, tris.shape[0], 0, gpu.OnesweepSort())
+    rays = random_rays(100_000, 4)
+    cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+    gh = gpu.trace(b, t, torch.from_numpy(rays).cuda(), counters=cnt)
+    torch.cuda.synchronize()
+    nodes, root, _, _ = orc.build(tris)
+    oh, oc = orc.trace(nodes, tris, root, rays)
+    assert np.array_equal(gh.cpu().numpy().view(np.uint32).reshape(-1, 8), oh.view(np.uint32).reshape(-1, 8))
+    assert np.array_equal(cnt.cpu().numpy().view(np.uint64), oc)
+
+
+def small_scene(orc):
+    tris = scenes.random_triangles(65536, 0)
+    nodes, root, _, _ = orc.build(tris)
+    eye, ctr, up, fov = scenes.CAMERA_SMALL
+    return tris, nodes, root, (eye, ctr, up, math.radians(fov))
+
+
+def test_render_normal_bit_exact(gpu, orc):
+    tris, nodes, root, cam = small_scene(orc)
+    w, h = 256, 256
+    g = gpu.RayGenerator().lookat(*cam, w, h)
+    og = orc.raygen_lookat(*cam, w, h)
+    ref = orc.render_normal(og, w, h, root, nodes, tris)
+    nt, rt = upload_bvh(nodes, root)
+    px = torch.zeros(w * h, dtype=torch.int32, device="cuda")
+    gpu.render_normal(px, w, h, g, rt, nt, torch.from_numpy(tris).cuda())
+    torch.cuda.synchronize()
+    got = px.cpu().numpy().view(np.uint32)
+    assert (ref != 0).sum() > 1000
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("w,h,nsample,radius,seed", [(256, 256, 4, 0.1, 0), (200, 150, 16, 1.0, 12345),
+                                                     (67, 45, 3, 0.05, 7)])
+def test_render_ao_matches_oracle(gpu, orc, w, h, nsample, radius, seed):
+    tris, nodes, root, cam = small_scene(orc)
+    g = gpu.RayGenerator().lookat(*cam, w, h)
+    og = orc.raygen_lookat(*cam, w, h)
+    rpx, rao, rcnt = orc.render_ao(og, w, h, root, nodes, tris, 1, nsample, radius, seed)
+    nt, rt = upload_bvh(nodes, root)
+    px = torch.zeros(w * h, dtype=torch.int32, device="cuda")
+    ao = torch.zeros(w * h, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(3, dtype=torch.int64, device="cuda")
+    gpu.render_ao(px, w, h, g, rt, nt, torch.from_numpy(tris).cuda(), 1, nsample, radius, seed, ao=ao, counters=cnt)
+    torch.cuda.synchronize()
+    gao = ao.cpu().numpy()
+    gpx = px.cpu().numpy().view(np.uint32)
+    miss = rao < 0
+    assert np.array_equal(gao < 0, miss), "primary hit/miss differs"
+    assert np.array_equal(gpx[miss], rpx[miss])
+    hit = ~miss
+    assert hit.sum() > 100
+    diff = np.abs(gao[hit] - rao[hit])
+    assert diff.max() <= 1.0 / nsample + 1e-6
+    assert (diff > 0).mean() <= 0.005, f"{(diff > 0).sum()} of {hit.sum()} AO pixels differ"
+    same = hit.copy()
+    same[hit] = diff == 0
+    # where the occlusion matches, the pixel colour (powf + packRGBA) matches
+    # within one 8-bit step
+    dpx = np.abs((gpx[same] & 0xFF).astype(int) - (rpx[same] & 0xFF).astype(int))
+    assert dpx.max() <= 1
+    gc = cnt.cpu().numpy().view(np.uint64)
+    assert gc[0] == rcnt[0] or (diff > 0).any()
+
+
+def test_pt_and_pack_match_oracle_statistically(gpu, orc):
+    tris = scenes.random_triangles(20000, 5, sigma=0.03)
+    n = tris.shape[0]
+    attribs = np.zeros((n, 15), np.float32)
+    rng = np.random.default_rng(1)
+    attribs[:, 9:12] = rng.uniform(0.2, 0.8, (n, 3))
+    attribs[rng.random(n) < 0.05, 12:15] = 4.0
+    nodes, root, _, _ = orc.build(tris)
+    eye, ctr, up, fov = scenes.CAMERA_SMALL
+    w, h, spp = 96, 96, 4
+    og = orc.raygen_lookat(eye, ctr, up, math.radians(fov), w, h)
+    acc = np.zeros((w * h, 4), np.float32)
+    for _ in range(spp):
+        acc = orc.render_pt(acc, og, w, h, root, nodes, tris, attribs, 1, 1024, 4)
+    g = gpu.RayGenerator().lookat(eye, ctr, up, math.radians(fov), w, h)
+    nt, rt = upload_bvh(nodes, root)
+    gacc = torch.zeros((w * h, 4), dtype=torch.float32, device="cuda")
+    tt = torch.from_numpy(tris).cuda()
+    at = torch.from_numpy(attribs).cuda()
+    for _ in range(spp):
+        gpu.render_pt(gacc, w, h, g, rt, nt, tt, at, 1, 1024, 4)
+    px = torch.zeros(w * h, dtype=torch.int32, device="cuda")
+    gpu.pack(px, gacc, w * h)
+    torch.cuda.synchronize()
+    ga = gacc.cpu().numpy()
+    assert np.array_equal(ga[:, 3], acc[:, 3])  # samples per pixel
+    m_ref, m_gpu = acc[:, :3].mean(), ga[:, :3].mean()
+    assert m_ref > 0
+    assert abs(m_gpu - m_ref) <= 0.02 * m_ref
+    close = np.all(np.abs(ga[:, :3] - acc[:, :3]) <= 0.05 * np.maximum(np.abs(acc[:, :3]), 1e-3), axis=1)
+    assert close.mean() >= 0.98
+    # pack of identical accumulators is bit-identical up to powf rounding
+    ref_px = orc.pack(ga)
+    got_px = px.cpu().numpy().view(np.uint32)
+    d = np.abs((got_px & 0xFF).astype(int) - (ref_px & 0xFF).astype(int))
+    assert d.max() <= 1
