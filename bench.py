@@ -80,9 +80,12 @@ def main():
     import numpy as np
     import torch
 
-    from minimumlbvh_amd import lbvh, scenes
+    from minimumlbvh_amd import autotune, lbvh, scenes
 
     dev = torch.device("cuda:0")
+    log("[bench] autotuning the sort tile")
+    best_cfg, cfg_ms = autotune.tune_sort(1 << 24, 64)
+    log(f"[bench] sort tile ms by config: {cfg_ms} -> {best_cfg}")
     W, H, SPP, RADIUS, SEED = 1920, 1080, 16, 0.2, 0
     log(f"[bench] generating icosphere level {args.level}")
     verts, faces = scenes.displaced_icosphere(args.level, seed=0)
@@ -196,7 +199,9 @@ def main():
             "algorithmic_bytes": sort_bytes,
             "achieved_gbs": round(sort_gbs, 1),
             "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
+            "tile": "x".join(str(v) for v in autotune.SORT_CONFIGS[best_cfg]),
         },
+        "tuning": autotune.get_tuning(),
     }
 
     if not args.no_cpu:

@@ -118,6 +118,47 @@ MLBVH_API int mlbvh_trace(mlbvh_ctx* ctx, const void* d_nodes, const void* d_tri
                 const float* d_rays, void* d_hits, int nRays, int rayQuery, unsigned long long* d_counters,
                 void* stream);
 
+/* Batched build of many meshes in one pipeline.  No reference counterpart:
+ * the reference builds one mesh per BVHGPUBuilder::build call
+ * (minimum_lbvh.h:783-871).  Meshes are concatenated in d_triangles; mesh m
+ * owns triangles [off[m], off[m+1]) of the device array d_triOffsets
+ * (nMeshes+1 entries, off[0] = 0, every mesh >= 2 triangles); the caller
+ * states totalTriangles (= off[nMeshes]) and an upper bound on the triangles
+ * of one mesh.  Mesh m gets exactly the BVH mlbvh_build would give it alone,
+ * with mesh-local indices, in the n_m - 1 nodes at d_nodes + (off[m] - m);
+ * its root goes to d_roots[m].  Needs ceil(log2(maxTrianglesPerMesh)) + 32 +
+ * ceil(log2(nMeshes)) <= 64. */
+MLBVH_API int mlbvh_build_batched(mlbvh_ctx* ctx, const void* d_triangles, const uint32_t* d_triOffsets, int nMeshes,
+                                  uint32_t totalTriangles, uint32_t maxTrianglesPerMesh, uint32_t buildOption,
+                                  void* d_nodes, uint32_t* d_roots, void* stream);
+
+/* Persistent work-queue trace over a batched build: each Ray names its mesh
+ * in its pad word (u32); hits hold mesh-local triangle indices.  Same
+ * traversal as mlbvh_trace (minimum_lbvh.h:981-1061).  d_counters as in
+ * mlbvh_trace. */
+MLBVH_API int mlbvh_trace_batched(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,
+                                  const uint32_t* d_triOffsets, const uint32_t* d_roots, int nMeshes,
+                                  const float* d_rays, void* d_hits, int nRays, int rayQuery,
+                                  unsigned long long* d_counters, void* stream);
+
+/* The plain per-mesh path over the same batch, for comparison: one
+ * mlbvh_build and one mlbvh_trace per mesh, launched asynchronously.  Offsets
+ * are host arrays here (tri offsets nMeshes+1, ray offsets nMeshes+1); output
+ * layout as mlbvh_build_batched / mlbvh_trace_batched. */
+MLBVH_API int mlbvh_build_trace_per_mesh(mlbvh_ctx* ctx, const void* d_triangles, const uint32_t* h_triOffsets,
+                                         int nMeshes, uint32_t buildOption, void* d_nodes, uint32_t* d_roots,
+                                         const float* d_rays, const uint32_t* h_rayOffsets, void* d_hits,
+                                         int rayQuery, void* stream);
+
+/* Launch shapes chosen by the autotuner (minimumlbvh_amd/autotune.py):
+ * sortConfig selects the radix-sort tile (0: 256x16, 1: 256x32, 2: 512x16,
+ * 3: 512x24, 4: 1024x16 threads x keys per thread); the persistent trace runs
+ * CUs x traceBlocksPerCU blocks of 256 threads and each wave takes traceChunk
+ * rays (a multiple of 64) per work-queue pull.  Results do not depend on
+ * these settings. */
+MLBVH_API int mlbvh_set_tuning(mlbvh_ctx* ctx, int sortConfig, int traceBlocksPerCU, int traceChunk);
+MLBVH_API int mlbvh_get_tuning(mlbvh_ctx* ctx, int* sortConfig, int* traceBlocksPerCU, int* traceChunk);
+
 /* Host-side RayGenerator::lookat (camera.h:10-22); fovy in radians. */
 MLBVH_API int mlbvh_raygen_lookat(mlbvh_raygen* out, const float eye[3], const float center[3], const float up[3], float fovy,
                         int width, int height);

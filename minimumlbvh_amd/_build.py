@@ -74,7 +74,26 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_facade_demo(verbose)
     return LIB
+
+
+FACADE_SRC = os.path.join(ROOT, "tests", "facade", "facade_demo.hip")
+FACADE_BIN = os.path.join(ROOT, "tests", "facade", "_build", "facade_demo")
+
+
+def build_facade_demo(verbose: bool = False) -> str:
+    """The C++-facade test application (reference-style host code over
+    include/minimum_lbvh.h), linked against libmlbvh.so."""
+    os.makedirs(os.path.dirname(FACADE_BIN), exist_ok=True)
+    cmd = [hipcc(), *[f for f in HIPFLAGS if f != "-fvisibility=hidden"], FACADE_SRC, "-o", FACADE_BIN,
+           f"-L{LIBDIR}", "-lmlbvh", "-Wl,-rpath,$ORIGIN/../../../minimumlbvh_amd/_lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade demo build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {FACADE_BIN}")
+    return FACADE_BIN
 
 
 if __name__ == "__main__":

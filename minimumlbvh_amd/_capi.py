@@ -34,6 +34,11 @@ SIGNATURES = {
     "mlbvh_build_ex": ([_P, _P, c_int, c_uint32, _P, _P, _P, _P, _P], c_int),
     "mlbvh_refit": ([_P, _P, c_int, _P, _P, _P, _P, _P], c_int),
     "mlbvh_trace": ([_P, _P, _P, _P, _P, _P, c_int, c_int, _P, _P], c_int),
+    "mlbvh_build_batched": ([_P, _P, _P, c_int, c_uint32, c_uint32, c_uint32, _P, _P, _P], c_int),
+    "mlbvh_trace_batched": ([_P, _P, _P, _P, _P, c_int, _P, _P, c_int, c_int, _P, _P], c_int),
+    "mlbvh_build_trace_per_mesh": ([_P, _P, _P, c_int, c_uint32, _P, _P, _P, _P, _P, c_int, _P], c_int),
+    "mlbvh_set_tuning": ([_P, c_int, c_int, c_int], c_int),
+    "mlbvh_get_tuning": ([_P, _P, _P, _P], c_int),
     "mlbvh_raygen_lookat": ([_P, POINTER(c_float), POINTER(c_float), POINTER(c_float), c_float, c_int, c_int], c_int),
     "mlbvh_render_normal": ([_P, _P, c_int, c_int, _P, _P, _P, _P, _P], c_int),
     "mlbvh_render_ao": ([_P, _P, _P, c_int, c_int, _P, _P, _P, _P, c_int, c_int, c_float, c_uint32, _P, _P], c_int),

@@ -18,13 +18,14 @@ namespace mlbvh {
 
 constexpr int kBuildBlock = 256;
 
-__global__ void k_build_init(uint32_t* sceneBox, uint32_t* hist, uint32_t* tickets)
+// Scene-box slots plus the sort state of the 4-pass record sort.
+__global__ void k_build_init(uint32_t* sceneBox, uint32_t* ctrl, uint32_t* hist, uint64_t* status,
+                             size_t statusWords)
 {
     int t = threadIdx.x;
-    if (t < 3) sceneBox[t] = 0xFFFFFFFFu;      // atomicMin slots (order-mapped)
-    else if (t < 6) sceneBox[t] = 0u;          // atomicMax slots
-    for (int i = t; i < 4 * 256; i += blockDim.x) hist[i] = 0;
-    if (t < 16) tickets[t] = 0;
+    if (t < 3) sceneBox[t] = 0xFFFFFFFFu;  // atomicMin slots (order-mapped)
+    else if (t < 6) sceneBox[t] = 0u;      // atomicMax slots
+    sort_state_init(ctrl, 4, hist, 4, status, statusWords);
 }
 
 // Scene bounds over all vertices: per-thread running min/max, wave and block
@@ -64,17 +65,19 @@ __global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(const Tri* __restric
 
 // Morton record per triangle (minimum_lbvh.cu:68-92 / minimum_lbvh.h:534-553),
 // the 4 digit histograms of the 32-bit key, and the climb's arrival words.
+// Grid-stride over a few blocks per CU so that the histograms are merged into
+// global memory by few blocks (global atomics on 1024 shared counters are the
+// cost otherwise).
 __global__ void __launch_bounds__(kBuildBlock)
     k_morton(const Tri* __restrict__ tris, int n, const uint32_t* __restrict__ sceneBox, uint32_t opt,
-             uint64_t* __restrict__ rec, uint32_t* __restrict__ hist, uint32_t* __restrict__ flags)
+             uint64_t* __restrict__ rec, uint32_t* __restrict__ partials, uint32_t* __restrict__ flags)
 {
     __shared__ uint32_t s_h[4][256];
     for (int p = 0; p < 4; p++) s_h[p][threadIdx.x] = 0;
     __syncthreads();
     const f3 lo = mk(ord2f(sceneBox[0]), ord2f(sceneBox[1]), ord2f(sceneBox[2]));
     const f3 hi = mk(ord2f(sceneBox[3]), ord2f(sceneBox[4]), ord2f(sceneBox[5]));
-    int i = blockIdx.x * kBuildBlock + threadIdx.x;
-    if (i < n) {
+    for (int i = blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += gridDim.x * kBuildBlock) {
         Tri tri = tris[i];
         f3 center = ((tri.v[0] + tri.v[1]) + tri.v[2]) / 3.0f;
         uint32_t morton;
@@ -93,10 +96,7 @@ __global__ void __launch_bounds__(kBuildBlock)
     }
     __syncthreads();
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-        uint32_t c = s_h[p][threadIdx.x];
-        if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
-    }
+    for (int p = 0; p < 4; p++) partials[((size_t)blockIdx.x * 4 + p) * 256 + threadIdx.x] = s_h[p][threadIdx.x];
 }
 
 // delta(a, b) = 32 - clz(a ^ b) of neighbouring sorted keys
@@ -118,57 +118,19 @@ __global__ void __launch_bounds__(kBuildBlock) k_fill_u32(uint32_t* p, int n, ui
 }
 
 // Agglomerative climb, one thread per sorted leaf (build_lbvh,
-// minimum_lbvh.h:284-361).  At each level the thread writes its subtree into
-// the parent's child slot, then swaps its range bound into the parent's
-// arrival word with an agent-scope acq_rel exchange (the two children may sit
-// on different XCDs, whose L2s are not coherent).  The first arriver stops;
-// the second merges the range and the sibling's AABB and climbs on.
+// minimum_lbvh.h:284-361): see climb_leaf in mlbvh_device.h.
 __global__ void __launch_bounds__(kBuildBlock)
     k_climb(const Tri* __restrict__ tris, int n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
             Node* nodes, uint32_t* flags, uint32_t* root)
 {
     int i = blockIdx.x * kBuildBlock + threadIdx.x;
     if (i >= n) return;
-    const uint32_t nInternals = (uint32_t)(n - 1);
-    uint32_t triIndex = (uint32_t)(rec[i] >> 32);
-    Tri tri = tris[triIndex];
-    Box box;
-    box.lo = mk(kFltMax, kFltMax, kFltMax);
-    box.hi = mk(-kFltMax, -kFltMax, -kFltMax);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        box.lo = hmin3(box.lo, tri.v[k]);
-        box.hi = hmax3(box.hi, tri.v[k]);
-    }
-    uint32_t lo = (uint32_t)i, hi = (uint32_t)i;
-    uint32_t node = triIndex | kLeafBit;
-    while (hi - lo < nInternals) {
-        uint32_t dL = 0 < lo ? (uint32_t)deltas[lo - 1] : 0xFFFFFFFFu;
-        uint32_t dR = hi < nInternals ? (uint32_t)deltas[hi] : 0xFFFFFFFFu;
-        int goLeft = dL < dR ? 1 : 0;
-        uint32_t parent = goLeft ? lo - 1 : hi;
-        Node* p = nodes + parent;
-        p->child[goLeft] = node;
-        p->box[goLeft] = box;
-        if (!(node & kLeafBit)) nodes[node].parent = parent;
-        uint32_t idx = goLeft ? hi : lo;
-        idx = __hip_atomic_exchange(&flags[parent], idx, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (idx == 0xFFFFFFFFu) return;
-        lo = lo < idx ? lo : idx;
-        hi = hi < idx ? idx : hi;
-        p->context = hi < nInternals ? hi : 0xFFFFFFFFu;
-        node = parent;
-        Box other = p->box[goLeft ^ 1];
-        box.lo = hmin3(box.lo, other.lo);
-        box.hi = hmax3(box.hi, other.hi);
-    }
-    *root = node;
-    nodes[node & ~kLeafBit].parent = kInvalidNode;
+    climb_leaf(tris, (uint32_t)n, (uint32_t)i, (uint32_t)(rec[i] >> 32), deltas, nodes, flags, root);
 }
 
 static inline unsigned grid1(size_t n) { return (unsigned)((n + kBuildBlock - 1) / kBuildBlock); }
 
-static int build_impl(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_nodes, void* d_root,
+int build_one(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_nodes, void* d_root,
                       uint64_t* d_sorted, uint8_t* d_deltas, hipStream_t stream)
 {
     if (!ctx || !d_tris || !d_nodes || !d_root) {
@@ -191,11 +153,15 @@ static int build_impl(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, v
     Node* nodes = (Node*)d_nodes;
     uint32_t* root = (uint32_t*)d_root;
 
-    k_build_init<<<1, 256, 0, stream>>>(ctx->sceneBox, ctx->hist, ctx->tickets);
+    if ((rc = ensure_status(ctx, sort_tiles(ctx, (uint32_t)n)))) return rc;
+    k_build_init<<<1, 256, 0, stream>>>(ctx->sceneBox, ctx->ctrl, ctx->hist, ctx->status, ctx->statusWords);
     unsigned g = grid1(nn);
     unsigned gAabb = g < (unsigned)ctx->cuCount * 8 ? g : (unsigned)ctx->cuCount * 8;
     k_scene_aabb<<<gAabb, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox);
-    k_morton<<<g, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->hist, ctx->flags);
+    unsigned gMorton = g < (unsigned)ctx->cuCount * 4 ? g : (unsigned)ctx->cuCount * 4;
+    if ((rc = ensure_partials(ctx, (int)gMorton, 4))) return rc;
+    k_morton<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials, ctx->flags);
+    launch_hist_reduce(ctx, (int)gMorton, 4, stream);
     rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream);
     if (rc) return rc;
     k_deltas<<<g, kBuildBlock, 0, stream>>>(ctx->recA, n, ctx->deltas);
@@ -220,13 +186,13 @@ using namespace mlbvh;
 extern "C" int mlbvh_build(mlbvh_ctx* ctx, const void* d_triangles, int n, uint32_t buildOption, void* d_nodes,
                            void* d_root, void* stream)
 {
-    return build_impl(ctx, d_triangles, n, buildOption, d_nodes, d_root, nullptr, nullptr, (hipStream_t)stream);
+    return build_one(ctx, d_triangles, n, buildOption, d_nodes, d_root, nullptr, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int mlbvh_build_ex(mlbvh_ctx* ctx, const void* d_triangles, int n, uint32_t buildOption, void* d_nodes,
                               void* d_root, uint64_t* d_sorted_records, uint8_t* d_deltas, void* stream)
 {
-    return build_impl(ctx, d_triangles, n, buildOption, d_nodes, d_root, d_sorted_records, d_deltas,
+    return build_one(ctx, d_triangles, n, buildOption, d_nodes, d_root, d_sorted_records, d_deltas,
                       (hipStream_t)stream);
 }
 

@@ -73,7 +73,11 @@ extern "C" int mlbvh_ctx_create(int device, mlbvh_ctx** out)
     size_t one = 0, cap = 0;
     if (!rc) rc = ensure(&ctx->hist, &one, (size_t)8 * 256, "hipMalloc(hist)");
     cap = 0;
-    if (!rc) rc = ensure(&ctx->tickets, &cap, (size_t)16, "hipMalloc(tickets)");
+    if (!rc) rc = ensure(&ctx->ctrl, &cap, (size_t)kCtrlWords, "hipMalloc(ctrl)");
+    if (!rc) {
+        hipError_t e = hipMemset(ctx->ctrl, 0, kCtrlWords * sizeof(uint32_t));
+        if (e != hipSuccess) rc = hip_status(e, "hipMemset(ctrl)");
+    }
     cap = 0;
     if (!rc) rc = ensure(&ctx->sceneBox, &cap, (size_t)8, "hipMalloc(sceneBox)");
     if (rc) {
@@ -88,7 +92,8 @@ extern "C" int mlbvh_ctx_destroy(mlbvh_ctx* ctx)
 {
     if (!ctx) return MLBVH_OK;
     hipSetDevice(ctx->device);
-    void* bufs[] = {ctx->status, ctx->hist, ctx->tickets, ctx->recA, ctx->recB, ctx->deltas, ctx->flags, ctx->sceneBox};
+    void* bufs[] = {ctx->status, ctx->hist,     ctx->ctrl,     ctx->recA,    ctx->recB,
+                    ctx->deltas, ctx->flags,    ctx->sceneBox, ctx->meshBox, ctx->partials};
     int rc = MLBVH_OK;
     for (void* p : bufs) {
         if (!p) continue;

@@ -1,0 +1,252 @@
+// Differentiable primary-hit rendering (no reference counterpart; built on
+// the reference's intersectRayTriangle, minimum_lbvh.h:363-413, and its
+// camera, camera.h:24-31).
+//
+// Forward (k_render_primary): for each pixel the closest hit of the camera
+// ray; out[p] = (t, s) with s = max(dot(n, L), 0), n the geometric normal
+// normalised and flipped toward the viewer; miss = (0, 0).  The hit triangle
+// id is kept for the backward pass.
+//
+// Adjoint (k_primary_adjoint): with the hit triangle fixed, t and s are
+// smooth functions of its three vertices:
+//   t = dot(v0 - ro, ng) / dot(ng, rd),  ng = (v1 - v0) x (v2 - v1)
+//   w = (v0 - ro - t rd) / dot(ng, rd)
+//   dt/dv0 = ng/den - e1 x w,  dt/dv1 = e1 x w - w x e0,  dt/dv2 = w x e0
+//   q = sgn (L - (nh.L) nh) / |ng|  (nh = ng/|ng|, sgn flips toward the viewer;
+//                                    q = 0 where s = 0)
+//   ds/dv0 = -(e1 x q), ds/dv1 = e1 x q - q x e0, ds/dv2 = q x e0
+// and dL/dv accumulates g_t dt/dv + g_s ds/dv into the per-vertex gradient
+// with fp32 atomics (9 per hit pixel).
+#include "mlbvh_ctx.h"
+
+namespace mlbvh {
+
+constexpr int kDiffBlock = 256;
+
+__global__ void __launch_bounds__(kDiffBlock)
+    k_gather(const float* __restrict__ v, const uint32_t* __restrict__ idx, int nTris, Tri* __restrict__ tris)
+{
+    int f = blockIdx.x * kDiffBlock + threadIdx.x;
+    if (f >= nTris) return;
+    Tri t;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        uint32_t j = idx[3 * f + k];
+        t.v[k] = mk(v[3 * j], v[3 * j + 1], v[3 * j + 2]);
+    }
+    tris[f] = t;
+}
+
+// slabs / triangle test / traversal shared with trace.hip through this file's
+// own copy of the closest-hit loop (kept local so diff.hip links on its own).
+__device__ inline float2 d_slabs(f3 ro, f3 inv, f3 lo, f3 hi, float tHit)
+{
+    f3 t0 = (lo - ro) * inv;
+    f3 t1 = (hi - ro) * inv;
+    float rmin = fmaxf(fmaxf(fminf(t0.x, t1.x), fminf(t0.y, t1.y)), fminf(t0.z, t1.z));
+    float rmax = fminf(fminf(fmaxf(t0.x, t1.x), fmaxf(t0.y, t1.y)), fmaxf(t0.z, t1.z)) * 1.00000024f;
+    return make_float2(fmaxf(rmin, 0.0f), fminf(rmax, tHit));
+}
+
+__device__ inline float d_isect_t(float tmin, float tmax, f3 ro, f3 rd, f3 v0, f3 v1, f3 v2, f3* ngOut, bool* ok)
+{
+    f3 e0 = v1 - v0;
+    f3 e1 = v2 - v1;
+    f3 ng = cross(e0, e1);
+    float t = dot(v0 - ro, ng) / dot(ng, rd);
+    *ok = false;
+    if (tmin <= t && t <= tmax) {
+        f3 e2 = v0 - v2;
+        f3 ro2 = ro * 2.0f;
+        float u = dot(cross(e2, (v0 + v2) - ro2), rd);
+        float v = dot(cross(e0, (v1 + v0) - ro2), rd);
+        float w = dot(cross(e1, (v2 + v1) - ro2), rd);
+        if (!(fminf(fminf(u, v), w) < 0.0f && 0.0f < fmaxf(fmaxf(u, v), w))) *ok = true;
+    }
+    *ngOut = ng;
+    return t;
+}
+
+__device__ inline uint32_t d_closest(const Node* __restrict__ nodes, const Tri* __restrict__ tris, uint32_t root,
+                                     f3 ro, f3 rd, float* tHit, f3* ngHit)
+{
+    f3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    inv = mk(fmaxf(-kFltMax, fminf(inv.x, kFltMax)), fmaxf(-kFltMax, fminf(inv.y, kFltMax)),
+             fmaxf(-kFltMax, fminf(inv.z, kFltMax)));
+    uint32_t curr = root, prev = kInvalidNode, best = 0xFFFFFFFFu;
+    uint32_t guard = 1u << 26;
+    while (curr != kInvalidNode && guard-- != 0) {
+        if (curr & kLeafBit) {
+            const Tri tri = tris[curr & ~kLeafBit];
+            f3 ng;
+            bool ok;
+            float t = d_isect_t(0.0f, *tHit, ro, rd, tri.v[0], tri.v[1], tri.v[2], &ng, &ok);
+            if (ok) {
+                *tHit = t;
+                *ngHit = ng;
+                best = curr & ~kLeafBit;
+            }
+            uint32_t tmp = curr;
+            curr = prev;
+            prev = tmp;
+            continue;
+        }
+        const Node& nd = nodes[curr];
+        float2 rl = d_slabs(ro, inv, nd.box[0].lo, nd.box[0].hi, *tHit);
+        float2 rr = d_slabs(ro, inv, nd.box[1].lo, nd.box[1].hi, *tHit);
+        bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y;
+        uint32_t nearN = nd.child[0], farN = nd.child[1];
+        int nHits = 0;
+        if (hitL && hitR) {
+            if (rr.x < rl.x) {
+                nearN = nd.child[1];
+                farN = nd.child[0];
+            }
+            nHits = 2;
+        } else if (hitL || hitR) {
+            nHits = 1;
+            nearN = hitR ? farN : nearN;
+        }
+        uint32_t next;
+        if (prev == nd.parent) next = 0 < nHits ? nearN : nd.parent;
+        else if (prev == nearN) next = nHits == 2 ? farN : nd.parent;
+        else next = nd.parent;
+        prev = curr;
+        curr = next;
+    }
+    return best;
+}
+
+__global__ void __launch_bounds__(kDiffBlock)
+    k_render_primary(float2* __restrict__ out, uint32_t* __restrict__ hitTri, int w, int h, mlbvh_raygen cam,
+                     f3 light, const uint32_t* __restrict__ rootPtr, const Node* __restrict__ nodes,
+                     const Tri* __restrict__ tris)
+{
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int xi = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    int yi = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    if (xi >= w || yi >= h) return;
+    int p = xi + yi * w;
+    f3 ro, rd;
+    shoot(cam, &ro, &rd, (float)xi / (float)w, (float)yi / (float)h);
+    float t = kFltMax;
+    f3 ng = mk(0, 0, 0);
+    uint32_t tri = d_closest(nodes, tris, *rootPtr, ro, rd, &t, &ng);
+    hitTri[p] = tri;
+    if (tri == 0xFFFFFFFFu) {
+        out[p] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    f3 n = normalize(ng);
+    if (0.0f < dot(n, rd)) n = -n;
+    float s = dot(n, light);
+    out[p] = make_float2(t, s > 0.0f ? s : 0.0f);
+}
+
+__global__ void __launch_bounds__(kDiffBlock)
+    k_primary_adjoint(const float2* __restrict__ gout, const uint32_t* __restrict__ hitTri, int w, int h,
+                      mlbvh_raygen cam, f3 light, const float* __restrict__ vtx, const uint32_t* __restrict__ idx,
+                      float* __restrict__ grad)
+{
+    int p = blockIdx.x * kDiffBlock + threadIdx.x;
+    if (p >= w * h) return;
+    uint32_t tri = hitTri[p];
+    if (tri == 0xFFFFFFFFu) return;
+    float2 g = gout[p];
+    if (g.x == 0.0f && g.y == 0.0f) return;
+    int xi = p % w, yi = p / w;
+    f3 ro, rd;
+    shoot(cam, &ro, &rd, (float)xi / (float)w, (float)yi / (float)h);
+    uint32_t j[3];
+    f3 v[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        j[k] = idx[3 * tri + k];
+        v[k] = mk(vtx[3 * j[k]], vtx[3 * j[k] + 1], vtx[3 * j[k] + 2]);
+    }
+    f3 e0 = v[1] - v[0], e1 = v[2] - v[1];
+    f3 ng = cross(e0, e1);
+    float den = dot(ng, rd);
+    float t = dot(v[0] - ro, ng) / den;
+    f3 wv = (v[0] - ro - rd * t) * (1.0f / den);
+    f3 e1w = cross(e1, wv), we0 = cross(wv, e0);
+    f3 dt0 = ng * (1.0f / den) - e1w, dt1 = e1w - we0, dt2 = we0;
+    f3 d0 = dt0 * g.x, d1 = dt1 * g.x, d2 = dt2 * g.x;
+    float len = sqrtf(dot(ng, ng));
+    f3 nh = ng * (1.0f / len);
+    float sgn = dot(nh, rd) > 0.0f ? -1.0f : 1.0f;
+    float s = sgn * dot(nh, light);
+    if (s > 0.0f && g.y != 0.0f) {
+        f3 q = (light - nh * dot(nh, light)) * (sgn / len);
+        f3 e1q = cross(e1, q), qe0 = cross(q, e0);
+        d0 = d0 - e1q * g.y;
+        d1 = d1 + (e1q - qe0) * g.y;
+        d2 = d2 + qe0 * g.y;
+    }
+    const f3 d[3] = {d0, d1, d2};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        atomicAdd(&grad[3 * j[k] + 0], d[k].x);
+        atomicAdd(&grad[3 * j[k] + 1], d[k].y);
+        atomicAdd(&grad[3 * j[k] + 2], d[k].z);
+    }
+}
+
+}  // namespace mlbvh
+
+using namespace mlbvh;
+
+extern "C" int mlbvh_gather_triangles(mlbvh_ctx* ctx, const float* d_vertices, const uint32_t* d_indices, int nTris,
+                                      void* d_triangles, void* stream)
+{
+    if (!ctx || nTris < 0 || (nTris > 0 && (!d_vertices || !d_indices || !d_triangles))) {
+        set_error("gather_triangles: invalid arguments");
+        return MLBVH_E_INVALID_ARG;
+    }
+    if (nTris == 0) return MLBVH_OK;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    k_gather<<<(unsigned)((nTris + kDiffBlock - 1) / kDiffBlock), kDiffBlock, 0, (hipStream_t)stream>>>(
+        d_vertices, d_indices, nTris, (Tri*)d_triangles);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, "gather launch");
+}
+
+extern "C" int mlbvh_render_primary(mlbvh_ctx* ctx, float* d_out, uint32_t* d_hitTri, int width, int height,
+                                    const mlbvh_raygen* camera, const float lightDir[3], const void* d_root,
+                                    const void* d_nodes, const void* d_triangles, void* stream)
+{
+    if (!ctx || !d_out || !d_hitTri || !camera || !lightDir || !d_root || !d_nodes || !d_triangles || width <= 0 ||
+        height <= 0) {
+        set_error("render_primary: invalid arguments");
+        return MLBVH_E_INVALID_ARG;
+    }
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    dim3 g((unsigned)((width + 15) / 16), (unsigned)((height + 15) / 16), 1);
+    k_render_primary<<<g, kDiffBlock, 0, (hipStream_t)stream>>>(
+        (float2*)d_out, d_hitTri, width, height, *camera, mk(lightDir[0], lightDir[1], lightDir[2]),
+        (const uint32_t*)d_root, (const Node*)d_nodes, (const Tri*)d_triangles);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, "render_primary launch");
+}
+
+extern "C" int mlbvh_primary_adjoint(mlbvh_ctx* ctx, const float* d_gradOut, const uint32_t* d_hitTri, int width,
+                                     int height, const mlbvh_raygen* camera, const float lightDir[3],
+                                     const float* d_vertices, const uint32_t* d_indices, float* d_gradVertices,
+                                     void* stream)
+{
+    if (!ctx || !d_gradOut || !d_hitTri || !camera || !lightDir || !d_vertices || !d_indices || !d_gradVertices ||
+        width <= 0 || height <= 0) {
+        set_error("primary_adjoint: invalid arguments");
+        return MLBVH_E_INVALID_ARG;
+    }
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    int n = width * height;
+    k_primary_adjoint<<<(unsigned)((n + kDiffBlock - 1) / kDiffBlock), kDiffBlock, 0, (hipStream_t)stream>>>(
+        (const float2*)d_gradOut, d_hitTri, width, height, *camera, mk(lightDir[0], lightDir[1], lightDir[2]),
+        d_vertices, d_indices, d_gradVertices);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, "primary_adjoint launch");
+}

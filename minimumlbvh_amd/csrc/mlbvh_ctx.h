@@ -10,9 +10,15 @@ struct mlbvh_ctx {
     // --- radix sort state -------------------------------------------------
     uint64_t* status = nullptr;  // decoupled look-back words, [tiles][256]
     size_t statusWords = 0;
-    uint32_t* hist = nullptr;     // digit histograms, [8][256]
-    uint32_t* tickets = nullptr;  // per-pass tile tickets, [16]
-    uint32_t epoch = 0;           // pass tag stored in every look-back word
+    uint32_t* hist = nullptr;  // digit histograms, [8][256]
+    uint32_t* partials = nullptr;  // per-block histogram partials, [blocks][passes][256]
+    size_t partialsCap = 0;
+    int sortConfig = 0;  // tile shape index into the sort's configuration table
+    // Device control words (see kCtrl* below): the pass-tag counter, the
+    // per-pass tile tickets and the per-pass tags.  They live in device memory
+    // and are advanced by the init kernels, so a captured graph replays with
+    // fresh tags.
+    uint32_t* ctrl = nullptr;
 
     // --- build scratch ------------------------------------------------------
     uint64_t* recA = nullptr;  // IndexedMorton records (ping)
@@ -24,9 +30,51 @@ struct mlbvh_ctx {
     uint32_t* flags = nullptr;  // per-internal-node arrival word of the climb
     size_t flagsCap = 0;
     uint32_t* sceneBox = nullptr;  // 6 order-mapped uint32 (lo xyz, hi xyz)
+
+    // --- batched build / trace ------------------------------------------------
+    uint32_t* meshBox = nullptr;  // [meshes][6] order-mapped bounds
+    size_t meshBoxCap = 0;
+    int traceBlocksPerCU = 8;  // persistent trace grid = cuCount * this
+    int traceChunk = 64;       // rays a wave takes per work-queue pull (multiple of 64)
 };
 
 namespace mlbvh {
+
+// Layout of mlbvh_ctx::ctrl (uint32 words).
+constexpr int kCtrlEpochCounter = 0;
+constexpr int kCtrlTickets = 16;  // [16]
+constexpr int kCtrlEpochs = 32;   // [16]
+constexpr int kCtrlWorkQueue = 48;  // ray counter of the persistent trace
+constexpr int kCtrlWords = 64;
+constexpr int kNumSortCfgs = 5;  // tile shapes of the radix sort (onesweep.hip)
+constexpr uint32_t kEpochLimit = (1u << 30) - 1;  // look-back words keep 30 tag bits
+
+// Sort-state set-up for one sort of nPass digit passes; run by a single
+// 256-thread block on the sort's stream before the passes: zero the nHist
+// digit histograms and the tile tickets, and hand each pass a fresh tag.
+// When the tag counter would leave 30 bits the look-back table is cleared and
+// the tags restart at 1 (tag 0 never matches, the table starts zeroed).
+__device__ inline void sort_state_init(uint32_t* ctrl, int nPass, uint32_t* hist, int nHist, uint64_t* status,
+                                       size_t statusWords)
+{
+    __shared__ uint32_t s_base;
+    __shared__ int s_clear;
+    if (threadIdx.x == 0) {
+        uint32_t c = ctrl[kCtrlEpochCounter];
+        s_clear = c + (uint32_t)nPass >= kEpochLimit;
+        if (s_clear) c = 0;
+        s_base = c;
+        ctrl[kCtrlEpochCounter] = c + (uint32_t)nPass;
+    }
+    __syncthreads();
+    if (s_clear)
+        for (size_t i = threadIdx.x; i < statusWords; i += blockDim.x) status[i] = 0;
+    for (int i = threadIdx.x; i < nHist * 256; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x < 16) {
+        ctrl[kCtrlTickets + threadIdx.x] = 0;
+        ctrl[kCtrlEpochs + threadIdx.x] = s_base + 1 + threadIdx.x;
+    }
+}
 
 // Grow a device buffer to hold at least `need` elements (contents dropped).
 template <class T>
@@ -51,10 +99,22 @@ int ensure(T** p, size_t* cap, size_t need, const char* what)
 
 int set_device(mlbvh_ctx* ctx);
 
-// Radix sort passes shared by mlbvh_sort_* and the builder.  When histReady
-// is true the caller has already filled ctx->hist[0..passes) and zeroed
-// ctx->tickets (the builder fuses both into its Morton kernel).
+// Radix sort passes shared by mlbvh_sort_* and the builders.  When histReady
+// is true the caller has already run sort_state_init and filled
+// ctx->hist[0..passes) (the builders fuse both into their own kernels) and
+// has called ensure_status(ctx, sort_tiles(n)).
 int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, int startBit, int endBit,
                      bool histReady, hipStream_t stream);
+uint32_t sort_tiles(mlbvh_ctx* ctx, uint32_t n);
+int sort_tile_keys(mlbvh_ctx* ctx);
+int ensure_status(mlbvh_ctx* ctx, uint32_t tiles);
+// Histogram partials of `blocks` blocks x nPass passes, and their reduction
+// into ctx->hist (one launch on `stream`).
+int ensure_partials(mlbvh_ctx* ctx, int blocks, int nPass);
+void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream);
+
+// One plain build (the body of mlbvh_build_ex).
+int build_one(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_nodes, void* d_root,
+              uint64_t* d_sorted, uint8_t* d_deltas, hipStream_t stream);
 
 }  // namespace mlbvh

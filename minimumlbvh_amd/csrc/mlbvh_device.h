@@ -137,6 +137,93 @@ __device__ inline void shoot(const mlbvh_raygen& g, f3* ro, f3* rd, float u, flo
     *rd = to - origin;
 }
 
+// ---------------------------------------------------------------------------
+// Agglomerative climb for one sorted leaf (build_lbvh, minimum_lbvh.h:284-361)
+// on mesh-local indices: leaf position `pos` of an n-triangle mesh, its
+// triangle `triIndex`, deltas of the mesh, its nodes and arrival words.
+//
+// Cross-XCD hand-off without L2 flushes: the only data one thread reads from
+// another is the sibling subtree's AABB.  It is written with device-scope
+// (sc1) stores, which complete at the device coherence point; the wait on the
+// vector memory counter makes them complete before the device-scope exchange
+// on the arrival word; the second arriver reads the box back with
+// device-scope loads issued after the exchange returned.  All other node
+// fields are read only after the kernel ends.
+// ---------------------------------------------------------------------------
+__device__ inline void store_box_coherent(Box* dst, const Box& b)
+{
+    uint64_t* d = reinterpret_cast<uint64_t*>(dst);  // 8-byte aligned (offset 8 or 32 in the node)
+    __hip_atomic_store(d + 0, (uint64_t)__float_as_uint(b.lo.x) | ((uint64_t)__float_as_uint(b.lo.y) << 32),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 1, (uint64_t)__float_as_uint(b.lo.z) | ((uint64_t)__float_as_uint(b.hi.x) << 32),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 2, (uint64_t)__float_as_uint(b.hi.y) | ((uint64_t)__float_as_uint(b.hi.z) << 32),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline Box load_box_coherent(Box* src)
+{
+    uint64_t* s = reinterpret_cast<uint64_t*>(src);
+    uint64_t a = __hip_atomic_load(s + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t b = __hip_atomic_load(s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t c = __hip_atomic_load(s + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Box r;
+    r.lo = mk(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)), __uint_as_float((uint32_t)b));
+    r.hi = mk(__uint_as_float((uint32_t)(b >> 32)), __uint_as_float((uint32_t)c), __uint_as_float((uint32_t)(c >> 32)));
+    return r;
+}
+
+// Wait until this lane's earlier vector memory operations (the sc1 box
+// stores) have completed: s_waitcnt vmcnt(0), gfx9 encoding.
+__device__ inline void wait_vector_memory()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Returns true for the thread that completed the root (it writes *root).
+__device__ inline bool climb_leaf(const Tri* __restrict__ tris, uint32_t n, uint32_t pos, uint32_t triIndex,
+                                  const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags, uint32_t* root)
+{
+    const uint32_t nInternals = n - 1;
+    Tri tri = tris[triIndex];
+    Box box;
+    box.lo = mk(kFltMax, kFltMax, kFltMax);
+    box.hi = mk(-kFltMax, -kFltMax, -kFltMax);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        box.lo = hmin3(box.lo, tri.v[k]);
+        box.hi = hmax3(box.hi, tri.v[k]);
+    }
+    uint32_t lo = pos, hi = pos;
+    uint32_t node = triIndex | kLeafBit;
+    while (hi - lo < nInternals) {
+        uint32_t dL = 0 < lo ? (uint32_t)deltas[lo - 1] : 0xFFFFFFFFu;
+        uint32_t dR = hi < nInternals ? (uint32_t)deltas[hi] : 0xFFFFFFFFu;
+        int goLeft = dL < dR ? 1 : 0;
+        uint32_t parent = goLeft ? lo - 1 : hi;
+        Node* p = nodes + parent;
+        store_box_coherent(&p->box[goLeft], box);
+        p->child[goLeft] = node;
+        if (!(node & kLeafBit)) nodes[node].parent = parent;
+        uint32_t idx = goLeft ? hi : lo;
+        wait_vector_memory();
+        idx = __hip_atomic_exchange(&flags[parent], idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (idx == 0xFFFFFFFFu) return false;
+        lo = lo < idx ? lo : idx;
+        hi = hi < idx ? idx : hi;
+        p->context = hi < nInternals ? hi : 0xFFFFFFFFu;
+        node = parent;
+        Box other = load_box_coherent(&p->box[goLeft ^ 1]);
+        box.lo = hmin3(box.lo, other.lo);
+        box.hi = hmax3(box.hi, other.hi);
+    }
+    *root = node;
+    nodes[node & ~kLeafBit].parent = kInvalidNode;
+    return true;
+}
+
 // Error plumbing shared by the entry points.
 void set_error(const char* fmt, ...);
 int hip_status(hipError_t e, const char* what);

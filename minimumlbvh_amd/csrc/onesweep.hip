@@ -5,15 +5,18 @@
 // startBit) / 8) passes, result left in the caller's keys/values.
 //
 // Structure (one upfront histogram, then one kernel per digit pass):
-//   k_sort_init   zero the per-pass digit histograms and tile tickets
-//   k_hist        all digit histograms of the whole input in one read
-//   k_pass        per tile: wave64 ballot multisplit ranking, per-digit
-//                 decoupled look-back over a [tile][digit] status table,
-//                 LDS-staged local scatter, coalesced writes of digit runs
+//   k_sort_init    hand every pass a fresh look-back tag, zero tickets
+//   k_hist         all digit histograms of the input in one read, written as
+//                  per-block partials (no global atomics)
+//   k_hist_reduce  partials -> per-pass histograms
+//   k_pass         per tile: wave64 ballot multisplit ranking, per-digit
+//                  decoupled look-back over a [tile][digit] status table,
+//                  LDS-staged local scatter, writes of contiguous digit runs
 //
 // Tiles are handed out by an atomic ticket, so a block only waits on tiles
-// that are already running; look-back words carry a pass tag (epoch), so the
-// status table is never cleared between passes.
+// that are already running; look-back words carry a pass tag, so the status
+// table is never cleared between passes.  The tile shape (threads x keys per
+// thread) is one of kSortCfgs, chosen per context (mlbvh_set_tuning).
 #include <stdarg.h>
 #include <stdio.h>
 
@@ -21,10 +24,10 @@
 
 namespace mlbvh {
 
-constexpr int kSortBlock = 256;  // 4 waves
-constexpr int kSortWaves = kSortBlock / 64;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortBlock * kSortItems;  // 4096 keys
+struct SortCfg {
+    int block, items;
+};
+constexpr SortCfg kSortCfgs[kNumSortCfgs] = {{256, 16}, {256, 32}, {512, 16}, {512, 24}, {1024, 16}};
 constexpr int kHistBlock = 256;
 
 constexpr uint64_t kFlagA = 1;  // tile aggregate published
@@ -38,8 +41,9 @@ __device__ inline uint64_t lb_pack(uint32_t epoch, uint64_t flag, uint32_t value
 __device__ inline uint32_t digit_of(uint32_t k, int shift) { return (k >> shift) & 0xFFu; }
 __device__ inline uint32_t digit_of(uint64_t k, int shift) { return (uint32_t)(k >> shift) & 0xFFu; }
 
-// Exclusive scan of one value per thread over a 256-thread block.
-__device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t* s_tmp, uint32_t* total)
+// Exclusive scan of one value per thread over threads 0..255 (threads past
+// 255 pass 0 and get an unspecified result); every thread of the block calls.
+__device__ inline uint32_t scan256(uint32_t v, uint32_t* s_tmp)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -49,79 +53,96 @@ __device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t* s_tmp, uint3
         uint32_t y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
-    if (lane == 63) s_tmp[wave] = x;
+    if (wave < 4 && lane == 63) s_tmp[wave] = x;
     __syncthreads();
-    uint32_t waveBase = 0, sum = 0;
+    uint32_t base = 0;
 #pragma unroll
-    for (int w = 0; w < kSortWaves; w++) {
-        uint32_t s = s_tmp[w];
-        if (w < wave) waveBase += s;
-        sum += s;
-    }
+    for (int w = 0; w < 4; w++)
+        if (w < wave) base += s_tmp[w];
     __syncthreads();
-    if (total) *total = sum;
-    return waveBase + x - v;
+    return base + x - v;
 }
 
-__global__ void k_sort_init(uint32_t* hist, int nHist, uint32_t* tickets)
+__global__ void k_sort_init(uint32_t* ctrl, int nPass, uint32_t* hist, uint64_t* status, size_t statusWords)
 {
-    for (int i = threadIdx.x; i < nHist * 256; i += blockDim.x) hist[i] = 0;
-    if (threadIdx.x < 16) tickets[threadIdx.x] = 0;
+    sort_state_init(ctrl, nPass, hist, nPass, status, statusWords);
 }
 
 template <class K>
 __global__ void __launch_bounds__(kHistBlock) k_hist(const K* __restrict__ keys, uint32_t n, int startBit,
-                                                     int nPass, uint32_t* __restrict__ hist)
+                                                     int nPass, uint32_t* __restrict__ partials)
 {
     __shared__ uint32_t s_h[8][256];
     for (int p = 0; p < 8; p++) s_h[p][threadIdx.x] = 0;
     __syncthreads();
     const int keyBits = 8 * (int)sizeof(K);
-    for (uint32_t i = blockIdx.x * kHistBlock + threadIdx.x; i < n; i += gridDim.x * kHistBlock) {
+    const uint32_t stride = gridDim.x * kHistBlock;
+    uint32_t i = blockIdx.x * kHistBlock + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        K k[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) k[u] = keys[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            for (int p = 0; p < nPass; p++) {
+                int sh = startBit + 8 * p;
+                atomicAdd(&s_h[p][sh < keyBits ? digit_of(k[u], sh) : 0u], 1u);
+            }
+    }
+    for (; i < n; i += stride) {
         K k = keys[i];
         for (int p = 0; p < nPass; p++) {
             int sh = startBit + 8 * p;
-            uint32_t d = sh < keyBits ? digit_of(k, sh) : 0u;
-            atomicAdd(&s_h[p][d], 1u);
+            atomicAdd(&s_h[p][sh < keyBits ? digit_of(k, sh) : 0u], 1u);
         }
     }
     __syncthreads();
-    for (int p = 0; p < nPass; p++) {
-        uint32_t c = s_h[p][threadIdx.x];
-        if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
-    }
+    for (int p = 0; p < nPass; p++) partials[((size_t)blockIdx.x * nPass + p) * 256 + threadIdx.x] = s_h[p][threadIdx.x];
 }
 
-template <class K, bool kVals>
-__global__ void __launch_bounds__(kSortBlock)
+// hist[p][d] = sum over blocks of partials[block][p][d]; grid = nPass blocks.
+__global__ void __launch_bounds__(256) k_hist_reduce(const uint32_t* __restrict__ partials, int nBlocks, int nPass,
+                                                     uint32_t* __restrict__ hist)
+{
+    const int p = blockIdx.x;
+    uint32_t s = 0;
+    for (int b = 0; b < nBlocks; b++) s += partials[((size_t)b * nPass + p) * 256 + threadIdx.x];
+    hist[p * 256 + threadIdx.x] = s;
+}
+
+template <class K, bool kVals, int kBlock, int kItems>
+__global__ void __launch_bounds__(kBlock)
     k_pass(const K* __restrict__ kin, K* __restrict__ kout, const uint32_t* __restrict__ vin,
            uint32_t* __restrict__ vout, uint32_t n, int shift, const uint32_t* __restrict__ passHist,
-           uint64_t* status, uint32_t* ticket, uint32_t epoch)
+           uint64_t* status, uint32_t* ctrl, int pass)
 {
-    __shared__ K s_keys[kSortTile];
-    __shared__ uint32_t s_vals[kVals ? kSortTile : 1];
-    __shared__ uint32_t s_wcnt[kSortWaves][256];
+    constexpr int kTile = kBlock * kItems;
+    constexpr int kWaves = kBlock / 64;
+    __shared__ K s_keys[kTile];  // keys, then (as uint32) values
+    __shared__ uint32_t s_wcnt[kWaves][256];
     __shared__ uint32_t s_dstart[256];
     __shared__ uint32_t s_gbase[256];
-    __shared__ uint32_t s_tmp[kSortWaves];
+    __shared__ uint32_t s_tmp[4];
     __shared__ uint32_t s_tile;
 
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
+    uint32_t* ticket = ctrl + kCtrlTickets + pass;
+    const uint32_t epoch = ctrl[kCtrlEpochs + pass];
 
     if (t == 0) s_tile = atomicAdd(ticket, 1u);
-    for (int w = 0; w < kSortWaves; w++) s_wcnt[w][t] = 0;
+    for (int i = t; i < kWaves * 256; i += kBlock) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint64_t tileBase = (uint64_t)tile * kSortTile;
-    const uint64_t waveBase = tileBase + (uint64_t)wave * 64 * kSortItems;
+    const uint64_t tileBase = (uint64_t)tile * kTile;
+    const uint64_t waveBase = tileBase + (uint64_t)wave * 64 * kItems;
 
-    K key[kSortItems];
-    uint32_t val[kVals ? kSortItems : 1];
-    uint32_t off[kSortItems];
+    K key[kItems];
+    uint32_t val[kVals ? kItems : 1];
+    uint32_t off[kItems];
 #pragma unroll
-    for (int k = 0; k < kSortItems; k++) {
+    for (int k = 0; k < kItems; k++) {
         uint64_t idx = waveBase + (uint64_t)k * 64 + lane;
         if (idx < n) {
             key[k] = kin[idx];
@@ -136,7 +157,7 @@ __global__ void __launch_bounds__(kSortBlock)
     // ballots; a lane's rank is the number of peers below it.
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
-    for (int k = 0; k < kSortItems; k++) {
+    for (int k = 0; k < kItems; k++) {
         uint64_t idx = waveBase + (uint64_t)k * 64 + lane;
         bool valid = idx < n;
         uint32_t d = digit_of(key[k], shift);
@@ -155,63 +176,84 @@ __global__ void __launch_bounds__(kSortBlock)
     }
     __syncthreads();
 
-    // Per digit (thread t owns digit t): wave offsets inside the tile.
+    // Per digit (threads 0..255 own digit t): wave offsets inside the tile.
     uint32_t tot = 0;
+    if (t < 256) {
 #pragma unroll
-    for (int w = 0; w < kSortWaves; w++) {
-        uint32_t c = s_wcnt[w][t];
-        s_wcnt[w][t] = tot;
-        tot += c;
+        for (int w = 0; w < kWaves; w++) {
+            uint32_t c = s_wcnt[w][t];
+            s_wcnt[w][t] = tot;
+            tot += c;
+        }
     }
-    uint32_t dstart = block_excl_scan256(tot, s_tmp, nullptr);
-    uint32_t gstart = block_excl_scan256(passHist[t], s_tmp, nullptr);
-    s_dstart[t] = dstart;
+    uint32_t dstart = scan256(tot, s_tmp);
+    uint32_t gstart = scan256(t < 256 ? passHist[t] : 0u, s_tmp);
 
-    // Decoupled look-back for digit t.
-    uint64_t* my = status + (uint64_t)tile * 256 + t;
-    uint32_t excl = 0;
-    if (tile == 0) {
-        __hip_atomic_store(my, lb_pack(epoch, kFlagP, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(my, lb_pack(epoch, kFlagA, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int64_t j = (int64_t)tile - 1;
-        while (true) {
-            const uint64_t* src = status + (uint64_t)j * 256 + t;
-            uint64_t w;
+    if (t < 256) {
+        s_dstart[t] = dstart;
+        // Decoupled look-back for digit t.
+        uint64_t* my = status + (uint64_t)tile * 256 + t;
+        uint32_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(my, lb_pack(epoch, kFlagP, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(my, lb_pack(epoch, kFlagA, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)tile - 1;
             while (true) {
-                w = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((uint32_t)(w >> 34) == epoch && ((w >> 32) & 3u) != 0) break;
-                __builtin_amdgcn_s_sleep(1);
+                const uint64_t* src = status + (uint64_t)j * 256 + t;
+                uint64_t w;
+                while (true) {
+                    w = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((uint32_t)(w >> 34) == epoch && ((w >> 32) & 3u) != 0) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                excl += (uint32_t)w;
+                if (((w >> 32) & 3u) == kFlagP) break;
+                j--;
             }
-            excl += (uint32_t)w;
-            if (((w >> 32) & 3u) == kFlagP) break;
-            j--;
+            __hip_atomic_store(my, lb_pack(epoch, kFlagP, excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __hip_atomic_store(my, lb_pack(epoch, kFlagP, excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_gbase[t] = gstart + excl - dstart;
     }
-    s_gbase[t] = gstart + excl - dstart;
     __syncthreads();
 
-    // Stable local scatter into LDS (digit-major, original order within digit).
+    // Stable local scatter into LDS (digit-major, input order within digit).
+    uint32_t pos[kItems];
 #pragma unroll
-    for (int k = 0; k < kSortItems; k++) {
+    for (int k = 0; k < kItems; k++) {
         uint64_t idx = waveBase + (uint64_t)k * 64 + lane;
-        if (idx < n) {
-            uint32_t d = digit_of(key[k], shift);
-            uint32_t pos = s_dstart[d] + s_wcnt[wave][d] + off[k];
-            s_keys[pos] = key[k];
-            if (kVals) s_vals[pos] = val[k];
-        }
+        uint32_t d = digit_of(key[k], shift);
+        pos[k] = s_dstart[d] + s_wcnt[wave][d] + off[k];
+        if (idx < n) s_keys[pos[k]] = key[k];
     }
     __syncthreads();
 
-    const uint32_t tileCount = (uint32_t)((n - tileBase) < (uint64_t)kSortTile ? (n - tileBase) : kSortTile);
-    for (uint32_t i = t; i < tileCount; i += kSortBlock) {
-        K k = s_keys[i];
-        uint32_t d = digit_of(k, shift);
-        uint32_t dst = s_gbase[d] + i;
-        kout[dst] = k;
-        if (kVals) vout[dst] = s_vals[i];
+    const uint32_t tileCount = (uint32_t)((n - tileBase) < (uint64_t)kTile ? (n - tileBase) : kTile);
+    uint32_t dst[kVals ? kItems : 1];
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        uint32_t i = (uint32_t)(t + j * kBlock);
+        if (i < tileCount) {
+            K k = s_keys[i];
+            uint32_t o = s_gbase[digit_of(k, shift)] + i;
+            kout[o] = k;
+            if (kVals) dst[j] = o;
+        }
+    }
+    if (kVals) {
+        uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            uint64_t idx = waveBase + (uint64_t)k * 64 + lane;
+            if (idx < n) s_vals[pos[k]] = val[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kItems; j++) {
+            uint32_t i = (uint32_t)(t + j * kBlock);
+            if (i < tileCount) vout[dst[j]] = s_vals[i];
+        }
     }
 }
 
@@ -223,19 +265,17 @@ static int hist_grid(mlbvh_ctx* ctx, uint32_t n)
     return (int)(want < cap ? want : cap);
 }
 
-static int bump_epoch(mlbvh_ctx* ctx, hipStream_t stream)
+int sort_tile_keys(mlbvh_ctx* ctx) { return kSortCfgs[ctx->sortConfig].block * kSortCfgs[ctx->sortConfig].items; }
+
+uint32_t sort_tiles(mlbvh_ctx* ctx, uint32_t n)
 {
-    // 30-bit pass tag; on wrap, clear the table so no stale word can match.
-    if (ctx->epoch >= (1u << 30) - 2) {
-        hipError_t e = hipMemsetAsync(ctx->status, 0, ctx->statusWords * sizeof(uint64_t), stream);
-        if (e != hipSuccess) return hip_status(e, "hipMemsetAsync(status)");
-        ctx->epoch = 0;
-    }
-    ctx->epoch++;
-    return MLBVH_OK;
+    uint64_t tile = (uint64_t)sort_tile_keys(ctx);
+    return (uint32_t)((n + tile - 1) / tile);
 }
 
-static int ensure_status(mlbvh_ctx* ctx, uint32_t tiles)
+// Look-back table for `tiles` tiles; a new table starts zeroed (tag 0 never
+// matches a pass tag).
+int ensure_status(mlbvh_ctx* ctx, uint32_t tiles)
 {
     size_t need = (size_t)tiles * 256;
     if (need <= ctx->statusWords && ctx->status) return MLBVH_OK;
@@ -243,8 +283,33 @@ static int ensure_status(mlbvh_ctx* ctx, uint32_t tiles)
     if (rc) return rc;
     hipError_t e = hipMemset(ctx->status, 0, ctx->statusWords * sizeof(uint64_t));
     if (e != hipSuccess) return hip_status(e, "hipMemset(status)");
-    ctx->epoch = 0;
     return MLBVH_OK;
+}
+
+int ensure_partials(mlbvh_ctx* ctx, int blocks, int nPass)
+{
+    return ensure(&ctx->partials, &ctx->partialsCap, (size_t)blocks * nPass * 256, "hipMalloc(histogram partials)");
+}
+
+template <class K, bool kVals, int B, int I>
+static void launch_pass(hipStream_t s, uint32_t tiles, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
+                        uint32_t n, int shift, const uint32_t* hist, uint64_t* status, uint32_t* ctrl, int p)
+{
+    k_pass<K, kVals, B, I><<<tiles, B, 0, s>>>(src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p);
+}
+
+template <class K, bool kVals>
+static void dispatch_pass(int cfg, hipStream_t s, uint32_t tiles, const K* src, K* dst, const uint32_t* vsrc,
+                          uint32_t* vdst, uint32_t n, int shift, const uint32_t* hist, uint64_t* status,
+                          uint32_t* ctrl, int p)
+{
+    switch (cfg) {
+    case 0: launch_pass<K, kVals, 256, 16>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p); break;
+    case 1: launch_pass<K, kVals, 256, 32>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p); break;
+    case 2: launch_pass<K, kVals, 512, 16>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p); break;
+    case 3: launch_pass<K, kVals, 512, 24>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p); break;
+    default: launch_pass<K, kVals, 1024, 16>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p); break;
+    }
 }
 
 template <class K>
@@ -271,27 +336,28 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
 
     int rc = set_device(ctx);
     if (rc) return rc;
-    uint32_t tiles = (uint32_t)((n + (uint64_t)kSortTile - 1) / kSortTile);
+    uint32_t tiles = sort_tiles(ctx, n);
     if ((rc = ensure_status(ctx, tiles))) return rc;
 
     if (!histReady) {
-        k_sort_init<<<1, 256, 0, stream>>>(ctx->hist, live, ctx->tickets);
-        k_hist<K><<<hist_grid(ctx, n), kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->hist);
+        int hg = hist_grid(ctx, n);
+        if ((rc = ensure_partials(ctx, hg, live))) return rc;
+        k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, live, ctx->hist, ctx->status, ctx->statusWords);
+        k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials);
+        k_hist_reduce<<<live, 256, 0, stream>>>(ctx->partials, hg, live, ctx->hist);
     }
     K* src = keys;
     K* dst = ktmp;
     uint32_t* vsrc = vals;
     uint32_t* vdst = vtmp;
     for (int p = 0; p < live; p++) {
-        if ((rc = bump_epoch(ctx, stream))) return rc;
         int shift = startBit + 8 * p;
         if (vals)
-            k_pass<K, true><<<tiles, kSortBlock, 0, stream>>>(src, dst, vsrc, vdst, n, shift, ctx->hist + 256 * p,
-                                                             ctx->status, ctx->tickets + p, ctx->epoch);
+            dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, src, dst, vsrc, vdst, n, shift,
+                                   ctx->hist + 256 * p, ctx->status, ctx->ctrl, p);
         else
-            k_pass<K, false><<<tiles, kSortBlock, 0, stream>>>(src, dst, nullptr, nullptr, n, shift,
-                                                              ctx->hist + 256 * p, ctx->status, ctx->tickets + p,
-                                                              ctx->epoch);
+            dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, src, dst, nullptr, nullptr, n, shift,
+                                    ctx->hist + 256 * p, ctx->status, ctx->ctrl, p);
         K* tk = src;
         src = dst;
         dst = tk;
@@ -316,6 +382,11 @@ int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, 
                      bool histReady, hipStream_t stream)
 {
     return sort_impl<uint64_t>(ctx, keys, nullptr, tmp, nullptr, n, startBit, endBit, histReady, stream);
+}
+
+void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream)
+{
+    k_hist_reduce<<<nPass, 256, 0, stream>>>(ctx->partials, blocks, nPass, ctx->hist);
 }
 
 }  // namespace mlbvh

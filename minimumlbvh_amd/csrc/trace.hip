@@ -184,6 +184,60 @@ __global__ void __launch_bounds__(kTraceBlock)
     }
 }
 
+__global__ void k_queue_reset(uint32_t* counter) { *counter = 0; }
+
+// Persistent work-queue trace over the meshes of a batched build.  Each wave
+// pulls `chunk` consecutive rays per atomic and traces chunk/64 of them per
+// lane; a ray names its mesh in the Ray pad word (u32).  Hits carry
+// mesh-local triangle indices.
+template <bool kAny, bool kCount>
+__global__ void __launch_bounds__(kTraceBlock)
+    k_trace_batched(const Node* __restrict__ nodesAll, const Tri* __restrict__ trisAll,
+                    const uint32_t* __restrict__ off, const uint32_t* __restrict__ roots, int nMeshes,
+                    const float4* __restrict__ rays, HitRec* __restrict__ hits, uint32_t nRays, uint32_t chunk,
+                    uint32_t* queue, unsigned long long* counters)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t nFetch = 0, nTest = 0;
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(queue, chunk);
+        base = __shfl(base, 0, 64);
+        if (base >= nRays) break;
+        for (uint32_t k = 0; k < chunk; k += 64) {
+            uint32_t i = base + k + lane;
+            if (i >= nRays) break;
+            float4 a = rays[2 * i], b = rays[2 * i + 1];
+            uint32_t m = __float_as_uint(b.w);
+            Hit h;
+            h.t = a.w;
+            h.u = h.v = 0.0f;
+            h.ng = mk(0, 0, 0);
+            h.tri = 0xFFFFFFFFu;
+            if (m < (uint32_t)nMeshes) {
+                uint32_t b0 = off[m];
+                f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
+                traverse<kAny, kCount>(nodesAll + (b0 - m), trisAll + b0, roots[m], ro, rd, inv_rd(rd), h, nFetch,
+                                       nTest);
+            }
+            HitRec r;
+            r.t = h.t;
+            r.pad = 0;
+            r.u = h.u;
+            r.v = h.v;
+            r.ngx = h.ng.x;
+            r.ngy = h.ng.y;
+            r.ngz = h.ng.z;
+            r.tri = h.tri;
+            hits[i] = r;
+        }
+    }
+    if (kCount) {
+        wave_add_u64(&counters[0], nFetch);
+        wave_add_u64(&counters[1], nTest);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Shading helpers of main_gpu.cu:17-82
 // ---------------------------------------------------------------------------
@@ -510,6 +564,73 @@ extern "C" int mlbvh_trace(mlbvh_ctx* ctx, const void* d_nodes, const void* d_tr
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? MLBVH_OK : hip_status(e, "trace launch");
+}
+
+extern "C" int mlbvh_trace_batched(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,
+                                   const uint32_t* d_triOffsets, const uint32_t* d_roots, int nMeshes,
+                                   const float* d_rays, void* d_hits, int nRays, int rayQuery,
+                                   unsigned long long* d_counters, void* stream)
+{
+    int rc = check_scene(ctx, d_roots, d_nodes, d_triangles, "trace_batched");
+    if (rc) return rc;
+    if (!d_triOffsets || nMeshes < 1 || nRays < 0 || (nRays > 0 && (!d_rays || !d_hits)) ||
+        (rayQuery != MLBVH_RAY_QUERY_CLOSEST && rayQuery != MLBVH_RAY_QUERY_ANY)) {
+        set_error("trace_batched: invalid arguments");
+        return MLBVH_E_INVALID_ARG;
+    }
+    if (nRays == 0) return MLBVH_OK;
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* queue = ctx->ctrl + kCtrlWorkQueue;
+    uint32_t chunk = (uint32_t)(ctx->traceChunk < 64 ? 64 : ctx->traceChunk / 64 * 64);
+    unsigned grid = (unsigned)(ctx->cuCount * (ctx->traceBlocksPerCU < 1 ? 1 : ctx->traceBlocksPerCU));
+    uint64_t waves = ((uint64_t)nRays + chunk - 1) / chunk;
+    if ((uint64_t)grid * (kTraceBlock / 64) > waves) grid = (unsigned)((waves + 3) / 4);
+    k_queue_reset<<<1, 1, 0, s>>>(queue);
+    const Node* nodes = (const Node*)d_nodes;
+    const Tri* tris = (const Tri*)d_triangles;
+    const float4* rays = (const float4*)d_rays;
+    HitRec* hits = (HitRec*)d_hits;
+    uint32_t n = (uint32_t)nRays;
+    bool any = rayQuery == MLBVH_RAY_QUERY_ANY;
+#define LAUNCH(A, C)                                                                                             \
+    k_trace_batched<A, C><<<grid, kTraceBlock, 0, s>>>(nodes, tris, d_triOffsets, d_roots, nMeshes, rays, hits, n, \
+                                                       chunk, queue, d_counters)
+    if (d_counters) {
+        if (any) LAUNCH(true, true);
+        else LAUNCH(false, true);
+    } else {
+        if (any) LAUNCH(true, false);
+        else LAUNCH(false, false);
+    }
+#undef LAUNCH
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, "trace_batched launch");
+}
+
+extern "C" int mlbvh_set_tuning(mlbvh_ctx* ctx, int sortConfig, int traceBlocksPerCU, int traceChunk)
+{
+    if (!ctx || sortConfig < 0 || sortConfig >= kNumSortCfgs || traceBlocksPerCU < 1 || traceBlocksPerCU > 64 ||
+        traceChunk < 64 || traceChunk > 4096 || traceChunk % 64) {
+        set_error("set_tuning: sortConfig in [0,%d), blocksPerCU in [1,64], chunk a multiple of 64 in [64,4096]",
+                  kNumSortCfgs);
+        return MLBVH_E_INVALID_ARG;
+    }
+    ctx->sortConfig = sortConfig;
+    ctx->traceBlocksPerCU = traceBlocksPerCU;
+    ctx->traceChunk = traceChunk;
+    return MLBVH_OK;
+}
+
+extern "C" int mlbvh_get_tuning(mlbvh_ctx* ctx, int* sortConfig, int* traceBlocksPerCU, int* traceChunk)
+{
+    if (!ctx || !sortConfig || !traceBlocksPerCU || !traceChunk) {
+        set_error("get_tuning: null argument");
+        return MLBVH_E_INVALID_ARG;
+    }
+    *sortConfig = ctx->sortConfig;
+    *traceBlocksPerCU = ctx->traceBlocksPerCU;
+    *traceChunk = ctx->traceChunk;
+    return MLBVH_OK;
 }
 
 extern "C" int mlbvh_render_normal(mlbvh_ctx* ctx, uint32_t* d_pixels, int width, int height,

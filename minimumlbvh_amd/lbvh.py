@@ -173,6 +173,65 @@ def trace(builder_or_nodes, triangles: torch.Tensor, rays: torch.Tensor, query: 
     return hits
 
 
+class BatchedBVH:
+    """Many meshes built in one pipeline (mlbvh_build_batched).  ``tri_offsets``
+    is the (nMeshes+1,) prefix of triangle counts; mesh m's nodes are rows
+    ``[off[m]-m, off[m+1]-m-1)`` of ``nodes`` and its root is ``roots[m]``."""
+
+    def __init__(self, tri_offsets, device: int = 0):
+        import numpy as np
+
+        self.ctx = context(device)
+        off = np.asarray(tri_offsets, dtype=np.uint32)
+        if off.ndim != 1 or off.shape[0] < 2 or off[0] != 0 or np.any(np.diff(off.astype(np.int64)) < 2):
+            raise ValueError("tri_offsets must start at 0 and give every mesh >= 2 triangles")
+        self.h_off = off
+        self.n_meshes = off.shape[0] - 1
+        self.total = int(off[-1])
+        self.max_tris = int(np.diff(off.astype(np.int64)).max())
+        dev = f"cuda:{device}"
+        self.d_off = torch.from_numpy(off.view(np.int32)).to(dev)
+        self.nodes = torch.empty((self.total - self.n_meshes, 16), dtype=torch.int32, device=dev)
+        self.roots = torch.empty(self.n_meshes, dtype=torch.int32, device=dev)
+
+    def build(self, triangles: torch.Tensor, buildOption: int = 0, stream=None) -> None:
+        check(self.ctx.lib.mlbvh_build_batched(self.ctx.handle, _ptr(triangles), _ptr(self.d_off), self.n_meshes,
+                                               self.total, self.max_tris, buildOption, _ptr(self.nodes),
+                                               _ptr(self.roots), _stream(stream)))
+
+    def trace(self, triangles: torch.Tensor, rays: torch.Tensor, query: int = RAY_QUERY_CLOSEST,
+              hits: Optional[torch.Tensor] = None, counters: Optional[torch.Tensor] = None, stream=None):
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 8), dtype=torch.int32, device=rays.device)
+        check(self.ctx.lib.mlbvh_trace_batched(self.ctx.handle, _ptr(self.nodes), _ptr(triangles), _ptr(self.d_off),
+                                               _ptr(self.roots), self.n_meshes, _ptr(rays), _ptr(hits), n, query,
+                                               _ptr(counters), _stream(stream)))
+        return hits
+
+    def build_trace_per_mesh(self, triangles: torch.Tensor, rays: torch.Tensor, ray_offsets,
+                             query: int = RAY_QUERY_CLOSEST, buildOption: int = 0,
+                             hits: Optional[torch.Tensor] = None, stream=None):
+        """The plain path: one build + one trace launch sequence per mesh."""
+        import numpy as np
+
+        roff = np.ascontiguousarray(ray_offsets, dtype=np.uint32)
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 8), dtype=torch.int32, device=rays.device)
+        check(self.ctx.lib.mlbvh_build_trace_per_mesh(
+            self.ctx.handle, _ptr(triangles), self.h_off.ctypes.data_as(ctypes.c_void_p), self.n_meshes, buildOption,
+            _ptr(self.nodes), _ptr(self.roots), _ptr(rays), roff.ctypes.data_as(ctypes.c_void_p), _ptr(hits), query,
+            _stream(stream)))
+        return hits
+
+
+def set_tuning(sort_config: int, blocks_per_cu: int, chunk: int, device: int = 0) -> None:
+    """Launch shapes (see mlbvh_set_tuning); results do not depend on them."""
+    ctx = context(device)
+    check(ctx.lib.mlbvh_set_tuning(ctx.handle, sort_config, blocks_per_cu, chunk))
+
+
 def render_normal(pixels: torch.Tensor, width: int, height: int, rayGenerator: RayGenerator, rootNode, internals,
                   triangles, stream=None, device: int = 0) -> None:
     ctx = context(device)

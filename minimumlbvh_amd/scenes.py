@@ -122,6 +122,44 @@ def pt_scene(n_spheres: int = 64, level: int = 7, seed: int = 0):
     return np.ascontiguousarray(np.concatenate(tris)), np.ascontiguousarray(np.concatenate(attrs))
 
 
+def batch_sizes(n_meshes: int, seed: int = 0, tri_range=(64, 262144), ray_range=(256, 65536)):
+    """Per-mesh triangle and ray counts, log-uniform in the given ranges
+    (4096 meshes with the defaults is the batching workload)."""
+    rng = np.random.default_rng(seed)
+    lt = np.log(np.array(tri_range, np.float64))
+    lr = np.log(np.array(ray_range, np.float64))
+    tris = np.round(np.exp(rng.uniform(lt[0], lt[1], n_meshes))).astype(np.int64)
+    rays = np.round(np.exp(rng.uniform(lr[0], lr[1], n_meshes))).astype(np.int64)
+    return np.clip(tris, 2, None), rays
+
+
+def batch_scene(tri_counts, ray_counts, seed: int = 0, sigma: float = 0.01):
+    """Concatenated independent random-triangle meshes (each like
+    random_triangles) and, per mesh, rays from uniform points on the sphere of
+    radius 2 about the mesh centre (0.5,0.5,0.5) aimed at the centre.  Ray pad
+    word = mesh id.  Returns (tris (T,3,3), tri_offsets (M+1,), rays (R,8),
+    ray_offsets (M+1,))."""
+    rng = np.random.default_rng(seed)
+    tri_counts = np.asarray(tri_counts, np.int64)
+    ray_counts = np.asarray(ray_counts, np.int64)
+    toff = np.concatenate([[0], np.cumsum(tri_counts)]).astype(np.uint32)
+    roff = np.concatenate([[0], np.cumsum(ray_counts)]).astype(np.uint32)
+    T, R = int(toff[-1]), int(roff[-1])
+    c = rng.random((T, 1, 3), dtype=np.float32)
+    tris = (c + rng.normal(0.0, sigma, size=(T, 3, 3)).astype(np.float32)).astype(np.float32)
+    d = rng.normal(size=(R, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    centre = np.array([0.5, 0.5, 0.5])
+    o = centre + 2.0 * d
+    rays = np.zeros((R, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = np.float32(3.402823466e38)
+    rays[:, 4:7] = centre - o
+    mesh_id = np.repeat(np.arange(len(ray_counts), dtype=np.uint32), ray_counts)
+    rays[:, 7] = mesh_id.view(np.float32)
+    return np.ascontiguousarray(tris), toff, rays, roff
+
+
 # Cameras of the workloads: (eye, center, up, fovy_degrees)
 CAMERA_SMALL = ((0.5, 0.5, -1.5), (0.5, 0.5, -0.5), (0.0, 1.0, 0.0), 60.0)
 CAMERA_SPHERE = ((0.0, 0.2, -1.8), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0)

@@ -1,0 +1,84 @@
+"""Batched build + persistent trace (mlbvh_build_batched / mlbvh_trace_batched):
+every mesh's BVH is bit-identical to building it alone (oracle and the plain
+GPU path), and batched hits are bit-identical to the oracle's per-mesh
+traces."""
+import numpy as np
+import pytest
+import torch
+
+from minimumlbvh_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh_nodes(nodes, toff, m):
+    return nodes[toff[m] - m: toff[m + 1] - m - 1]
+
+
+def _run_batched(gpu, tris, toff, rays, opt=0, query=0):
+    bb = gpu.BatchedBVH(toff)
+    t = torch.from_numpy(tris).cuda()
+    bb.build(t, opt)
+    r = torch.from_numpy(rays).cuda()
+    hits = bb.trace(t, r, query)
+    torch.cuda.synchronize()
+    return bb, t, r, bb.nodes.cpu().numpy().view(np.uint32), bb.roots.cpu().numpy().view(np.uint32), \
+        hits.cpu().numpy().view(np.uint32).reshape(-1, 8)
+
+
+@pytest.mark.parametrize("opt", [0, 1])
+def test_batched_build_and_trace_match_oracle(gpu, orc, opt):
+    counts = np.array([2, 3, 257, 4096, 65, 1000, 2, 7777, 300, 12000])
+    rcounts = np.array([100, 50, 400, 1000, 10, 800, 5, 1200, 300, 2000])
+    tris, toff, rays, roff = scenes.batch_scene(counts, rcounts, seed=3)
+    _, _, _, nodes, roots, hits = _run_batched(gpu, tris, toff, rays, opt)
+    for m in range(len(counts)):
+        mt = tris[toff[m]:toff[m + 1]]
+        on, oroot, _, _ = orc.build(mt, opt=opt)
+        assert roots[m] == oroot, m
+        assert np.array_equal(_mesh_nodes(nodes, toff, m), on), m
+        mr = rays[roff[m]:roff[m + 1]].copy()
+        mr[:, 7] = 0
+        oh, _ = orc.trace(on, mt, oroot, mr)
+        assert np.array_equal(hits[roff[m]:roff[m + 1]], oh.view(np.uint32).reshape(-1, 8)), m
+
+
+def test_batched_equals_plain_per_mesh_path(gpu):
+    tc, rc = scenes.batch_sizes(300, seed=5, tri_range=(64, 20000), ray_range=(64, 4000))
+    tris, toff, rays, roff = scenes.batch_scene(tc, rc, seed=5)
+    bb, t, r, nodes, roots, hits = _run_batched(gpu, tris, toff, rays)
+    bb.nodes.zero_()
+    bb.roots.zero_()
+    phits = bb.build_trace_per_mesh(t, r, roff)
+    torch.cuda.synchronize()
+    assert np.array_equal(bb.roots.cpu().numpy().view(np.uint32), roots)
+    assert np.array_equal(bb.nodes.cpu().numpy().view(np.uint32), nodes)
+    assert np.array_equal(phits.cpu().numpy().view(np.uint32).reshape(-1, 8), hits)
+    assert (hits[:, 7] != 0xFFFFFFFF).mean() > 0.05
+
+
+def test_batched_any_hit_and_tuning(gpu, orc):
+    counts = np.array([500, 20, 9000])
+    rcounts = np.array([3000, 40, 5000])
+    tris, toff, rays, roff = scenes.batch_scene(counts, rcounts, seed=8)
+    rays[:, 3] = 1.5
+    ref = None
+    for bpc, chunk in [(8, 64), (2, 256), (16, 128)]:
+        gpu.set_tuning(0, bpc, chunk)
+        _, _, _, _, _, hits = _run_batched(gpu, tris, toff, rays, query=1)
+        if ref is None:
+            ref = hits
+        assert np.array_equal(hits, ref)
+    gpu.set_tuning(0, 8, 64)
+    for m in range(3):
+        mt = tris[toff[m]:toff[m + 1]]
+        on, oroot, _, _ = orc.build(mt)
+        mr = rays[roff[m]:roff[m + 1]].copy()
+        mr[:, 7] = 0
+        oh, _ = orc.trace(on, mt, oroot, mr, query=1)
+        assert np.array_equal(ref[roff[m]:roff[m + 1]], oh.view(np.uint32).reshape(-1, 8))
+
+
+def test_batched_rejects_bad_offsets(gpu):
+    with pytest.raises(ValueError):
+        gpu.BatchedBVH([0, 5, 6])  # a one-triangle mesh

@@ -88,3 +88,15 @@ def test_refit_matches_climb_on_moved_vertices(gpu, orc):
     groot = int(b.m_rootNode.cpu().numpy().view(np.uint32)[0])
     en, eroot = orc.build_from_deltas(deltas, tri_index=order, tris=moved)
     assert groot == eroot and np.array_equal(gn, en)
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
+def test_build_every_sort_config(gpu, orc, cfg):
+    tris = scenes.random_triangles(70_001, 40 + cfg)
+    gpu.set_tuning(cfg, 8, 64)
+    try:
+        gn, groot, _, _ = gpu_build(gpu, tris)
+    finally:
+        gpu.set_tuning(0, 8, 64)
+    on, oroot, _, _ = orc.build(tris)
+    assert groot == oroot and np.array_equal(gn, on)

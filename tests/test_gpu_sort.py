@@ -98,3 +98,25 @@ def test_sort_32m_u64_properties(gpu):
     after = k.cpu().numpy().view(np.uint64)
     assert np.all(after[1:] >= after[:-1])
     assert np.array_equal(after, np.sort(before))
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
+def test_every_tile_config_is_bit_exact(gpu, orc, cfg):
+    """The autotuner may pick any tile shape; all must give the same result."""
+    print(f"sort config {cfg}", flush=True)
+    gpu.set_tuning(cfg, 8, 64)
+    try:
+        for dtype, bits in [(np.uint32, 32), (np.uint64, 64)]:
+            for n in [1, 5000, 300_007]:
+                r = np.random.default_rng(cfg * 7 + n)
+                keys = r.integers(0, np.iinfo(dtype).max, size=n, dtype=dtype, endpoint=True)
+                keys[::9] = keys[0]
+                vals = np.arange(n, dtype=np.uint32)
+                for start, end in [(0, bits), (4, 28)]:
+                    gk, gv = _run(gpu, keys, vals, start, end)
+                    ok, ov, _ = orc.radix_sort(keys, vals, start, end)
+                    assert np.array_equal(gk, ok) and np.array_equal(gv, ov), (dtype, n, start, end)
+                    gk, _ = _run(gpu, keys, None, start, end)
+                    assert np.array_equal(gk, ok)
+    finally:
+        gpu.set_tuning(0, 8, 64)
