@@ -159,6 +159,27 @@ MLBVH_API int mlbvh_build_trace_per_mesh(mlbvh_ctx* ctx, const void* d_triangles
 MLBVH_API int mlbvh_set_tuning(mlbvh_ctx* ctx, int sortConfig, int traceBlocksPerCU, int traceChunk);
 MLBVH_API int mlbvh_get_tuning(mlbvh_ctx* ctx, int* sortConfig, int* traceBlocksPerCU, int* traceChunk);
 
+/* Differentiable primary-hit rendering (no reference counterpart; built on
+ * intersectRayTriangle, minimum_lbvh.h:363-413, and the camera of camera.h).
+ * gather: Triangle records from an indexed mesh (fp32 xyz vertices, 3 u32
+ * indices per triangle). */
+MLBVH_API int mlbvh_gather_triangles(mlbvh_ctx* ctx, const float* d_vertices, const uint32_t* d_indices, int nTris,
+                                     void* d_triangles, void* stream);
+/* render_primary: per pixel (x + y*width) the closest hit of the camera ray
+ * (intersect_stackfree order); d_out = (t, max(dot(n, L), 0)) as 2 floats, n
+ * the geometric normal normalised and turned toward the viewer; (0, 0) on a
+ * miss.  d_hitTri receives the hit triangle (0xFFFFFFFF on a miss). */
+MLBVH_API int mlbvh_render_primary(mlbvh_ctx* ctx, float* d_out, uint32_t* d_hitTri, int width, int height,
+                                   const mlbvh_raygen* camera, const float lightDir[3], const void* d_root,
+                                   const void* d_nodes, const void* d_triangles, void* stream);
+/* primary_adjoint: d_gradVertices += sum over hit pixels of
+ * d_gradOut[p] . d(t, s)/d(vertices of the hit triangle), with the hit
+ * triangle of each pixel held fixed (fp32 atomics; zero the gradient first). */
+MLBVH_API int mlbvh_primary_adjoint(mlbvh_ctx* ctx, const float* d_gradOut, const uint32_t* d_hitTri, int width,
+                                    int height, const mlbvh_raygen* camera, const float lightDir[3],
+                                    const float* d_vertices, const uint32_t* d_indices, float* d_gradVertices,
+                                    void* stream);
+
 /* Host-side RayGenerator::lookat (camera.h:10-22); fovy in radians. */
 MLBVH_API int mlbvh_raygen_lookat(mlbvh_raygen* out, const float eye[3], const float center[3], const float up[3], float fovy,
                         int width, int height);

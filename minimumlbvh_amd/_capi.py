@@ -39,6 +39,9 @@ SIGNATURES = {
     "mlbvh_build_trace_per_mesh": ([_P, _P, _P, c_int, c_uint32, _P, _P, _P, _P, _P, c_int, _P], c_int),
     "mlbvh_set_tuning": ([_P, c_int, c_int, c_int], c_int),
     "mlbvh_get_tuning": ([_P, _P, _P, _P], c_int),
+    "mlbvh_gather_triangles": ([_P, _P, _P, c_int, _P, _P], c_int),
+    "mlbvh_render_primary": ([_P, _P, _P, c_int, c_int, _P, POINTER(c_float), _P, _P, _P, _P], c_int),
+    "mlbvh_primary_adjoint": ([_P, _P, _P, c_int, c_int, _P, POINTER(c_float), _P, _P, _P, _P], c_int),
     "mlbvh_raygen_lookat": ([_P, POINTER(c_float), POINTER(c_float), POINTER(c_float), c_float, c_int, c_int], c_int),
     "mlbvh_render_normal": ([_P, _P, c_int, c_int, _P, _P, _P, _P, _P], c_int),
     "mlbvh_render_ao": ([_P, _P, _P, c_int, c_int, _P, _P, _P, _P, c_int, c_int, c_float, c_uint32, _P, _P], c_int),
@@ -75,7 +78,11 @@ def load(path: str | None = None) -> ctypes.CDLL:
         )
     lib = ctypes.CDLL(p)
     for name, (argtypes, restype) in SIGNATURES.items():
-        fn = getattr(lib, name)
+        # an entry point missing from an older build fails when it is called
+        # (AttributeError); tests/test_capi.py requires all of them
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
         fn.argtypes = argtypes
         fn.restype = restype
     if lib.mlbvh_abi_version() != 1:

@@ -1027,3 +1027,99 @@ void orc_pack(uint32_t* pixels, const float* acc, int n) /* main_gpu.cu:273-286 
                                   powf(a[2] / a[3], 1.0f / 2.2f), 1.0f);
     }
 }
+
+/* ---------------------------------------------------------------------- */
+/* Differentiable primary shading in fp64 (no reference counterpart).      */
+/* ---------------------------------------------------------------------- */
+typedef struct { double x, y, z; } d3;
+static inline d3 D(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static inline d3 dsub(d3 a, d3 b) { return D(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline d3 dadd(d3 a, d3 b) { return D(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline d3 dmul(d3 a, double s) { return D(a.x * s, a.y * s, a.z * s); }
+static inline double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline d3 dcross(d3 a, d3 b) { return D(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+static void pixel_ray64(const orc_raygen* g, int xi, int yi, int w, int h, d3* ro, d3* rd)
+{
+    orc_v3 o, d;
+    orc_raygen_shoot(g, &o, &d, (float)xi / (float)w, (float)yi / (float)h);
+    *ro = D(o.x, o.y, o.z);
+    *rd = D(d.x, d.y, d.z);
+}
+
+static void tri64(const double* v, const uint32_t* idx, uint32_t tri, d3* v0, d3* v1, d3* v2, uint32_t j[3])
+{
+    for (int k = 0; k < 3; k++) j[k] = idx[3 * tri + k];
+    *v0 = D(v[3 * j[0]], v[3 * j[0] + 1], v[3 * j[0] + 2]);
+    *v1 = D(v[3 * j[1]], v[3 * j[1] + 1], v[3 * j[1] + 2]);
+    *v2 = D(v[3 * j[2]], v[3 * j[2] + 1], v[3 * j[2] + 2]);
+}
+
+void orc_primary_eval64(const double* verts, const uint32_t* idx, const uint32_t* hitTri, int w, int h,
+                        const orc_raygen* g, const double* light, double* out)
+{
+    d3 L = D(light[0], light[1], light[2]);
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < w * h; p++) {
+        uint32_t tri = hitTri[p];
+        if (tri == 0xFFFFFFFFu) {
+            out[2 * p] = 0.0;
+            out[2 * p + 1] = 0.0;
+            continue;
+        }
+        d3 ro, rd, v0, v1, v2;
+        uint32_t j[3];
+        pixel_ray64(g, p % w, p / w, w, h, &ro, &rd);
+        tri64(verts, idx, tri, &v0, &v1, &v2, j);
+        d3 ng = dcross(dsub(v1, v0), dsub(v2, v1));
+        double t = ddot(dsub(v0, ro), ng) / ddot(ng, rd);
+        double len = sqrt(ddot(ng, ng));
+        d3 nh = dmul(ng, 1.0 / len);
+        double sgn = ddot(nh, rd) > 0.0 ? -1.0 : 1.0;
+        double s = sgn * ddot(nh, L);
+        out[2 * p] = t;
+        out[2 * p + 1] = s > 0.0 ? s : 0.0;
+    }
+}
+
+void orc_primary_grad64(const double* verts, const uint32_t* idx, const uint32_t* hitTri, int w, int h,
+                        const orc_raygen* g, const double* light, const double* gout, double* grad)
+{
+    d3 L = D(light[0], light[1], light[2]);
+    for (int p = 0; p < w * h; p++) {
+        uint32_t tri = hitTri[p];
+        if (tri == 0xFFFFFFFFu) continue;
+        double gt = gout[2 * p], gs = gout[2 * p + 1];
+        d3 ro, rd, v0, v1, v2;
+        uint32_t j[3];
+        pixel_ray64(g, p % w, p / w, w, h, &ro, &rd);
+        tri64(verts, idx, tri, &v0, &v1, &v2, j);
+        d3 e0 = dsub(v1, v0), e1 = dsub(v2, v1);
+        d3 ng = dcross(e0, e1);
+        double den = ddot(ng, rd);
+        double t = ddot(dsub(v0, ro), ng) / den;
+        /* t: d t = dv0.ng/den + w.dng, w = (v0 - ro - t rd)/den, dng = de0 x e1 + e0 x de1 */
+        d3 wv = dmul(dsub(dsub(v0, ro), dmul(rd, t)), 1.0 / den);
+        d3 e1w = dcross(e1, wv), we0 = dcross(wv, e0);
+        d3 g0 = dmul(dsub(dmul(ng, 1.0 / den), e1w), gt);
+        d3 g1 = dmul(dsub(e1w, we0), gt);
+        d3 g2 = dmul(we0, gt);
+        double len = sqrt(ddot(ng, ng));
+        d3 nh = dmul(ng, 1.0 / len);
+        double sgn = ddot(nh, rd) > 0.0 ? -1.0 : 1.0;
+        if (sgn * ddot(nh, L) > 0.0) {
+            /* s = sgn nh.L: ds = q.dng, q = sgn (L - (nh.L) nh)/|ng| */
+            d3 q = dmul(dsub(L, dmul(nh, ddot(nh, L))), sgn / len);
+            d3 e1q = dcross(e1, q), qe0 = dcross(q, e0);
+            g0 = dsub(g0, dmul(e1q, gs));
+            g1 = dadd(g1, dmul(dsub(e1q, qe0), gs));
+            g2 = dadd(g2, dmul(qe0, gs));
+        }
+        d3 gg[3] = {g0, g1, g2};
+        for (int k = 0; k < 3; k++) {
+            grad[3 * j[k]] += gg[k].x;
+            grad[3 * j[k] + 1] += gg[k].y;
+            grad[3 * j[k] + 2] += gg[k].z;
+        }
+    }
+}

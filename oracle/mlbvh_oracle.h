@@ -123,6 +123,16 @@ void orc_render_pt(float* acc /* float4 per pixel */, int w, int h, const orc_ra
                    int maxSPP, int maxDepth, int nthreads);
 void orc_pack(uint32_t* pixels, const float* acc, int n);
 
+/* Differentiable primary-hit shading, fp64 reference (no reference
+   counterpart).  Per pixel p = x + y*w with its hit triangle hitTri[p] held
+   fixed (0xFFFFFFFF = miss): out[2p] = t = dot(v0-ro, ng)/dot(ng, rd),
+   out[2p+1] = max(sgn dot(ng/|ng|, L), 0) with sgn turning the normal toward
+   the viewer; rays from orc_raygen_shoot.  grad += sum_p gout[p] . d out[p] / dV. */
+void orc_primary_eval64(const double* verts, const uint32_t* idx, const uint32_t* hitTri, int w, int h,
+                        const orc_raygen* g, const double* light, double* out);
+void orc_primary_grad64(const double* verts, const uint32_t* idx, const uint32_t* hitTri, int w, int h,
+                        const orc_raygen* g, const double* light, const double* gout, double* grad);
+
 double orc_now(void);
 int orc_max_threads(void);
 

@@ -78,6 +78,8 @@ def lib():
                                c_int, P, c_int], None),
             "orc_render_pt": ([P, c_int, c_int, P, c_uint32, P, P, P, c_int, c_int, c_int, c_int], None),
             "orc_pack": ([P, P, c_int], None),
+            "orc_primary_eval64": ([P, P, P, c_int, c_int, P, P, P], None),
+            "orc_primary_grad64": ([P, P, P, c_int, c_int, P, P, P, P], None),
             "orc_now": ([], c_double),
             "orc_max_threads": ([], c_int),
         }
@@ -242,6 +244,26 @@ def pack(acc):
     px = np.zeros(n, np.uint32)
     lib().orc_pack(_p(px), _p(acc), n)
     return px
+
+
+def primary_eval64(verts, idx, hit_tri, g, w, h, light):
+    """fp64 (t, shade) per pixel with the hit triangles held fixed."""
+    v = np.ascontiguousarray(verts, np.float64)
+    out = np.zeros((w * h, 2), np.float64)
+    lib().orc_primary_eval64(_p(v), _p(np.ascontiguousarray(idx, np.uint32)),
+                             _p(np.ascontiguousarray(hit_tri, np.uint32)), w, h, ctypes.byref(g),
+                             _p(np.ascontiguousarray(light, np.float64)), _p(out))
+    return out
+
+
+def primary_grad64(verts, idx, hit_tri, g, w, h, light, gout):
+    v = np.ascontiguousarray(verts, np.float64)
+    grad = np.zeros(v.shape, np.float64)
+    lib().orc_primary_grad64(_p(v), _p(np.ascontiguousarray(idx, np.uint32)),
+                             _p(np.ascontiguousarray(hit_tri, np.uint32)), w, h, ctypes.byref(g),
+                             _p(np.ascontiguousarray(light, np.float64)),
+                             _p(np.ascontiguousarray(gout, np.float64)), _p(grad))
+    return grad
 
 
 def now():
