@@ -100,14 +100,20 @@ __global__ void __launch_bounds__(kHistBlock) k_hist(const K* __restrict__ keys,
     for (int p = 0; p < nPass; p++) partials[((size_t)blockIdx.x * nPass + p) * 256 + threadIdx.x] = s_h[p][threadIdx.x];
 }
 
-// hist[p][d] = sum over blocks of partials[block][p][d]; grid = nPass blocks.
+// hist[p][d] += sum over blocks of partials[block][p][d].  Grid (nPass,
+// chunks): each block sums kReduceChunk partial blocks with independent,
+// unrolled loads and adds its sum atomically (hist starts zeroed).
+constexpr int kReduceChunk = 32;
 __global__ void __launch_bounds__(256) k_hist_reduce(const uint32_t* __restrict__ partials, int nBlocks, int nPass,
                                                      uint32_t* __restrict__ hist)
 {
     const int p = blockIdx.x;
+    const int b0 = blockIdx.y * kReduceChunk;
+    const int b1 = min(b0 + kReduceChunk, nBlocks);
     uint32_t s = 0;
-    for (int b = 0; b < nBlocks; b++) s += partials[((size_t)b * nPass + p) * 256 + threadIdx.x];
-    hist[p * 256 + threadIdx.x] = s;
+#pragma unroll 8
+    for (int b = b0; b < b1; b++) s += partials[((size_t)b * nPass + p) * 256 + threadIdx.x];
+    if (s) atomicAdd(&hist[p * 256 + threadIdx.x], s);
 }
 
 template <class K, bool kVals, int kBlock, int kItems>
@@ -344,7 +350,7 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
         if ((rc = ensure_partials(ctx, hg, live))) return rc;
         k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, live, ctx->hist, ctx->status, ctx->statusWords);
         k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials);
-        k_hist_reduce<<<live, 256, 0, stream>>>(ctx->partials, hg, live, ctx->hist);
+        launch_hist_reduce(ctx, hg, live, stream);
     }
     K* src = keys;
     K* dst = ktmp;
@@ -386,7 +392,8 @@ int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, 
 
 void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream)
 {
-    k_hist_reduce<<<nPass, 256, 0, stream>>>(ctx->partials, blocks, nPass, ctx->hist);
+    dim3 grid((unsigned)nPass, (unsigned)((blocks + kReduceChunk - 1) / kReduceChunk), 1);
+    k_hist_reduce<<<grid, 256, 0, stream>>>(ctx->partials, blocks, nPass, ctx->hist);
 }
 
 }  // namespace mlbvh

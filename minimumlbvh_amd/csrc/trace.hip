@@ -89,12 +89,57 @@ struct Hit {
     uint32_t tri;
 };
 
+// One step of the stack-free state machine at an internal node (the body of
+// intersect_stackfree's loop, minimum_lbvh.h:1018-1059), plus a count of the
+// far children still owed on the current path: `pend` rises when both
+// children are entered-able on the way down and falls when the far child is
+// taken on the way back.  Once the walk would climb with nothing owed, every
+// remaining step would only climb to the root, so the caller can stop: the
+// leaves tested, and so the hit, are exactly those of the full walk.
+// hit.t only shrinks, so a far child taken on the way back was always
+// counted on the way down.
+struct StepOut {
+    uint32_t next;
+    bool done;
+};
+__device__ inline StepOut stackfree_step(const NodeView& nd, uint32_t prev, f3 ro, f3 inv, float tHit, uint32_t& pend)
+{
+    float2 rl = slabs(ro, inv, nd.lo0, nd.hi0, tHit);
+    float2 rr = slabs(ro, inv, nd.lo1, nd.hi1, tHit);
+    bool hitL = rl.x <= rl.y;
+    bool hitR = rr.x <= rr.y;
+    uint32_t nearN = nd.c0, farN = nd.c1;
+    int nHits = 0;
+    if (hitL && hitR) {
+        if (rr.x < rl.x) {
+            nearN = nd.c1;
+            farN = nd.c0;
+        }
+        nHits = 2;
+    } else if (hitL || hitR) {
+        nHits = 1;
+        nearN = hitR ? farN : nearN;
+    }
+    uint32_t next;
+    if (prev == nd.parent) {
+        next = 0 < nHits ? nearN : nd.parent;
+        if (nHits == 2) pend++;
+    } else if (prev == nearN && nHits == 2) {
+        next = farN;
+        pend--;
+    } else {
+        next = nd.parent;
+    }
+    return StepOut{next, next == nd.parent && pend == 0};
+}
+
 template <bool kAny, bool kCount>
 __device__ inline void traverse(const Node* __restrict__ nodes, const Tri* __restrict__ tris, uint32_t root, f3 ro,
                                 f3 rd, f3 inv, Hit& hit, uint32_t& nFetch, uint32_t& nTest)
 {
     uint32_t curr = root;
     uint32_t prev = kInvalidNode;
+    uint32_t pend = 0;
     // A well-formed tree visits each node at most three times; the cap only
     // stops a malformed node array from looping forever.
     uint32_t guard = kMaxTraversalSteps;
@@ -119,28 +164,10 @@ __device__ inline void traverse(const Node* __restrict__ nodes, const Tri* __res
         }
         const NodeView nd = load_node(nodes, curr);
         if (kCount) nFetch++;
-        float2 rl = slabs(ro, inv, nd.lo0, nd.hi0, hit.t);
-        float2 rr = slabs(ro, inv, nd.lo1, nd.hi1, hit.t);
-        bool hitL = rl.x <= rl.y;
-        bool hitR = rr.x <= rr.y;
-        uint32_t nearN = nd.c0, farN = nd.c1;
-        int nHits = 0;
-        if (hitL && hitR) {
-            if (rr.x < rl.x) {
-                nearN = nd.c1;
-                farN = nd.c0;
-            }
-            nHits = 2;
-        } else if (hitL || hitR) {
-            nHits = 1;
-            nearN = hitR ? farN : nearN;
-        }
-        uint32_t next;
-        if (prev == nd.parent) next = 0 < nHits ? nearN : nd.parent;
-        else if (prev == nearN) next = nHits == 2 ? farN : nd.parent;
-        else next = nd.parent;
+        StepOut s = stackfree_step(nd, prev, ro, inv, hit.t, pend);
+        if (s.done) break;
         prev = curr;
-        curr = next;
+        curr = s.next;
     }
 }
 
@@ -379,47 +406,99 @@ __global__ void __launch_bounds__(kTraceBlock)
     int xi, yi;
     tile_pixel(&xi, &yi);
     uint32_t nRays = 0, nFetch = 0, nTest = 0;
-    if (xi < w && yi < h) {
-        const uint32_t root = *rootPtr;
-        int pixel = xi + yi * w;
-        f3 ro, rd;
+    const bool inImage = xi < w && yi < h;
+    const uint32_t root = *rootPtr;
+    const int pixel = xi + yi * w;
+
+    // Primary ray: closest hit, exactly the reference's visit order.
+    f3 ro = mk(0, 0, 0), rd = mk(0, 0, 1);
+    Hit hit;
+    hit.t = kFltMax;
+    hit.ng = mk(0, 0, 0);
+    hit.tri = 0xFFFFFFFFu;
+    if (inImage) {
         shoot(cam, &ro, &rd, (float)xi / (float)w, (float)yi / (float)h);
-        Hit hit;
-        hit.t = kFltMax;
-        hit.ng = mk(0, 0, 0);
-        hit.tri = 0xFFFFFFFFu;
         traverse<false, kCount>(nodes, tris, root, ro, rd, inv_rd(rd), hit, nFetch, nTest);
         nRays++;
-        if (hit.t == kFltMax) {
+    }
+    const bool hasHit = inImage && hit.t != kFltMax;
+
+    // AO rays, "while-while": a lane whose ray ends starts its next sample at
+    // once instead of idling until the whole wave has finished that sample.
+    // An any-hit ray's answer (occluded or not) does not depend on the visit
+    // order, so the lanes need not stay in step.
+    f3 n = mk(0, 0, 1), xaxis = mk(1, 0, 0), zaxis = mk(0, 1, 0), org = mk(0, 0, 0);
+    if (hasHit) {
+        n = normalize(hit.ng);
+        if (0.0f < dot(n, rd)) n = -n;
+        onb(n, &xaxis, &zaxis);
+        org = (ro + rd * hit.t) + n * 0.001f;
+    }
+    Pcg rng(hash_pcg((uint32_t)pixel), 0);
+    int sample = 0, occluded = 0;
+    bool active = hasHit && nSample > 0;
+    f3 dir = mk(0, 0, 1), inv = mk(0, 0, 1);
+    uint32_t curr = kInvalidNode, prev = kInvalidNode, pend = 0;
+    auto start_ray = [&](int s) {
+        float r0, r1;
+        if (useSobol) {
+            sobol2d(&r0, &r1, (uint32_t)s, (uint32_t)xi, (uint32_t)yi, seed);
+        } else {
+            r0 = rng.uniformf();
+            r1 = rng.uniformf();
+        }
+        f3 dl = hemisphere_cos(r0, r1);
+        dir = ((xaxis * dl.x) + (zaxis * dl.z)) + (n * dl.y);
+        inv = inv_rd(dir);
+        curr = root;
+        prev = kInvalidNode;
+        pend = 0;
+    };
+    if (active) start_ray(0);
+    uint32_t guard = kMaxTraversalSteps;
+    while (__any(active) && guard-- != 0) {
+        if (!active) continue;
+        bool done = false, blocked = false;
+        if (curr & kLeafBit) {
+            const Tri tri = tris[curr & ~kLeafBit];
+            if (kCount) nTest++;
+            float t, u, v;
+            f3 ng;
+            if (isect_tri(&t, &u, &v, &ng, 0.0f, radius, org, dir, tri.v[0], tri.v[1], tri.v[2])) {
+                done = blocked = true;
+            } else {
+                uint32_t tmp = curr;
+                curr = prev;
+                prev = tmp;
+            }
+        } else {
+            const NodeView nd = load_node(nodes, curr);
+            if (kCount) nFetch++;
+            StepOut s = stackfree_step(nd, prev, org, inv, radius, pend);
+            if (s.done || s.next == kInvalidNode) {
+                done = true;
+            } else {
+                prev = curr;
+                curr = s.next;
+            }
+        }
+        if (done) {
+            nRays++;
+            occluded += blocked ? 1 : 0;
+            if (++sample < nSample) start_ray(sample);
+            else active = false;
+        }
+    }
+
+    if (inImage) {
+        if (!hasHit) {
             pixels[pixel] = pack_rgba(0, 0, 0);
             if (aoOut) aoOut[pixel] = -1.0f;
         } else {
-            f3 n = normalize(hit.ng);
-            if (0.0f < dot(n, rd)) n = -n;
-            f3 xaxis, zaxis;
-            onb(n, &xaxis, &zaxis);
-            f3 org = (ro + rd * hit.t) + n * 0.001f;
-            Pcg rng(hash_pcg((uint32_t)pixel), 0);
-            float ao = 0.0f;
+            // ao += 1/N once per occluded sample, in float, as the reference does
             const float step = 1.0f / (float)nSample;
-            for (int i = 0; i < nSample; i++) {
-                float r0, r1;
-                if (useSobol) {
-                    sobol2d(&r0, &r1, (uint32_t)i, (uint32_t)xi, (uint32_t)yi, seed);
-                } else {
-                    r0 = rng.uniformf();
-                    r1 = rng.uniformf();
-                }
-                f3 dl = hemisphere_cos(r0, r1);
-                f3 dir = ((xaxis * dl.x) + (zaxis * dl.z)) + (n * dl.y);
-                Hit h2;
-                h2.t = radius;
-                h2.ng = mk(0, 0, 0);
-                h2.tri = 0xFFFFFFFFu;
-                traverse<true, kCount>(nodes, tris, root, org, dir, inv_rd(dir), h2, nFetch, nTest);
-                nRays++;
-                if (h2.t != radius) ao += step;
-            }
+            float ao = 0.0f;
+            for (int k = 0; k < occluded; k++) ao += step;
             float brightness = powf(1.0f - ao, 1.0f / 2.2f);
             pixels[pixel] = pack_rgba(brightness, brightness, brightness);
             if (aoOut) aoOut[pixel] = ao;

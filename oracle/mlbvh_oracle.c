@@ -480,13 +480,17 @@ static inline void hit_reset(orc_hit* h, float t)
 }
 
 /* Parent-pointer stack-free traversal, minimum_lbvh.h:981-1061.
-   counters (nullable): [0] += internal-node fetches, [1] += triangle tests. */
-void orc_intersect_stackfree(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root, orc_v3 ro,
-                             orc_v3 rd, orc_v3 inv, int query, uint64_t* counters)
+   counters (nullable): [0] += internal-node fetches, [1] += triangle tests.
+   earlyExit: also track the far children still owed on the current path and
+   stop once the walk would climb with none owed (the remaining steps would
+   only climb to the root); the leaves tested and the hit are unchanged. */
+void orc_intersect_stackfree_ex(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root, orc_v3 ro,
+                                orc_v3 rd, orc_v3 inv, int query, uint64_t* counters, int earlyExit)
 {
     uint32_t curr = root;
     uint32_t prev = ORC_NODE_INVALID;
     uint64_t nFetch = 0, nTest = 0;
+    uint32_t pend = 0;
     while (curr != ORC_NODE_INVALID) {
         if (curr & ORC_LEAF_BIT) {
             float t, u, v;
@@ -531,12 +535,16 @@ void orc_intersect_stackfree(orc_hit* hit, const orc_node* nodes, const orc_tri*
             near_node = hitR ? far_node : near_node;
         }
         uint32_t next;
-        if (prev == parent_node)
+        if (prev == parent_node) {
             next = 0 < nHits ? near_node : parent_node;
-        else if (prev == near_node)
+            if (nHits == 2) pend++;
+        } else if (prev == near_node) {
             next = nHits == 2 ? far_node : parent_node;
-        else
+            if (nHits == 2) pend--;
+        } else {
             next = parent_node;
+        }
+        if (earlyExit && next == parent_node && pend == 0) break;
         prev = curr;
         curr = next;
     }
@@ -544,6 +552,12 @@ void orc_intersect_stackfree(orc_hit* hit, const orc_node* nodes, const orc_tri*
         counters[0] += nFetch;
         counters[1] += nTest;
     }
+}
+
+void orc_intersect_stackfree(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root, orc_v3 ro,
+                             orc_v3 rd, orc_v3 inv, int query, uint64_t* counters)
+{
+    orc_intersect_stackfree_ex(hit, nodes, tris, root, ro, rd, inv, query, counters, 0);
 }
 
 /* Explicit-stack traversal, minimum_lbvh.h:912-973 (closest hit). */
@@ -665,8 +679,8 @@ void orc_trace(const orc_node* nodes, const orc_tri* tris, uint32_t root, const 
             orc_hit h;
             hit_reset(&h, r[3]);
             uint64_t cnt[2] = {0, 0};
-            if (method == 0)
-                orc_intersect_stackfree(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt);
+            if (method == 0 || method == 3)
+                orc_intersect_stackfree_ex(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt, method == 3);
             else if (method == 1)
                 orc_intersect_stackfull(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), stack);
             else
@@ -885,7 +899,7 @@ void orc_render_normal(uint32_t* pixels, int w, int h, const orc_raygen* g, uint
 
 void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_raygen* g, uint32_t root,
                    const orc_node* nodes, const orc_tri* tris, int useSobol, int nSample, float radius, uint32_t seed,
-                   int row0, int row1, uint64_t* counters, int nthreads) /* main_gpu.cu:111-177 */
+                   int row0, int row1, uint64_t* counters, int nthreads, int earlyExit) /* main_gpu.cu:111-177 */
 {
     int nt = pick_threads(nthreads);
     uint64_t cRays = 0, cFetch = 0, cTest = 0;
@@ -900,7 +914,8 @@ void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_rayge
             orc_raygen_shoot(g, &ro, &rd, (float)xi / (float)w, (float)yi / (float)h);
             orc_hit hit;
             hit_reset(&hit, FLT_MAX_H);
-            orc_intersect_stackfree(&hit, nodes, tris, root, ro, rd, orc_inv_rd(rd), ORC_QUERY_CLOSEST, cnt);
+            orc_intersect_stackfree_ex(&hit, nodes, tris, root, ro, rd, orc_inv_rd(rd), ORC_QUERY_CLOSEST, cnt,
+                                       earlyExit);
             cRays++;
             if (hit.t == FLT_MAX_H) {
                 pixels[pixel] = orc_pack_rgba(0, 0, 0, 1);
@@ -929,7 +944,8 @@ void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_rayge
                 orc_v3 dir = vadd(vadd(vmuls(xaxis, dl.x), vmuls(zaxis, dl.z)), vmuls(n, dl.y));
                 orc_hit h2;
                 hit_reset(&h2, radius);
-                orc_intersect_stackfree(&h2, nodes, tris, root, ao_origin, dir, orc_inv_rd(dir), ORC_QUERY_ANY, cnt);
+                orc_intersect_stackfree_ex(&h2, nodes, tris, root, ao_origin, dir, orc_inv_rd(dir), ORC_QUERY_ANY, cnt,
+                                           earlyExit);
                 cRays++;
                 if (h2.t != radius) ao += 1.0f / (float)nSample;
             }

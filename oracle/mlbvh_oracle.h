@@ -94,11 +94,14 @@ int orc_intersect_tri(float* t, float* u, float* v, orc_v3* ng, float tmin, floa
                       orc_v3 ro, orc_v3 rd, orc_v3 v0, orc_v3 v1, orc_v3 v2);
 void orc_intersect_stackfree(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root,
                              orc_v3 ro, orc_v3 rd, orc_v3 inv, int query, uint64_t* counters);
+void orc_intersect_stackfree_ex(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root,
+                                orc_v3 ro, orc_v3 rd, orc_v3 inv, int query, uint64_t* counters, int earlyExit);
 void orc_intersect_stackfull(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root,
                              orc_v3 ro, orc_v3 rd, orc_v3 inv, uint32_t* stack);
 void orc_intersect_escape_link(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root,
                                orc_v3 ro, orc_v3 rd, orc_v3 inv, int query);
-/* rays: 8 floats each (ox,oy,oz,tmax,dx,dy,dz,pad); method 0 stackfree, 1 stackfull, 2 escape link */
+/* rays: 8 floats each (ox,oy,oz,tmax,dx,dy,dz,pad); method 0 stackfree, 1 stackfull, 2 escape link,
+   3 stackfree with the pending-count early exit */
 void orc_trace(const orc_node* nodes, const orc_tri* tris, uint32_t root, const float* rays, int nRays,
                int query, int method, orc_hit* hits, uint64_t* counters, int nthreads);
 void orc_brute_force(const orc_tri* tris, int n, const float* rays, int nRays, orc_hit* hits);
@@ -117,7 +120,7 @@ void orc_render_normal(uint32_t* pixels, int w, int h, const orc_raygen* g, uint
    rows [row0,row1) only (for bounded CPU baselines); counters: rays traced, node fetches, tri tests */
 void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_raygen* g, uint32_t root,
                    const orc_node* nodes, const orc_tri* tris, int useSobol, int nSample, float radius,
-                   uint32_t seed, int row0, int row1, uint64_t* counters, int nthreads);
+                   uint32_t seed, int row0, int row1, uint64_t* counters, int nthreads, int earlyExit);
 void orc_render_pt(float* acc /* float4 per pixel */, int w, int h, const orc_raygen* g, uint32_t root,
                    const orc_node* nodes, const orc_tri* tris, const orc_attrib* attribs, int useSobol,
                    int maxSPP, int maxDepth, int nthreads);

@@ -75,7 +75,7 @@ def lib():
             "orc_pack_rgba": ([c_float, c_float, c_float, c_float], c_uint32),
             "orc_render_normal": ([P, c_int, c_int, P, c_uint32, P, P, c_int], None),
             "orc_render_ao": ([P, P, c_int, c_int, P, c_uint32, P, P, c_int, c_int, c_float, c_uint32, c_int,
-                               c_int, P, c_int], None),
+                               c_int, P, c_int, c_int], None),
             "orc_render_pt": ([P, c_int, c_int, P, c_uint32, P, P, P, c_int, c_int, c_int, c_int], None),
             "orc_pack": ([P, P, c_int], None),
             "orc_primary_eval64": ([P, P, P, c_int, c_int, P, P, P], None),
@@ -221,13 +221,13 @@ def render_normal(g, w, h, root, nodes, tris, nthreads=0):
 
 
 def render_ao(g, w, h, root, nodes, tris, use_sobol=1, n_sample=16, radius=1.0, seed=12345, row0=0, row1=None,
-              nthreads=0):
+              nthreads=0, early_exit=False):
     px = np.zeros(w * h, np.uint32)
     ao = np.full(w * h, np.nan, np.float32)
     cnt = np.zeros(3, np.uint64)
     lib().orc_render_ao(_p(px), _p(ao), w, h, ctypes.byref(g), root, _p(np.ascontiguousarray(nodes, np.uint32)),
                         _p(_f32(tris)), use_sobol, n_sample, radius, seed & 0xFFFFFFFF, row0,
-                        h if row1 is None else row1, _p(cnt), nthreads)
+                        h if row1 is None else row1, _p(cnt), nthreads, 1 if early_exit else 0)
     return px, ao, cnt
 
 

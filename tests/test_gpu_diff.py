@@ -75,8 +75,15 @@ def test_adjoint_matches_fp64_gradient(gpu, orc):
     ref = orc.primary_grad64(v, f, hit, _oracle_cam(orc, w, h), w, h, LIGHT, gout.cpu().numpy().reshape(-1, 2))
     gg = grad.cpu().numpy().astype(np.float64)
     assert np.linalg.norm(gg - ref) <= 1e-4 * np.linalg.norm(ref)
-    big = np.abs(ref) > 1e-3 * np.abs(ref).max()
-    assert np.all(np.abs(gg[big] - ref[big]) <= 1e-3 * np.abs(ref[big]) + 1e-6 * np.abs(ref).max())
+    # per coordinate: fp32 atomics accumulate tens of contributions whose
+    # magnitudes reach max|ref|, so a cancelling sum keeps an absolute error
+    # of order (#terms) x eps32 x max|ref|
+    err = np.abs(gg - ref)
+    tol = 1e-3 * np.abs(ref) + 2e-5 * np.abs(ref).max()
+    worst = np.unravel_index(np.argmax(err - tol), err.shape)
+    print(f"adjoint vs fp64: norm rel {np.linalg.norm(gg - ref) / np.linalg.norm(ref):.2e}, worst {worst} "
+          f"{gg[worst]:.6g} vs {ref[worst]:.6g}")
+    assert np.all(err <= tol)
 
 
 def test_autograd_refit_and_rebuild(gpu, orc):

@@ -74,3 +74,19 @@ def test_counters_count_fetches(orc):
     _, c = orc.trace(nodes, tris, root, rays)
     assert np.array_equal(c, c1 + c2)
     assert c[0] >= 1000  # at least the root per ray
+
+
+@pytest.mark.parametrize("query,tmax", [(0, None), (1, 0.2)])
+def test_early_exit_is_exact(orc, query, tmax):
+    """The pending-count early exit (trace method 3) stops only where the
+    reference walk would just climb to the root: identical hits, fewer
+    fetches."""
+    for n, seed in [(3, 1), (500, 2), (65536, 3)]:
+        tris = scenes.random_triangles(n, seed, sigma=0.02)
+        nodes, root, _, _ = orc.build(tris)
+        rays = random_rays(20000, seed + 10, tmax=tmax)
+        ref, cref = orc.trace(nodes, tris, root, rays, query=query, method=0)
+        ee, cee = orc.trace(nodes, tris, root, rays, query=query, method=3)
+        assert np.array_equal(ee.view(np.uint32), ref.view(np.uint32))
+        assert cee[0] <= cref[0] and cee[1] == cref[1]
+    assert cee[0] < cref[0]
