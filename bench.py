@@ -7,16 +7,20 @@ the GPU builder, then ambient occlusion at 1920x1080 with 1 primary + 16 AO
 rays per hit pixel (ray length 0.2, Owen-scrambled Sobol seed 0).
 
 A step is one AO frame (primary + AO rays) over the resident BVH; `value` is
-rays traced per second over the K timed steps.  The same run also times the
-full BVH build of that mesh (Mtris/s) and the standalone sort of 2^25 uniform
-u64 keys, each against the 8 TB/s HBM roofline, and the CPU oracle (build +
-a band of the same AO frame) on this host's cores.
+rays traced per second over the K timed steps.  The same run also times:
+  * the full BVH build of that mesh (Mtris/s) and the standalone sort of 2^25
+    uniform u64 keys, each against the 8 TB/s HBM roofline;
+  * the 4096-mesh batched build + persistent trace, eager and replayed from a
+    captured graph, against the plain per-mesh launch loop;
+  * 200 Adam steps of the differentiable primary render (level-7 sphere,
+    512x512) against its forward-only time;
+  * the CPU oracle (full build + the whole AO frame) on this host's cores.
+Workload set-up lives in minimumlbvh_amd/workloads.py.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-diff]
 """
 import argparse
 import json
-import math
 import os
 import platform
 import sys
@@ -71,6 +75,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--level", type=int, default=9, help="icosphere subdivision level (9 = 5,242,880 tris)")
     ap.add_argument("--sort-log2", type=int, default=25, help="standalone sort size = 2^k u64 keys")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 4096-mesh batched leg")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this path has no multi-GPU "
@@ -80,54 +86,39 @@ def main():
     import numpy as np
     import torch
 
-    from minimumlbvh_amd import autotune, lbvh, scenes
+    from minimumlbvh_amd import autotune, workloads
+    from minimumlbvh_amd.workloads import AO_H as H, AO_RADIUS as RADIUS, AO_SEED as SEED, AO_SPP as SPP, \
+        AO_W as W
 
     dev = torch.device("cuda:0")
     log("[bench] autotuning the sort tile")
     best_cfg, cfg_ms = autotune.tune_sort(1 << 24, 64)
     log(f"[bench] sort tile ms by config: {cfg_ms} -> {best_cfg}")
-    W, H, SPP, RADIUS, SEED = 1920, 1080, 16, 0.2, 0
     log(f"[bench] generating icosphere level {args.level}")
-    verts, faces = scenes.displaced_icosphere(args.level, seed=0)
-    tris_np = scenes.deindex(verts, faces)
-    n = tris_np.shape[0]
-    tris = torch.from_numpy(tris_np).to(dev)
+    sc = workloads.AOScene(args.level)
+    n = sc.n
     stream = torch.cuda.current_stream()
-
-    sorter = lbvh.OnesweepSort()
-    builder = lbvh.BVHGPUBuilder()
     log(f"[bench] build n={n}")
 
-    def build_step():
-        builder.build(tris, n, 0, sorter, stream)
-
-    build_wall, build_ev = timed(build_step, max(3, args.steps), 2)
+    build_wall, build_ev = timed(lambda: sc.build(stream), max(3, args.steps), 2)
     build_mtris = n / (build_ev / 1e3) / 1e6
     build_gbs = BUILD_BYTES_PER_TRI * n / (build_ev / 1e3) / 1e9
 
-    eye, ctr, up, fov = scenes.CAMERA_SPHERE
-    cam = lbvh.RayGenerator().lookat(eye, ctr, up, math.radians(fov), W, H)
-    px = torch.zeros(W * H, dtype=torch.int32, device=dev)
     cnt = torch.zeros(3, dtype=torch.int64, device=dev)
-    lbvh.render_ao(px, W, H, cam, builder.m_rootNode, builder.m_internals, tris, 1, SPP, RADIUS, SEED,
-                   counters=cnt)
+    sc.frame(counters=cnt)
     torch.cuda.synchronize()
     rays, fetches, tests = (int(x) for x in cnt.cpu().numpy().view(np.uint64))
     ao_bytes = NODE_BYTES * fetches + TRI_BYTES * tests
 
-    def ao_step():
-        lbvh.render_ao(px, W, H, cam, builder.m_rootNode, builder.m_internals, tris, 1, SPP, RADIUS, SEED)
-
     log(f"[bench] AO frame: {rays} rays, {fetches} node fetches, {tests} triangle tests")
-    ao_wall, ao_ev = timed(ao_step, args.steps, args.warmup)
+    ao_wall, ao_ev = timed(lambda: sc.frame(stream=stream), args.steps, args.warmup)
     mrays = rays / (ao_wall / 1e3) / 1e6
     ao_gbs = ao_bytes / (ao_ev / 1e3) / 1e9
 
     # standalone sort of 2^k uniform u64 keys, 8 passes of 8-bit digits
     N = 1 << args.sort_log2
     log(f"[bench] sort N={N}")
-    g = torch.Generator(device=dev).manual_seed(0)
-    keys0 = torch.randint(-(2**63), 2**63 - 1, (N,), dtype=torch.int64, device=dev, generator=g)
+    keys0 = workloads.random_u64(N)
     keys = torch.empty_like(keys0)
     tmp = torch.empty_like(keys0)
 
@@ -136,7 +127,7 @@ def main():
         keys.copy_(keys0)
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
-        sorter.sort(keys, None, tmp, None, N, 0, 64, stream)
+        sc.sorter.sort(keys, None, tmp, None, N, 0, 64, stream)
         e.record()
         torch.cuda.synchronize()
         if i >= args.warmup:
@@ -146,6 +137,16 @@ def main():
     sort_bytes = (2 * P + 1) * 8 * N
     sort_gbs = sort_bytes / (sort_avg / 1e3) / 1e9
     del keys0, keys, tmp
+
+    batched = {}
+    if not args.no_batched:
+        log("[bench] batched build + trace (4096 meshes)")
+        workloads.run_batched(max(args.steps, 3), batched)
+        torch.cuda.empty_cache()
+    diffres = {}
+    if not args.no_diff:
+        log("[bench] differentiable render: 200 Adam steps")
+        workloads.run_diff(200, diffres)
 
     out = {
         "metric": METRIC,
@@ -201,6 +202,8 @@ def main():
             "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
             "tile": "x".join(str(v) for v in autotune.SORT_CONFIGS[best_cfg]),
         },
+        "batched": batched or None,
+        "diff": diffres or None,
         "tuning": autotune.get_tuning(),
     }
 
@@ -210,20 +213,19 @@ def main():
         threads = orc.max_threads()
         log(f"[bench] CPU oracle baseline on {threads} threads")
         t0 = orc.now()
-        nodes, root, _, _ = orc.build(tris_np)
+        nodes, root, _, _ = orc.build(sc.tris_np)
         cpu_build_s = orc.now() - t0
-        og = orc.raygen_lookat(eye, ctr, up, math.radians(fov), W, H)
-        r0, r1 = H // 2 - 135, H // 2 + 135
+        og = orc.raygen_lookat(sc.eye, sc.ctr, sc.up, sc.fovy, W, H)
         t0 = orc.now()
-        _, _, ccnt = orc.render_ao(og, W, H, root, nodes, tris_np, 1, SPP, RADIUS, SEED, row0=r0, row1=r1)
+        _, _, ccnt = orc.render_ao(og, W, H, root, nodes, sc.tris_np, 1, SPP, RADIUS, SEED)
         cpu_ao_s = orc.now() - t0
         out["cpu_baseline"] = {
             "value": round(int(ccnt[0]) / cpu_ao_s / 1e6, 3),
             "unit": "MRays/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"rows {r0}-{r1} of the same AO frame ({int(ccnt[0])} rays, {cpu_ao_s:.2f} s), oracle "
-                      f"stack-free traversal, OpenMP rows; plus one full oracle build of the {n}-tri mesh",
+            "sample": f"the whole AO frame ({int(ccnt[0])} rays, {cpu_ao_s:.2f} s on {threads} threads), oracle "
+                      f"stack-free traversal, OpenMP over rows; plus one full oracle build of the {n}-tri mesh",
             "build_mtris_per_s": round(n / cpu_build_s / 1e6, 2),
             "build_s": round(cpu_build_s, 3),
             "cpu_model": cpu_model(),

@@ -82,3 +82,44 @@ def test_batched_any_hit_and_tuning(gpu, orc):
 def test_batched_rejects_bad_offsets(gpu):
     with pytest.raises(ValueError):
         gpu.BatchedBVH([0, 5, 6])  # a one-triangle mesh
+
+
+def test_batched_build_trace_graph_replays(gpu):
+    """build -> trace captured in one graph: every replay, with eager work of
+    the same library (a standalone sort, a plain build) and clobbered outputs
+    in between, reproduces the eager results bit for bit."""
+    tc, rc = scenes.batch_sizes(64, seed=11, tri_range=(64, 30000), ray_range=(64, 3000))
+    tris, toff, rays, roff = scenes.batch_scene(tc, rc, seed=11)
+    bb, t, r, nodes, roots, hits = _run_batched(gpu, tris, toff, rays)
+    out = torch.empty((rays.shape[0], 8), dtype=torch.int32, device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        bb.build(t, 0, s)
+        bb.trace(t, r, hits=out, stream=s)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        bb.build(t)
+        bb.trace(t, r, hits=out)
+    keys = torch.randint(-(2**63), 2**63 - 1, (100_000,), dtype=torch.int64, device="cuda")
+    keys_np = keys.cpu().numpy().view(np.uint64)
+    ktmp = torch.empty_like(keys)
+    plain = gpu.BVHGPUBuilder()
+    sorter = gpu.OnesweepSort()
+    for rep in range(4):
+        print(f"graph replay {rep}", flush=True)
+        out.fill_(-1)
+        bb.nodes.fill_(-1)
+        bb.roots.fill_(-1)
+        k = keys.clone()
+        sorter.sort(k, None, ktmp, None, k.numel(), 0, 64)
+        plain.build(t[: 5000 + 1000 * rep].contiguous(), 5000 + 1000 * rep, 0, sorter)
+        torch.cuda.synchronize()
+        assert np.array_equal(k.cpu().numpy().view(np.uint64), np.sort(keys_np))
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy().view(np.uint32).reshape(-1, 8), hits), rep
+        assert np.array_equal(bb.nodes.cpu().numpy().view(np.uint32), nodes), rep
+        assert np.array_equal(bb.roots.cpu().numpy().view(np.uint32), roots), rep

@@ -5,8 +5,9 @@ Bars: hit triangles and t bit-identical to the oracle's closest-hit trace;
 shading within 2e-6 of the fp64 reference; the fp32-atomic adjoint within
 1e-4 (norm) / 1e-3 (per coordinate) of the fp64 analytic gradient; and, at
 the full differentiation workload (level-7 sphere, 512x512), the GPU gradient
-of the L2 image loss within 1e-3 (norm over 1,000 random vertices) of fp64
-central differences with eps 1e-4 and the hit triangles held fixed.
+of the L2 image loss within 1e-3 (norm over 1,000 random kink-free vertices)
+of fp64 central differences with eps 1e-4, and within 1e-4 of their
+Richardson extrapolation, with the hit triangles held fixed.
 """
 import math
 
@@ -116,7 +117,17 @@ def test_autograd_refit_and_rebuild(gpu, orc):
 
 def test_fd_1000_vertices_full_workload(gpu, orc):
     """Level-7 displaced sphere (327,680 tris), 512x512, L2 loss against the
-    mesh with an extra seeded displacement of amplitude 0.02."""
+    mesh with an extra seeded displacement of amplitude 0.02.
+
+    Central differences are only meaningful where the loss is smooth within
+    +-eps: pixels on the shading clamp max(n.L, 0) (s below 1e-2, including
+    s == 0 at the terminator) or hit at grazing incidence (|cos| <= 0.05) sit
+    on a kink, and vertices touching such a pixel are left out of the 1,000.
+    On the rest, fp64 central differences with eps 1e-4 must match the GPU
+    fp32 adjoint to 1e-3 (relative L2 over the 1,000 vertices); their
+    Richardson extrapolation (eps 1e-4, 5e-5) removes the eps^2 truncation
+    term and must match the fp64 analytic adjoint to 1e-5 and the GPU adjoint
+    to 1e-4."""
     v, f = scenes.displaced_icosphere(7, seed=0)
     dirs = v / np.linalg.norm(v, axis=1, keepdims=True)
     vt_np = (v * scenes.displacement(dirs, 99, 0.02)[:, None].astype(np.float32)).astype(np.float32)
@@ -134,9 +145,25 @@ def test_fd_1000_vertices_full_workload(gpu, orc):
     tgt = target.cpu().numpy().reshape(-1, 2).astype(np.float64)
     g = _oracle_cam(orc, w, h)
     v64 = v.astype(np.float64)
-    rng = np.random.default_rng(7)
-    touched = np.unique(f[hit[hit != 0xFFFFFFFF]].ravel())
-    pick = rng.choice(touched, size=1000, replace=False)
+    o = orc.primary_eval64(v64, f, hit, g, w, h, LIGHT)
+    ref = orc.primary_grad64(v64, f, hit, g, w, h, LIGHT,
+                             np.stack([2 * depth_w * (o[:, 0] - tgt[:, 0]), 2 * (o[:, 1] - tgt[:, 1])], 1))
+
+    # kink pixels: shading clamp or grazing incidence
+    hm = hit != 0xFFFFFFFF
+    rd = np.zeros((w * h, 3))
+    for p in np.nonzero(hm)[0]:
+        rd[p] = orc.raygen_shoot(g, (p % w) / w, (p // w) / h)[1]
+    tri = f[hit[hm]]
+    ng = np.cross(v64[tri[:, 1]] - v64[tri[:, 0]], v64[tri[:, 2]] - v64[tri[:, 0]])
+    cosg = np.abs(np.einsum("ij,ij->i", ng, rd[hm])) / np.linalg.norm(ng, axis=1) / np.linalg.norm(rd[hm], axis=1)
+    rough = np.zeros(w * h, bool)
+    rough[hm] = (cosg <= 0.05) | (o[hm, 1] <= 1e-2)
+    touched = np.unique(f[hit[hm]].ravel())
+    smooth = np.setdiff1d(touched, np.unique(f[hit[rough]].ravel()))
+    assert smooth.size > 0.8 * touched.size
+    pick = np.random.default_rng(7).choice(smooth, size=1000, replace=False)
+
     tri_of_pixel = hit.astype(np.int64)
     eps = 1e-4
     fd = np.zeros((1000, 3))
@@ -147,8 +174,8 @@ def test_fd_1000_vertices_full_workload(gpu, orc):
         sub = np.where(mask, hit, np.uint32(0xFFFFFFFF)).astype(np.uint32)
 
         def loss64(vv):
-            o = orc.primary_eval64(vv, f, sub, g, w, h, LIGHT)
-            return depth_w * np.sum((o[mask, 0] - tgt[mask, 0]) ** 2) + np.sum((o[mask, 1] - tgt[mask, 1]) ** 2)
+            oo = orc.primary_eval64(vv, f, sub, g, w, h, LIGHT)
+            return depth_w * np.sum((oo[mask, 0] - tgt[mask, 0]) ** 2) + np.sum((oo[mask, 1] - tgt[mask, 1]) ** 2)
 
         for c in range(3):
             for e, dst in ((eps, fd), (eps / 2, fd_half)):
@@ -156,12 +183,15 @@ def test_fd_1000_vertices_full_workload(gpu, orc):
                 vp[j, c] += e
                 vm[j, c] -= e
                 dst[k, c] = (loss64(vp) - loss64(vm)) / (2 * e)
-    gp = grad[pick]
     rich = (4 * fd_half - fd) / 3  # Richardson: removes the eps^2 truncation term
-    rel = np.linalg.norm(gp - fd) / np.linalg.norm(fd)
-    rel_r = np.linalg.norm(gp - rich) / np.linalg.norm(rich)
-    per = np.linalg.norm(gp - rich, axis=1) / np.maximum(np.linalg.norm(rich, axis=1), 1e-12)
-    print(f"FD check, 1000 vertices: relative L2 error {rel:.3e} (central, eps 1e-4), {rel_r:.3e} (Richardson "
-          f"eps 1e-4/5e-5), median per vertex {np.median(per):.3e}")
-    assert rel_r <= 1e-3
-    assert rel <= 5e-3
+    nr = np.linalg.norm(rich)
+    rel_ref = np.linalg.norm(ref[pick] - rich) / nr
+    rel_gpu_r = np.linalg.norm(grad[pick] - rich) / nr
+    rel_gpu = np.linalg.norm(grad[pick] - fd) / np.linalg.norm(fd)
+    per = np.linalg.norm(grad[pick] - rich, axis=1) / np.maximum(np.linalg.norm(rich, axis=1), 1e-12)
+    print(f"FD check, 1000 of {smooth.size} kink-free vertices: GPU vs central eps 1e-4 rel L2 {rel_gpu:.3e}; "
+          f"vs Richardson: fp64 adjoint {rel_ref:.3e}, GPU adjoint {rel_gpu_r:.3e}, median per vertex "
+          f"{np.median(per):.3e}")
+    assert rel_gpu <= 1e-3
+    assert rel_ref <= 1e-5
+    assert rel_gpu_r <= 1e-4
