@@ -18,6 +18,7 @@
 // and dL/dv accumulates g_t dt/dv + g_s ds/dv into the per-vertex gradient
 // with fp32 atomics (9 per hit pixel).
 #include "mlbvh_ctx.h"
+#include "traverse.h"
 
 namespace mlbvh {
 
@@ -37,86 +38,6 @@ __global__ void __launch_bounds__(kDiffBlock)
     tris[f] = t;
 }
 
-// slabs / triangle test / traversal shared with trace.hip through this file's
-// own copy of the closest-hit loop (kept local so diff.hip links on its own).
-__device__ inline float2 d_slabs(f3 ro, f3 inv, f3 lo, f3 hi, float tHit)
-{
-    f3 t0 = (lo - ro) * inv;
-    f3 t1 = (hi - ro) * inv;
-    float rmin = fmaxf(fmaxf(fminf(t0.x, t1.x), fminf(t0.y, t1.y)), fminf(t0.z, t1.z));
-    float rmax = fminf(fminf(fmaxf(t0.x, t1.x), fmaxf(t0.y, t1.y)), fmaxf(t0.z, t1.z)) * 1.00000024f;
-    return make_float2(fmaxf(rmin, 0.0f), fminf(rmax, tHit));
-}
-
-__device__ inline float d_isect_t(float tmin, float tmax, f3 ro, f3 rd, f3 v0, f3 v1, f3 v2, f3* ngOut, bool* ok)
-{
-    f3 e0 = v1 - v0;
-    f3 e1 = v2 - v1;
-    f3 ng = cross(e0, e1);
-    float t = dot(v0 - ro, ng) / dot(ng, rd);
-    *ok = false;
-    if (tmin <= t && t <= tmax) {
-        f3 e2 = v0 - v2;
-        f3 ro2 = ro * 2.0f;
-        float u = dot(cross(e2, (v0 + v2) - ro2), rd);
-        float v = dot(cross(e0, (v1 + v0) - ro2), rd);
-        float w = dot(cross(e1, (v2 + v1) - ro2), rd);
-        if (!(fminf(fminf(u, v), w) < 0.0f && 0.0f < fmaxf(fmaxf(u, v), w))) *ok = true;
-    }
-    *ngOut = ng;
-    return t;
-}
-
-__device__ inline uint32_t d_closest(const Node* __restrict__ nodes, const Tri* __restrict__ tris, uint32_t root,
-                                     f3 ro, f3 rd, float* tHit, f3* ngHit)
-{
-    f3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-    inv = mk(fmaxf(-kFltMax, fminf(inv.x, kFltMax)), fmaxf(-kFltMax, fminf(inv.y, kFltMax)),
-             fmaxf(-kFltMax, fminf(inv.z, kFltMax)));
-    uint32_t curr = root, prev = kInvalidNode, best = 0xFFFFFFFFu;
-    uint32_t guard = 1u << 26;
-    while (curr != kInvalidNode && guard-- != 0) {
-        if (curr & kLeafBit) {
-            const Tri tri = tris[curr & ~kLeafBit];
-            f3 ng;
-            bool ok;
-            float t = d_isect_t(0.0f, *tHit, ro, rd, tri.v[0], tri.v[1], tri.v[2], &ng, &ok);
-            if (ok) {
-                *tHit = t;
-                *ngHit = ng;
-                best = curr & ~kLeafBit;
-            }
-            uint32_t tmp = curr;
-            curr = prev;
-            prev = tmp;
-            continue;
-        }
-        const Node& nd = nodes[curr];
-        float2 rl = d_slabs(ro, inv, nd.box[0].lo, nd.box[0].hi, *tHit);
-        float2 rr = d_slabs(ro, inv, nd.box[1].lo, nd.box[1].hi, *tHit);
-        bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y;
-        uint32_t nearN = nd.child[0], farN = nd.child[1];
-        int nHits = 0;
-        if (hitL && hitR) {
-            if (rr.x < rl.x) {
-                nearN = nd.child[1];
-                farN = nd.child[0];
-            }
-            nHits = 2;
-        } else if (hitL || hitR) {
-            nHits = 1;
-            nearN = hitR ? farN : nearN;
-        }
-        uint32_t next;
-        if (prev == nd.parent) next = 0 < nHits ? nearN : nd.parent;
-        else if (prev == nearN) next = nHits == 2 ? farN : nd.parent;
-        else next = nd.parent;
-        prev = curr;
-        curr = next;
-    }
-    return best;
-}
-
 __global__ void __launch_bounds__(kDiffBlock)
     k_render_primary(float2* __restrict__ out, uint32_t* __restrict__ hitTri, int w, int h, mlbvh_raygen cam,
                      f3 light, const uint32_t* __restrict__ rootPtr, const Node* __restrict__ nodes,
@@ -129,15 +50,20 @@ __global__ void __launch_bounds__(kDiffBlock)
     int p = xi + yi * w;
     f3 ro, rd;
     shoot(cam, &ro, &rd, (float)xi / (float)w, (float)yi / (float)h);
-    float t = kFltMax;
-    f3 ng = mk(0, 0, 0);
-    uint32_t tri = d_closest(nodes, tris, *rootPtr, ro, rd, &t, &ng);
-    hitTri[p] = tri;
-    if (tri == 0xFFFFFFFFu) {
+    Hit hit;
+    hit.t = kFltMax;
+    hit.u = hit.v = 0.0f;
+    hit.ng = mk(0, 0, 0);
+    hit.tri = 0xFFFFFFFFu;
+    uint32_t nFetch = 0, nTest = 0;
+    traverse<false, false>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), hit, nFetch, nTest);
+    const float t = hit.t;
+    hitTri[p] = hit.tri;
+    if (hit.tri == 0xFFFFFFFFu) {
         out[p] = make_float2(0.0f, 0.0f);
         return;
     }
-    f3 n = normalize(ng);
+    f3 n = normalize(hit.ng);
     if (0.0f < dot(n, rd)) n = -n;
     float s = dot(n, light);
     out[p] = make_float2(t, s > 0.0f ? s : 0.0f);

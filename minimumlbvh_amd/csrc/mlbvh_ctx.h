@@ -13,7 +13,7 @@ struct mlbvh_ctx {
     uint32_t* hist = nullptr;  // digit histograms, [8][256]
     uint32_t* partials = nullptr;  // per-block histogram partials, [blocks][passes][256]
     size_t partialsCap = 0;
-    int sortConfig = 0;  // tile shape index into the sort's configuration table
+    int sortConfig = 2;  // tile shape index into the sort's configuration table (2: 512 x 16)
     // Device control words (see kCtrl* below): the pass-tag counter, the
     // per-pass tile tickets and the per-pass tags.  They live in device memory
     // and are advanced by the init kernels, so a captured graph replays with

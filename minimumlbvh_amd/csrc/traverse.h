@@ -1,0 +1,181 @@
+// Ray-BVH traversal core shared by the trace, render and differentiable
+// kernels: node loads, slab and triangle tests, and the stack-free
+// parent-pointer walk (intersect_stackfree, minimum_lbvh.h:981-1061) with an
+// exact early exit.
+//
+// A node visit is four 16-byte loads of the 64-byte InternalNode; the slab
+// test uses the hardware v_min/v_max (identical to the reference's host
+// min/max for the NaN-free values a finite ray produces).
+#pragma once
+
+#include "mlbvh_device.h"
+
+namespace mlbvh {
+
+constexpr uint32_t kMaxTraversalSteps = 1u << 26;
+
+struct NodeView {
+    uint32_t c0, c1, parent;
+    f3 lo0, hi0, lo1, hi1;
+};
+
+__device__ inline NodeView load_node(const Node* nodes, uint32_t i)
+{
+    const uint4* p = reinterpret_cast<const uint4*>(nodes + i);
+    uint4 a = p[0], b = p[1], c = p[2], d = p[3];
+    NodeView v;
+    v.c0 = a.x;
+    v.c1 = a.y;
+    v.lo0 = mk(__uint_as_float(a.z), __uint_as_float(a.w), __uint_as_float(b.x));
+    v.hi0 = mk(__uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w));
+    v.lo1 = mk(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z));
+    v.hi1 = mk(__uint_as_float(c.w), __uint_as_float(d.x), __uint_as_float(d.y));
+    v.parent = d.z;
+    return v;
+}
+
+// slabs(), minimum_lbvh.h:82-96
+__device__ inline float2 slabs(f3 ro, f3 inv, f3 lo, f3 hi, float tHit)
+{
+    f3 t0 = (lo - ro) * inv;
+    f3 t1 = (hi - ro) * inv;
+    float tminx = fminf(t0.x, t1.x), tminy = fminf(t0.y, t1.y), tminz = fminf(t0.z, t1.z);
+    float tmaxx = fmaxf(t0.x, t1.x), tmaxy = fmaxf(t0.y, t1.y), tmaxz = fmaxf(t0.z, t1.z);
+    float rmin = fmaxf(fmaxf(tminx, tminy), tminz);
+    float rmax = fminf(fminf(tmaxx, tmaxy), tmaxz) * 1.00000024f;
+    rmin = fmaxf(rmin, 0.0f);
+    rmax = fminf(rmax, tHit);
+    return make_float2(rmin, rmax);
+}
+
+// invRd(), minimum_lbvh.h:898-901
+__device__ inline f3 inv_rd(f3 rd)
+{
+    f3 r = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    r.x = fmaxf(-kFltMax, fminf(r.x, kFltMax));
+    r.y = fmaxf(-kFltMax, fminf(r.y, kFltMax));
+    r.z = fmaxf(-kFltMax, fminf(r.z, kFltMax));
+    return r;
+}
+
+// intersectRayTriangle(), minimum_lbvh.h:363-413
+__device__ inline bool isect_tri(float* tOut, float* uOut, float* vOut, f3* ngOut, float tmin, float tmax, f3 ro,
+                                 f3 rd, f3 v0, f3 v1, f3 v2)
+{
+    f3 e0 = v1 - v0;
+    f3 e1 = v2 - v1;
+    f3 ng = cross(e0, e1);
+    float t = dot(v0 - ro, ng) / dot(ng, rd);
+    if (tmin <= t && t <= tmax) {
+        f3 e2 = v0 - v2;
+        f3 ro2 = ro * 2.0f;
+        float u = dot(cross(e2, (v0 + v2) - ro2), rd);
+        float v = dot(cross(e0, (v1 + v0) - ro2), rd);
+        float w = dot(cross(e1, (v2 + v1) - ro2), rd);
+        if (fminf(fminf(u, v), w) < 0.0f && 0.0f < fmaxf(fmaxf(u, v), w)) return false;
+        float vol = (u + v) + w;
+        *tOut = t;
+        *uOut = u / vol;
+        *vOut = v / vol;
+        *ngOut = ng;
+        return true;
+    }
+    return false;
+}
+
+struct Hit {
+    float t, u, v;
+    f3 ng;
+    uint32_t tri;
+};
+
+// One step of the stack-free state machine at an internal node (the body of
+// intersect_stackfree's loop, minimum_lbvh.h:1018-1059), plus a count of the
+// far children still owed on the current path: `pend` rises when both
+// children are entered-able on the way down and falls when the far child is
+// taken on the way back.  Once the walk would climb with nothing owed, every
+// remaining step would only climb to the root, so the caller can stop: the
+// leaves tested, and so the hit, are exactly those of the full walk.
+// hit.t only shrinks, so a far child taken on the way back was always
+// counted on the way down.
+struct StepOut {
+    uint32_t next;
+    bool done;
+};
+__device__ inline StepOut stackfree_step(const NodeView& nd, uint32_t prev, f3 ro, f3 inv, float tHit, uint32_t& pend)
+{
+    float2 rl = slabs(ro, inv, nd.lo0, nd.hi0, tHit);
+    float2 rr = slabs(ro, inv, nd.lo1, nd.hi1, tHit);
+    bool hitL = rl.x <= rl.y;
+    bool hitR = rr.x <= rr.y;
+    uint32_t nearN = nd.c0, farN = nd.c1;
+    int nHits = 0;
+    if (hitL && hitR) {
+        if (rr.x < rl.x) {
+            nearN = nd.c1;
+            farN = nd.c0;
+        }
+        nHits = 2;
+    } else if (hitL || hitR) {
+        nHits = 1;
+        nearN = hitR ? farN : nearN;
+    }
+    uint32_t next;
+    if (prev == nd.parent) {
+        next = 0 < nHits ? nearN : nd.parent;
+        if (nHits == 2) pend++;
+    } else if (prev == nearN && nHits == 2) {
+        next = farN;
+        pend--;
+    } else {
+        next = nd.parent;
+    }
+    return StepOut{next, next == nd.parent && pend == 0};
+}
+
+template <bool kAny, bool kCount>
+__device__ inline void traverse(const Node* __restrict__ nodes, const Tri* __restrict__ tris, uint32_t root, f3 ro,
+                                f3 rd, f3 inv, Hit& hit, uint32_t& nFetch, uint32_t& nTest)
+{
+    uint32_t curr = root;
+    uint32_t prev = kInvalidNode;
+    uint32_t pend = 0;
+    // A well-formed tree visits each node at most three times; the cap only
+    // stops a malformed node array from looping forever.
+    uint32_t guard = kMaxTraversalSteps;
+    while (curr != kInvalidNode && guard-- != 0) {
+        if (curr & kLeafBit) {
+            const Tri tri = tris[curr & ~kLeafBit];
+            if (kCount) nTest++;
+            float t, u, v;
+            f3 ng;
+            if (isect_tri(&t, &u, &v, &ng, 0.0f, hit.t, ro, rd, tri.v[0], tri.v[1], tri.v[2])) {
+                hit.t = t;
+                hit.u = u;
+                hit.v = v;
+                hit.ng = ng;
+                hit.tri = curr & ~kLeafBit;
+                if (kAny) break;
+            }
+            uint32_t tmp = curr;
+            curr = prev;
+            prev = tmp;
+            continue;
+        }
+        const NodeView nd = load_node(nodes, curr);
+        if (kCount) nFetch++;
+        StepOut s = stackfree_step(nd, prev, ro, inv, hit.t, pend);
+        if (s.done) break;
+        prev = curr;
+        curr = s.next;
+    }
+}
+
+__device__ inline void wave_add_u64(unsigned long long* dst, uint64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+}  // namespace mlbvh

@@ -21,7 +21,8 @@ Scenes:
     generator (about 1.3e8 triangles, 4.7 GB).
   * diff: level-7 displaced icosphere (327,680 triangles), 512x512, depth +
     n.l shading, L2 loss against the mesh with an extra seeded displacement
-    of amplitude 0.02, Adam lr 1e-3, refit every step, rebuild every 20.
+    of amplitude 0.02 over the pixels that hit in both images, Adam lr 1e-3,
+    refit every step, rebuild every 20.
 """
 from __future__ import annotations
 
@@ -131,8 +132,12 @@ class DiffScene:
         self.depth_w = 0.5
 
     def loss_of(self, out):
-        return self.depth_w * ((out[..., 0] - self.target[..., 0]) ** 2).sum() + \
-            ((out[..., 1] - self.target[..., 1]) ** 2).sum()
+        # pixels that hit in both images: with the hit triangle held fixed the
+        # gradient has no term for a silhouette that moves, so a pixel that
+        # hits in one image only would pull its triangle along the ray
+        both = ((out[..., 0] > 0) & (self.target[..., 0] > 0)).to(out.dtype)
+        return self.depth_w * (both * (out[..., 0] - self.target[..., 0]) ** 2).sum() + \
+            (both * (out[..., 1] - self.target[..., 1]) ** 2).sum()
 
     def forward(self):
         with torch.no_grad():

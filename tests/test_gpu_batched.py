@@ -64,12 +64,12 @@ def test_batched_any_hit_and_tuning(gpu, orc):
     rays[:, 3] = 1.5
     ref = None
     for bpc, chunk in [(8, 64), (2, 256), (16, 128)]:
-        gpu.set_tuning(0, bpc, chunk)
+        gpu.set_tuning(2, bpc, chunk)
         _, _, _, _, _, hits = _run_batched(gpu, tris, toff, rays, query=1)
         if ref is None:
             ref = hits
         assert np.array_equal(hits, ref)
-    gpu.set_tuning(0, 8, 64)
+    gpu.set_tuning(2, 8, 64)
     for m in range(3):
         mt = tris[toff[m]:toff[m + 1]]
         on, oroot, _, _ = orc.build(mt)

@@ -97,6 +97,6 @@ def test_build_every_sort_config(gpu, orc, cfg):
     try:
         gn, groot, _, _ = gpu_build(gpu, tris)
     finally:
-        gpu.set_tuning(0, 8, 64)
+        gpu.set_tuning(2, 8, 64)
     on, oroot, _, _ = orc.build(tris)
     assert groot == oroot and np.array_equal(gn, on)

@@ -119,4 +119,4 @@ def test_every_tile_config_is_bit_exact(gpu, orc, cfg):
                     gk, _ = _run(gpu, keys, None, start, end)
                     assert np.array_equal(gk, ok)
     finally:
-        gpu.set_tuning(0, 8, 64)
+        gpu.set_tuning(2, 8, 64)
