@@ -8,8 +8,11 @@
 //  * the Morton kernel also builds the four 8-bit digit histograms the sort
 //    needs and initialises the climb's arrival words, so the sort starts with
 //    its first reorder pass and no memset of the node array is needed;
-//  * the climb exchanges a separate u32 arrival word per internal node and
-//    writes `context` once, after the merge, with the reference's final value;
+//  * the climb runs in two phases: nodes whose leaves all lie in one block of
+//    1024 sorted leaves are merged in LDS, the rest (about 1% of the nodes of
+//    a surface mesh) through device-scope arrival words, one u32 per internal
+//    node; `context` is written once, after the merge, with the reference's
+//    final value;
 //  * one thread per leaf for all n leaves (the reference launches
 //    ceil((n-1)/256) blocks and drops the last leaf when (n-1) % 256 == 0).
 #include "mlbvh_ctx.h"
@@ -70,7 +73,8 @@ __global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(const Tri* __restric
 // cost otherwise).
 __global__ void __launch_bounds__(kBuildBlock)
     k_morton(const Tri* __restrict__ tris, int n, const uint32_t* __restrict__ sceneBox, uint32_t opt,
-             uint64_t* __restrict__ rec, uint32_t* __restrict__ partials, uint32_t* __restrict__ flags)
+             uint64_t* __restrict__ rec, uint32_t* __restrict__ partials, uint32_t* __restrict__ flags,
+             uint32_t* __restrict__ ctrl)
 {
     __shared__ uint32_t s_h[4][256];
     for (int p = 0; p < 4; p++) s_h[p][threadIdx.x] = 0;
@@ -91,6 +95,7 @@ __global__ void __launch_bounds__(kBuildBlock)
         }
         rec[i] = (uint64_t)morton | ((uint64_t)(uint32_t)i << 32);
         if (i < n - 1) flags[i] = 0xFFFFFFFFu;
+        if (i == 0) ctrl[kCtrlFrontier] = 0;
 #pragma unroll
         for (int p = 0; p < 4; p++) atomicAdd(&s_h[p][(morton >> (8 * p)) & 0xFFu], 1u);
     }
@@ -111,21 +116,184 @@ __global__ void __launch_bounds__(kBuildBlock)
     }
 }
 
-__global__ void __launch_bounds__(kBuildBlock) k_fill_u32(uint32_t* p, int n, uint32_t v)
+// Agglomerative climb (build_lbvh, minimum_lbvh.h:284-361) in two phases.
+//
+// Internal node p (the split between sorted leaves p and p+1) covers the
+// leaves [Lp, Rp] with Lp = 1 + (last q < p with d[q] >= d[p], else -1) and
+// Rp = (first q > p with d[q] > d[p], else n-1): the climb merges across the
+// smaller boundary delta first and goes right on ties.  So p lies entirely
+// inside a block's leaves [b0, b1] iff
+//   (b0 == 0 or max d[b0-1 .. p-1] >= d[p]) and (b1 == n-1 or max d[p+1 .. b1] > d[p]),
+// which both of p's children evaluate alike from a prefix and a suffix
+// maximum of the block's deltas.
+//
+// Phase 1 (k_climb_local): a block of kLocalLeaves threads climbs its leaves
+// through such block-local nodes, handing the sibling bounds over in LDS.  A
+// thread whose next parent is not block-local appends (node, lo, hi) to the
+// frontier list and stops.  Node fields are written with plain stores; only
+// the next kernel reads them.
+//
+// Phase 2 (k_climb_upper): the frontier entries (about 1% of the nodes for
+// surface meshes) climb on through the device-scope arrival words
+// (climb_from).  The result is the same tree as one thread per leaf climbing
+// through device memory all the way.
+constexpr int kLocalLeaves = 1024;
+
+__global__ void __launch_bounds__(kLocalLeaves)
+    k_climb_local(const Tri* __restrict__ tris, uint32_t n, const uint64_t* __restrict__ rec,
+                  const uint8_t* __restrict__ deltas, Node* __restrict__ nodes, uint32_t* __restrict__ frontier,
+                  uint32_t* __restrict__ ctrl, uint32_t* __restrict__ root)
 {
-    int i = blockIdx.x * kBuildBlock + threadIdx.x;
-    if (i < n) p[i] = v;
+    constexpr int kWaves = kLocalLeaves / 64;
+    // Local delta j is d[b0 - 1 + j], j in [0, m] (m = leaves in the block).
+    __shared__ uint8_t s_d[kLocalLeaves + 1];
+    __shared__ uint8_t s_pm[kLocalLeaves];      // s_pm[j] = max s_d[0 .. j]
+    __shared__ uint8_t s_sm[kLocalLeaves + 1];  // s_sm[j] = max s_d[j .. kLocalLeaves]
+    __shared__ uint32_t s_wmax[2][kWaves];
+    __shared__ uint32_t s_flag[kLocalLeaves];
+    __shared__ Box s_box[kLocalLeaves][2];
+    __shared__ uint32_t s_out[kLocalLeaves * 3];
+    __shared__ uint32_t s_nOut, s_base;
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t b0 = blockIdx.x * kLocalLeaves;
+    const uint32_t m = n - b0 < (uint32_t)kLocalLeaves ? n - b0 : (uint32_t)kLocalLeaves;
+    const uint32_t b1 = b0 + m - 1;
+    const uint32_t nInternals = n - 1;
+
+    // deltas that exist and lie in [b0-1, b1]; others read as 0 (they never
+    // decide a comparison: the b0 == 0 / b1 == n-1 cases are tested apart)
+    auto local_delta = [&](uint32_t j) -> uint32_t {
+        uint32_t g = b0 + j;  // global delta index + 1
+        return (j <= m && g >= 1 && g - 1 < nInternals) ? (uint32_t)deltas[g - 1] : 0u;
+    };
+    uint32_t e0 = local_delta((uint32_t)t);      // prefix element j = t
+    uint32_t e1 = local_delta((uint32_t)t + 1);  // suffix element j = t + 1
+    s_d[t] = (uint8_t)e0;
+    if (t == kLocalLeaves - 1) s_d[kLocalLeaves] = (uint8_t)e1;
+    s_flag[t] = 0xFFFFFFFFu;
+    if (t == 0) s_nOut = 0;
+    uint32_t pm = e0, sm = e1;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t up = __shfl_up(pm, o, 64);
+        uint32_t dn = __shfl_down(sm, o, 64);
+        if (lane >= o) pm = pm > up ? pm : up;
+        if (lane + o < 64) sm = sm > dn ? sm : dn;
+    }
+    if (lane == 63) s_wmax[0][wave] = pm;
+    if (lane == 0) s_wmax[1][wave] = sm;
+    __syncthreads();
+    for (int w = 0; w < wave; w++) pm = pm > s_wmax[0][w] ? pm : s_wmax[0][w];
+    for (int w = wave + 1; w < kWaves; w++) sm = sm > s_wmax[1][w] ? sm : s_wmax[1][w];
+    s_pm[t] = (uint8_t)pm;
+    s_sm[t + 1] = (uint8_t)sm;
+    __syncthreads();
+
+    const uint32_t pos = b0 + (uint32_t)t;
+    bool rootHere = false;
+    uint32_t node = 0;
+    if ((uint32_t)t < m) {
+        const uint32_t triIndex = (uint32_t)(rec[pos] >> 32);
+        Box box = tri_box(tris[triIndex]);
+        uint32_t lo = pos, hi = pos;
+        node = triIndex | kLeafBit;
+        while (true) {
+            if (hi - lo >= nInternals) {
+                rootHere = true;
+                break;
+            }
+            uint32_t dL = 0 < lo ? (uint32_t)s_d[lo - b0] : 0xFFFFFFFFu;
+            uint32_t dR = hi < nInternals ? (uint32_t)s_d[hi - b0 + 1] : 0xFFFFFFFFu;
+            int goLeft = dL < dR ? 1 : 0;
+            uint32_t parent = goLeft ? lo - 1 : hi;
+            bool local = goLeft ? lo > b0 : hi < b1;
+            if (local) {
+                uint32_t jp = parent - b0 + 1;
+                uint32_t dp = s_d[jp];
+                local = (b0 == 0 || (uint32_t)s_pm[jp - 1] >= dp) && (b1 == nInternals || (uint32_t)s_sm[jp + 1] > dp);
+            }
+            if (!local) {
+                uint32_t k = atomicAdd(&s_nOut, 1u);
+                s_out[3 * k] = node;
+                s_out[3 * k + 1] = lo;
+                s_out[3 * k + 2] = hi;
+                break;
+            }
+            Node* p = nodes + parent;
+            p->child[goLeft] = node;
+            p->box[goLeft] = box;
+            if (!(node & kLeafBit)) nodes[node].parent = parent;
+            const uint32_t slot = parent - b0;
+            s_box[slot][goLeft] = box;
+            uint32_t idx = __hip_atomic_exchange(&s_flag[slot], goLeft ? hi : lo, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (idx == 0xFFFFFFFFu) break;
+            lo = lo < idx ? lo : idx;
+            hi = hi < idx ? idx : hi;
+            p->context = hi < nInternals ? hi : 0xFFFFFFFFu;
+            node = parent;
+            Box other = s_box[slot][goLeft ^ 1];
+            box.lo = hmin3(box.lo, other.lo);
+            box.hi = hmax3(box.hi, other.hi);
+        }
+    }
+    if (rootHere) {
+        *root = node;
+        nodes[node & ~kLeafBit].parent = kInvalidNode;
+    }
+    __syncthreads();
+    if (t == 0 && s_nOut) s_base = atomicAdd(&ctrl[kCtrlFrontier], s_nOut);
+    __syncthreads();
+    for (uint32_t i = t; i < 3 * s_nOut; i += kLocalLeaves) frontier[3 * (size_t)s_base + i] = s_out[i];
 }
 
-// Agglomerative climb, one thread per sorted leaf (build_lbvh,
-// minimum_lbvh.h:284-361): see climb_leaf in mlbvh_device.h.
 __global__ void __launch_bounds__(kBuildBlock)
-    k_climb(const Tri* __restrict__ tris, int n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
-            Node* nodes, uint32_t* flags, uint32_t* root)
+    k_climb_upper(const Tri* __restrict__ tris, uint32_t n, const uint8_t* __restrict__ deltas, Node* nodes,
+                  uint32_t* flags, const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ ctrl,
+                  uint32_t* root)
+{
+    const uint32_t count = ctrl[kCtrlFrontier];
+    for (uint32_t i = blockIdx.x * kBuildBlock + threadIdx.x; i < count; i += gridDim.x * kBuildBlock) {
+        const uint32_t node = frontier[3 * (size_t)i], lo = frontier[3 * (size_t)i + 1],
+                       hi = frontier[3 * (size_t)i + 2];
+        Box box;
+        if (node & kLeafBit) {
+            box = tri_box(tris[node & ~kLeafBit]);
+        } else {
+            const Node& nd = nodes[node];
+            box.lo = hmin3(nd.box[0].lo, nd.box[1].lo);
+            box.hi = hmax3(nd.box[0].hi, nd.box[1].hi);
+        }
+        climb_from(n, lo, hi, node, box, deltas, nodes, flags, root);
+    }
+}
+
+// Zero the frontier count and open every internal node's arrival word (the
+// build does both in k_morton).
+__global__ void __launch_bounds__(kBuildBlock) k_climb_reset(uint32_t* flags, int nInternals, uint32_t* ctrl)
 {
     int i = blockIdx.x * kBuildBlock + threadIdx.x;
-    if (i >= n) return;
-    climb_leaf(tris, (uint32_t)n, (uint32_t)i, (uint32_t)(rec[i] >> 32), deltas, nodes, flags, root);
+    if (i < nInternals) flags[i] = 0xFFFFFFFFu;
+    if (i == 0) ctrl[kCtrlFrontier] = 0;
+}
+
+static int launch_climb(mlbvh_ctx* ctx, const Tri* tris, int n, const uint64_t* rec, const uint8_t* deltas,
+                        Node* nodes, uint32_t* root, hipStream_t stream)
+{
+    int rc = ensure(&ctx->frontier, &ctx->frontierCap, (size_t)n * 3, "hipMalloc(climb frontier)");
+    if (rc) return rc;
+    unsigned blocks = (unsigned)(((size_t)n + kLocalLeaves - 1) / kLocalLeaves);
+    k_climb_local<<<blocks, kLocalLeaves, 0, stream>>>(tris, (uint32_t)n, rec, deltas, nodes, ctx->frontier,
+                                                        ctx->ctrl, root);
+    // one lane per expected entry (about n/64 for surface meshes), at most 8
+    // blocks per CU; lanes stride over a larger list
+    size_t want = ((size_t)n / 64 + kBuildBlock - 1) / kBuildBlock + 1;
+    unsigned cap = (unsigned)ctx->cuCount * 8;
+    unsigned upper = want < cap ? (unsigned)want : cap;
+    k_climb_upper<<<upper, kBuildBlock, 0, stream>>>(tris, (uint32_t)n, deltas, nodes, ctx->flags, ctx->frontier,
+                                                     ctx->ctrl, root);
+    return MLBVH_OK;
 }
 
 static inline unsigned grid1(size_t n) { return (unsigned)((n + kBuildBlock - 1) / kBuildBlock); }
@@ -160,12 +328,13 @@ int build_one(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_n
     k_scene_aabb<<<gAabb, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox);
     unsigned gMorton = g < (unsigned)ctx->cuCount * 4 ? g : (unsigned)ctx->cuCount * 4;
     if ((rc = ensure_partials(ctx, (int)gMorton, 4))) return rc;
-    k_morton<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials, ctx->flags);
+    k_morton<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials, ctx->flags,
+                                                  ctx->ctrl);
     launch_hist_reduce(ctx, (int)gMorton, 4, stream);
     rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream);
     if (rc) return rc;
     k_deltas<<<g, kBuildBlock, 0, stream>>>(ctx->recA, n, ctx->deltas);
-    k_climb<<<g, kBuildBlock, 0, stream>>>(tris, n, ctx->recA, ctx->deltas, nodes, ctx->flags, root);
+    if ((rc = launch_climb(ctx, tris, n, ctx->recA, ctx->deltas, nodes, root, stream))) return rc;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_status(e, "build kernel launch");
     if (d_sorted) {
@@ -212,9 +381,10 @@ extern "C" int mlbvh_refit(mlbvh_ctx* ctx, const void* d_triangles, int n, const
     if ((rc = ensure(&ctx->flags, &ctx->flagsCap, (size_t)n, "hipMalloc(flags)"))) return rc;
     hipStream_t s = (hipStream_t)stream;
     unsigned g = grid1((size_t)n);
-    k_fill_u32<<<g, kBuildBlock, 0, s>>>(ctx->flags, n - 1, 0xFFFFFFFFu);
-    k_climb<<<g, kBuildBlock, 0, s>>>((const Tri*)d_triangles, n, d_sorted_records, d_deltas, (Node*)d_nodes,
-                                      ctx->flags, (uint32_t*)d_root);
+    k_climb_reset<<<g, kBuildBlock, 0, s>>>(ctx->flags, n - 1, ctx->ctrl);
+    if ((rc = launch_climb(ctx, (const Tri*)d_triangles, n, d_sorted_records, d_deltas, (Node*)d_nodes,
+                           (uint32_t*)d_root, s)))
+        return rc;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_status(e, "refit kernel launch");
     return MLBVH_OK;

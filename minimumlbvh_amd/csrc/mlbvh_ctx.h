@@ -29,6 +29,8 @@ struct mlbvh_ctx {
     size_t deltasCap = 0;
     uint32_t* flags = nullptr;  // per-internal-node arrival word of the climb
     size_t flagsCap = 0;
+    uint32_t* frontier = nullptr;  // (node, lo, hi) where the block-local climb stops, [n][3]
+    size_t frontierCap = 0;
     uint32_t* sceneBox = nullptr;  // 6 order-mapped uint32 (lo xyz, hi xyz)
 
     // --- batched build / trace ------------------------------------------------
@@ -45,6 +47,7 @@ constexpr int kCtrlEpochCounter = 0;
 constexpr int kCtrlTickets = 16;  // [16]
 constexpr int kCtrlEpochs = 32;   // [16]
 constexpr int kCtrlWorkQueue = 48;  // ray counter of the persistent trace
+constexpr int kCtrlFrontier = 49;   // entries of the climb's frontier list
 constexpr int kCtrlWords = 64;
 constexpr int kNumSortCfgs = 5;  // tile shapes of the radix sort (onesweep.hip)
 constexpr uint32_t kEpochLimit = (1u << 30) - 1;  // look-back words keep 30 tag bits

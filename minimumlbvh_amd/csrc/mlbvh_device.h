@@ -182,12 +182,8 @@ __device__ inline void wait_vector_memory()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// Returns true for the thread that completed the root (it writes *root).
-__device__ inline bool climb_leaf(const Tri* __restrict__ tris, uint32_t n, uint32_t pos, uint32_t triIndex,
-                                  const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags, uint32_t* root)
+__device__ inline Box tri_box(const Tri& tri)
 {
-    const uint32_t nInternals = n - 1;
-    Tri tri = tris[triIndex];
     Box box;
     box.lo = mk(kFltMax, kFltMax, kFltMax);
     box.hi = mk(-kFltMax, -kFltMax, -kFltMax);
@@ -196,8 +192,18 @@ __device__ inline bool climb_leaf(const Tri* __restrict__ tris, uint32_t n, uint
         box.lo = hmin3(box.lo, tri.v[k]);
         box.hi = hmax3(box.hi, tri.v[k]);
     }
-    uint32_t lo = pos, hi = pos;
-    uint32_t node = triIndex | kLeafBit;
+    return box;
+}
+
+// Continue the agglomerative climb from `node`, which covers the sorted
+// leaves [lo, hi] and is bounded by `box`, through the device-scope arrival
+// words: at each parent the first of the two children's threads stops and
+// the second merges and goes on.  Returns true for the thread that
+// completed the root (it writes *root).
+__device__ inline bool climb_from(uint32_t n, uint32_t lo, uint32_t hi, uint32_t node, Box box,
+                                  const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags, uint32_t* root)
+{
+    const uint32_t nInternals = n - 1;
     while (hi - lo < nInternals) {
         uint32_t dL = 0 < lo ? (uint32_t)deltas[lo - 1] : 0xFFFFFFFFu;
         uint32_t dR = hi < nInternals ? (uint32_t)deltas[hi] : 0xFFFFFFFFu;
@@ -222,6 +228,13 @@ __device__ inline bool climb_leaf(const Tri* __restrict__ tris, uint32_t n, uint
     *root = node;
     nodes[node & ~kLeafBit].parent = kInvalidNode;
     return true;
+}
+
+// The whole climb of one sorted leaf through the device-scope protocol.
+__device__ inline bool climb_leaf(const Tri* __restrict__ tris, uint32_t n, uint32_t pos, uint32_t triIndex,
+                                  const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags, uint32_t* root)
+{
+    return climb_from(n, pos, pos, triIndex | kLeafBit, tri_box(tris[triIndex]), deltas, nodes, flags, root);
 }
 
 // Error plumbing shared by the entry points.
