@@ -137,7 +137,10 @@ __global__ void __launch_bounds__(kBuildBlock)
 // surface meshes) climb on through the device-scope arrival words
 // (climb_from).  The result is the same tree as one thread per leaf climbing
 // through device memory all the way.
-constexpr int kLocalLeaves = 1024;
+#ifndef MLBVH_CLIMB_LEAVES
+#define MLBVH_CLIMB_LEAVES 1024
+#endif
+constexpr int kLocalLeaves = MLBVH_CLIMB_LEAVES;
 
 __global__ void __launch_bounds__(kLocalLeaves)
     k_climb_local(const Tri* __restrict__ tris, uint32_t n, const uint64_t* __restrict__ rec,

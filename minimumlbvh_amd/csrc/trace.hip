@@ -11,6 +11,10 @@
 namespace mlbvh {
 
 constexpr int kTraceBlock = 256;
+#ifndef MLBVH_AO_REFILL
+#define MLBVH_AO_REFILL 16
+#endif
+constexpr int kAoRefill = MLBVH_AO_REFILL;  // lanes waiting before a wave starts new AO rays
 template <bool kAny, bool kCount>
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace(const Node* __restrict__ nodes, const Tri* __restrict__ tris, const uint32_t* __restrict__ rootPtr,
@@ -256,10 +260,14 @@ __global__ void __launch_bounds__(kTraceBlock)
     }
     const bool hasHit = inImage && hit.t != kFltMax;
 
-    // AO rays, "while-while": a lane whose ray ends starts its next sample at
-    // once instead of idling until the whole wave has finished that sample.
-    // An any-hit ray's answer (occluded or not) does not depend on the visit
-    // order, so the lanes need not stay in step.
+    // AO rays, "while-while": a lane whose ray ends takes its next sample
+    // without waiting for the rest of the wave.  Starting a ray (Sobol point,
+    // cosine-hemisphere direction with its cos/sin, basis change, three
+    // reciprocals) costs a few node steps of instructions and the whole wave
+    // executes it whenever any lane needs it, so lanes are refilled in batches:
+    // once kAoRefill lanes wait, or no lane has a ray in flight.  An any-hit
+    // ray's answer does not depend on when it starts or on the visit order of
+    // the other lanes, so results and counters are those of one ray at a time.
     f3 n = mk(0, 0, 1), xaxis = mk(1, 0, 0), zaxis = mk(0, 1, 0), org = mk(0, 0, 0);
     if (hasHit) {
         n = normalize(hit.ng);
@@ -269,28 +277,35 @@ __global__ void __launch_bounds__(kTraceBlock)
     }
     Pcg rng(hash_pcg((uint32_t)pixel), 0);
     int sample = 0, occluded = 0;
-    bool active = hasHit && nSample > 0;
+    bool live = false;                         // a ray is in flight
+    bool waiting = hasHit && 0 < nSample;      // the next sample is due
     f3 dir = mk(0, 0, 1), inv = mk(0, 0, 1);
     uint32_t curr = kInvalidNode, prev = kInvalidNode, pend = 0;
-    auto start_ray = [&](int s) {
-        float r0, r1;
-        if (useSobol) {
-            sobol2d(&r0, &r1, (uint32_t)s, (uint32_t)xi, (uint32_t)yi, seed);
-        } else {
-            r0 = rng.uniformf();
-            r1 = rng.uniformf();
-        }
-        f3 dl = hemisphere_cos(r0, r1);
-        dir = ((xaxis * dl.x) + (zaxis * dl.z)) + (n * dl.y);
-        inv = inv_rd(dir);
-        curr = root;
-        prev = kInvalidNode;
-        pend = 0;
-    };
-    if (active) start_ray(0);
     uint32_t guard = kMaxTraversalSteps;
-    while (__any(active) && guard-- != 0) {
-        if (!active) continue;
+    while (guard-- != 0) {
+        const uint64_t liveMask = __ballot(live);
+        const uint64_t waitMask = __ballot(waiting);
+        if (waitMask == 0 && liveMask == 0) break;
+        if (waitMask != 0 && (liveMask == 0 || kAoRefill <= __popcll(waitMask))) {
+            if (waiting) {
+                float r0, r1;
+                if (useSobol) {
+                    sobol2d(&r0, &r1, (uint32_t)sample, (uint32_t)xi, (uint32_t)yi, seed);
+                } else {
+                    r0 = rng.uniformf();
+                    r1 = rng.uniformf();
+                }
+                f3 dl = hemisphere_cos(r0, r1);
+                dir = ((xaxis * dl.x) + (zaxis * dl.z)) + (n * dl.y);
+                inv = inv_rd(dir);
+                curr = root;
+                prev = kInvalidNode;
+                pend = 0;
+                live = true;
+                waiting = false;
+            }
+        }
+        if (!live) continue;
         bool done = false, blocked = false;
         if (curr & kLeafBit) {
             const Tri tri = tris[curr & ~kLeafBit];
@@ -318,8 +333,8 @@ __global__ void __launch_bounds__(kTraceBlock)
         if (done) {
             nRays++;
             occluded += blocked ? 1 : 0;
-            if (++sample < nSample) start_ray(sample);
-            else active = false;
+            live = false;
+            waiting = ++sample < nSample;
         }
     }
 
