@@ -9,6 +9,11 @@ import ctypes
 import os
 from ctypes import POINTER, c_char_p, c_float, c_int, c_uint32, c_ulonglong, c_void_p
 
+# torch first: libmlbvh must bind to the HIP runtime that torch has loaded.
+# Loaded the other way round, a process can end up with two HIP runtimes and
+# the second one sees no device.
+import torch  # noqa: F401
+
 from ._build import LIB as LIB_PATH
 
 MLBVH_OK = 0

@@ -101,9 +101,11 @@ def test_autograd_refit_and_rebuild(gpu, orc):
         loss = ((out - target) ** 2).sum()
         loss.backward()
         losses.append(loss.item())
-        # the autograd gradient is the adjoint of d loss / d out
+        # the autograd gradient is the adjoint of d loss / d out; two runs of
+        # the adjoint sum each vertex's terms with fp32 atomics in a different
+        # order, so they agree to rounding of the largest terms
         g2 = R.adjoint((2 * (out - target)).detach(), vt.detach(), R.hit_tri)
-        assert torch.allclose(vt.grad, g2, rtol=1e-5, atol=1e-6)
+        assert torch.allclose(vt.grad, g2, rtol=1e-4, atol=1e-5 * g2.abs().max().item())
         opt.step()
     assert losses[-1] < losses[0]
     # after refits, a fresh build gives the same closest-hit distances
