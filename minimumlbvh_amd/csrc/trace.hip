@@ -12,7 +12,7 @@ namespace mlbvh {
 
 constexpr int kTraceBlock = 256;
 #ifndef MLBVH_AO_REFILL
-#define MLBVH_AO_REFILL 16
+#define MLBVH_AO_REFILL 1
 #endif
 constexpr int kAoRefill = MLBVH_AO_REFILL;  // lanes waiting before a wave starts new AO rays
 template <bool kAny, bool kCount>
