@@ -48,6 +48,7 @@ constexpr int kCtrlTickets = 16;  // [16]
 constexpr int kCtrlEpochs = 32;   // [16]
 constexpr int kCtrlWorkQueue = 48;  // ray counter of the persistent trace
 constexpr int kCtrlFrontier = 49;   // entries of the climb's frontier list
+constexpr int kCtrlSortFallback = 50;  // set when a prefix segment is too long for the LDS sort
 constexpr int kCtrlWords = 64;
 constexpr int kNumSortCfgs = 5;  // tile shapes of the radix sort (onesweep.hip)
 constexpr uint32_t kEpochLimit = (1u << 30) - 1;  // look-back words keep 30 tag bits
@@ -77,6 +78,7 @@ __device__ inline void sort_state_init(uint32_t* ctrl, int nPass, uint32_t* hist
         ctrl[kCtrlTickets + threadIdx.x] = 0;
         ctrl[kCtrlEpochs + threadIdx.x] = s_base + 1 + threadIdx.x;
     }
+    if (threadIdx.x == 0) ctrl[kCtrlSortFallback] = 0;
 }
 
 // Grow a device buffer to hold at least `need` elements (contents dropped).

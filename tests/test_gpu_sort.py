@@ -120,3 +120,64 @@ def test_every_tile_config_is_bit_exact(gpu, orc, cfg):
                     assert np.array_equal(gk, ok)
     finally:
         gpu.set_tuning(2, 8, 64)
+
+
+def _prefix_keys(r, n, dtype, prefix_of, shift):
+    """Random keys whose 16-bit field at `shift` is prefix_of (n,) values."""
+    bits = np.dtype(dtype).itemsize * 8
+    keys = r.integers(0, np.iinfo(dtype).max, size=n, dtype=dtype, endpoint=True)
+    mask = dtype(0xFFFF) << dtype(shift)
+    keys = (keys & ~mask) | (prefix_of.astype(dtype) << dtype(shift))
+    assert shift + 16 <= bits
+    return keys
+
+
+@pytest.mark.parametrize("case", ["tiny_segments", "one_segment", "half_one_segment", "seg_2049",
+                                  "seg_3000", "seg_4097", "all_equal"])
+def test_hybrid_segment_shapes(gpu, orc, case):
+    """Large sorts finish each 16-bit-prefix segment in LDS; segments longer
+    than a block's look-ahead switch the whole sort to the full pass sequence.
+    Every shape must give the oracle's stable result bit for bit."""
+    n = (1 << 20) + 17
+    print(f"hybrid sort case {case} n={n}", flush=True)
+    r = np.random.default_rng(hash(case) % 2**32)
+    if case == "tiny_segments":
+        pf = r.integers(0, 1 << 16, size=n)
+    elif case == "one_segment":
+        pf = np.full(n, 0x1234)
+    elif case == "half_one_segment":
+        pf = np.where(r.random(n) < 0.5, 0x8000, r.integers(0, 1 << 16, size=n))
+    elif case == "all_equal":
+        pf = None
+    else:
+        length = int(case.split("_")[1])
+        pf = r.permutation(np.arange(n) // length)
+    if pf is None:
+        keys = np.full(n, 0x0123456789ABCDEF, np.uint64)
+    else:
+        keys = _prefix_keys(r, n, np.uint64, pf, 48)
+        keys[::7] = keys[3]  # ties keep input order
+    vals = np.arange(n, dtype=np.uint32)
+    gk, gv = _run(gpu, keys, vals, 0, 64)
+    ok, ov, _ = orc.radix_sort(keys, vals, 0, 64)
+    assert np.array_equal(gk, ok)
+    assert np.array_equal(gv, ov)
+    gk, _ = _run(gpu, keys, None, 0, 64)
+    assert np.array_equal(gk, ok)
+
+
+@pytest.mark.parametrize("dtype,start,end", [(np.uint64, 4, 60), (np.uint64, 17, 41), (np.uint32, 3, 29),
+                                             (np.uint32, 0, 32), (np.uint64, 40, 64)])
+def test_hybrid_bit_ranges(gpu, orc, dtype, start, end):
+    """The hybrid's 16-bit prefix is the two highest digits of the requested
+    range (digits past endBit included, as in every pass)."""
+    n = 600_011
+    print(f"hybrid sort {np.dtype(dtype).name} [{start},{end}) n={n}", flush=True)
+    r = np.random.default_rng(start * 100 + end)
+    keys = r.integers(0, np.iinfo(dtype).max, size=n, dtype=dtype, endpoint=True)
+    keys[::11] = keys[5]
+    vals = r.integers(0, 2**32, size=n, dtype=np.uint32)
+    gk, gv = _run(gpu, keys, vals, start, end)
+    ok, ov, _ = orc.radix_sort(keys, vals, start, end)
+    assert np.array_equal(gk, ok)
+    assert np.array_equal(gv, ov)
