@@ -18,17 +18,12 @@
 // table is never cleared between passes.  The tile shape (threads x keys per
 // thread) is one of kSortCfgs, chosen per context (mlbvh_set_tuning).
 //
-// Large standalone sorts take a hybrid route with the same result: the two
-// highest digit passes run as above (the array is then stably ordered by a
-// 16-bit prefix), and k_local_sort finishes every prefix segment in LDS.  A
-// block owns the segments that start in its chunk of kLocalChunk positions
-// and sorts their union by the whole key (stable LSD in LDS, skipping digits
-// that are constant in the range).  Owned ranges are disjoint and cover the
-// array, and sorting a prefix-ordered range keeps the prefix at every
-// position, so a block may read its neighbours' boundary region while they
-// write it.  A segment longer than the block's look-ahead sets a device flag
-// that turns on the full LSD pass sequence (guarded launches), which is
-// correct from any stable intermediate order.
+// Large standalone sorts take a hybrid route with the same result: the
+// three highest digit passes run as above (the array is then stably ordered
+// by a 24-bit prefix), and k_segment_finish puts each run of equal prefix in
+// order -- for uniform keys such a run holds a few keys.  A run longer than
+// the finish kernel's halo sets a device flag that turns on the full pass
+// sequence (guarded launches) from the three-pass state.
 #include <stdarg.h>
 #include <stdio.h>
 
@@ -278,182 +273,68 @@ __global__ void __launch_bounds__(kBlock)
 
 
 // ---------------------------------------------------------------------------
-// Hybrid finish: per-block LDS sort of whole prefix segments.
+// Hybrid finish: after three top-digit passes the array is stably ordered by
+// a 24-bit prefix, and for large uniform inputs the runs of equal prefix
+// ("segments") hold a few keys each.  Every key finds its segment by looking
+// at its neighbours' prefixes and its place in it by counting the segment's
+// keys that order before it (sort bits compared, ties by position): the
+// stable order, written out of place.
 // ---------------------------------------------------------------------------
-constexpr int kLocalChunk = 2048;  // a block owns the segments starting in its chunk
-constexpr int kLocalAhead = 2048;  // positions read past the chunk for the end of its last segment
-constexpr int kLocalBlock = 256;
-constexpr int kLocalItems = (kLocalChunk + kLocalAhead) / kLocalBlock;
+constexpr int kFinishBlock = 256;
+constexpr int kFinishItems = 8;                        // positions per thread
+constexpr int kFinishChunk = kFinishBlock * kFinishItems;
+constexpr int kFinishHalo = 64;                        // longest segment handled
 constexpr uint32_t kHybridMinKeys = 1u << 18;
 
 template <class K>
-__device__ inline uint32_t digit_at(K k, int shift)
+__device__ inline uint32_t prefix24(K k, int shift)
 {
-    return shift < (int)(8 * sizeof(K)) ? digit_of(k, shift) : 0u;
+    return shift < (int)(8 * sizeof(K)) ? (uint32_t)(k >> shift) & 0xFFFFFFu : 0u;
 }
 
-template <class K>
-__device__ inline uint32_t prefix16(K k, int shift)
-{
-    return shift < (int)(8 * sizeof(K)) ? (uint32_t)(k >> shift) & 0xFFFFu : 0u;
-}
-
-// keys (and vals) are stably ordered by the 16 bits at prefixShift.  Each
-// block sorts, in place, the union of the prefix segments that start in its
-// chunk by the digits at startBit + 8p, p < nPass.
+// sortMask selects the key bits [startBit, startBit + 8 * passes) that order keys.
 template <class K, bool kVals>
-__global__ void __launch_bounds__(kLocalBlock)
-    k_local_sort(K* keys, uint32_t* vals, uint32_t n, int startBit, int nPass, int prefixShift, uint32_t* ctrl)
+__global__ void __launch_bounds__(kFinishBlock)
+    k_segment_finish(const K* __restrict__ src, K* __restrict__ dst, const uint32_t* __restrict__ vsrc,
+                     uint32_t* __restrict__ vdst, uint32_t n, int prefixShift, K sortMask, uint32_t* ctrl)
 {
-    constexpr int kWin = kLocalChunk + kLocalAhead;
-    constexpr int kWaves = kLocalBlock / 64;
-    __shared__ K s_keys[kWin];
-    __shared__ uint32_t s_vals[kVals ? kWin : 1];
-    __shared__ uint16_t s_pf[kWin + 1];
-    __shared__ uint32_t s_wcnt[kWaves][256];
-    __shared__ uint32_t s_dstart[256];
-    __shared__ uint32_t s_tmp[4];
-    __shared__ uint32_t s_first, s_next, s_dmin, s_dmax;
-
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const uint64_t base = (uint64_t)blockIdx.x * kLocalChunk;
-    const int j0 = wave * 64 * kLocalItems + lane;  // item k sits at window offset j0 + 64 k
-
-    K key[kLocalItems];
-    uint32_t val[kVals ? kLocalItems : 1];
-#pragma unroll
-    for (int k = 0; k < kLocalItems; k++) {
-        const uint64_t pos = base + (uint64_t)(j0 + 64 * k);
-        key[k] = pos < n ? keys[pos] : (K)0;
-        s_pf[j0 + 64 * k + 1] = (uint16_t)prefix16(key[k], prefixShift);
-    }
-    if (t == 0) {
-        s_pf[0] = base > 0 ? (uint16_t)prefix16(keys[base - 1], prefixShift) : (uint16_t)0;
-        s_first = 0xFFFFFFFFu;
-        s_next = 0xFFFFFFFFu;
+    constexpr int kWin = kFinishChunk + 2 * kFinishHalo;
+    __shared__ K s_k[kWin];
+    const int t = threadIdx.x;
+    const int64_t lo = (int64_t)blockIdx.x * kFinishChunk - kFinishHalo;  // window start (may be < 0)
+    for (int i = t; i < kWin; i += kFinishBlock) {
+        const int64_t pos = lo + i;
+        s_k[i] = (pos >= 0 && pos < (int64_t)n) ? src[pos] : (K)0;
     }
     __syncthreads();
-    // segment starts: the first in the chunk, and the first after it
+    const int64_t winLo = lo < 0 ? 0 : lo;  // first window position inside the array
+    const int64_t winHi = lo + kWin < (int64_t)n ? lo + kWin : (int64_t)n;
+    bool overflow = false;
 #pragma unroll
-    for (int k = 0; k < kLocalItems; k++) {
-        const uint32_t j = (uint32_t)(j0 + 64 * k);
-        const uint64_t pos = base + j;
-        if (pos < n && (pos == 0 || s_pf[j + 1] != s_pf[j])) {
-            if (j < (uint32_t)kLocalChunk) atomicMin(&s_first, j);
-            else atomicMin(&s_next, j);
+    for (int r = 0; r < kFinishItems; r++) {
+        const int w = kFinishHalo + t + r * kFinishBlock;  // window index of this thread's position
+        const int64_t pos = lo + w;
+        if (pos >= (int64_t)n) continue;
+        const K key = s_k[w];
+        const uint32_t pf = prefix24(key, prefixShift);
+        int a = w, e = w + 1;
+        while (lo + a > winLo && prefix24(s_k[a - 1], prefixShift) == pf) a--;
+        while (lo + e < winHi && prefix24(s_k[e], prefixShift) == pf) e++;
+        // a segment reaching the window edge inside the array is longer than the halo
+        if ((lo + a == winLo && winLo > 0) || (lo + e == winHi && winHi < (int64_t)n)) {
+            overflow = true;
+            continue;
         }
+        const K mine = key & sortMask;
+        int rank = 0;
+        for (int j = a; j < e; j++) {
+            const K other = s_k[j] & sortMask;
+            rank += (other < mine || (other == mine && j < w)) ? 1 : 0;
+        }
+        dst[lo + a + rank] = key;
+        if (kVals) vdst[lo + a + rank] = vsrc[pos];
     }
-    __syncthreads();
-    const uint32_t first = s_first;
-    if (first == 0xFFFFFFFFu) return;  // the chunk lies inside a segment owned further left
-    uint32_t last = s_next;
-    if (last == 0xFFFFFFFFu) {
-        if (base + kWin < n) {  // the last segment runs past the look-ahead
-            if (t == 0) __hip_atomic_store(&ctrl[kCtrlSortFallback], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return;
-        }
-        last = (uint32_t)(n - base);
-    }
-    const uint32_t m = last - first;  // owned keys, window offsets [first, last)
-
-    if (kVals) {
-#pragma unroll
-        for (int k = 0; k < kLocalItems; k++) {
-            const uint32_t j = (uint32_t)(j0 + 64 * k);
-            val[k] = (j >= first && j < last) ? vals[base + j] : 0u;
-        }
-    }
-    const uint64_t ltMask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    bool moved = false;  // keys now sit compactly in [0, m) of the LDS order
-    for (int p = 0; p < nPass; p++) {
-        const int shift = startBit + 8 * p;
-        // skip a digit that is the same for every owned key
-        uint32_t dmin = 0xFFFFFFFFu, dmax = 0;
-#pragma unroll
-        for (int k = 0; k < kLocalItems; k++) {
-            const uint32_t j = (uint32_t)(j0 + 64 * k);
-            const bool valid = moved ? j < m : (j >= first && j < last);
-            if (valid) {
-                const uint32_t d = digit_at(key[k], shift);
-                dmin = d < dmin ? d : dmin;
-                dmax = d > dmax ? d : dmax;
-            }
-        }
-        if (t == 0) {
-            s_dmin = 0xFFFFFFFFu;
-            s_dmax = 0;
-        }
-        for (int i = t; i < kWaves * 256; i += kLocalBlock) (&s_wcnt[0][0])[i] = 0;
-        __syncthreads();
-        atomicMin(&s_dmin, dmin);
-        atomicMax(&s_dmax, dmax);
-        __syncthreads();
-        const bool same = s_dmin == s_dmax;
-        __syncthreads();  // every thread has read s_dmin / s_dmax before the next pass resets them
-        if (same) continue;
-
-        uint32_t off[kLocalItems];
-#pragma unroll
-        for (int k = 0; k < kLocalItems; k++) {
-            const uint32_t j = (uint32_t)(j0 + 64 * k);
-            const bool valid = moved ? j < m : (j >= first && j < last);
-            const uint32_t d = digit_at(key[k], shift);
-            uint64_t peers = __ballot(valid);
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t mk = __ballot(bit);
-                peers &= bit ? mk : ~mk;
-            }
-            const uint32_t rank = (uint32_t)__popcll(peers & ltMask);
-            const uint32_t cnt = (uint32_t)__popcll(peers);
-            const uint32_t prior = valid ? s_wcnt[wave][d] : 0u;
-            off[k] = prior + rank;
-            if (valid && rank == 0) s_wcnt[wave][d] = prior + cnt;
-        }
-        __syncthreads();
-        uint32_t tot = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; w++) {  // kLocalBlock == 256: thread t owns digit t
-            const uint32_t c = s_wcnt[w][t];
-            s_wcnt[w][t] = tot;
-            tot += c;
-        }
-        const uint32_t dstart = scan256(tot, s_tmp);
-        s_dstart[t] = dstart;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kLocalItems; k++) {
-            const uint32_t j = (uint32_t)(j0 + 64 * k);
-            const bool valid = moved ? j < m : (j >= first && j < last);
-            if (valid) {
-                const uint32_t d = digit_at(key[k], shift);
-                const uint32_t dst = s_dstart[d] + s_wcnt[wave][d] + off[k];
-                s_keys[dst] = key[k];
-                if (kVals) s_vals[dst] = val[k];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kLocalItems; k++) {
-            const uint32_t j = (uint32_t)(j0 + 64 * k);
-            if (j < m) {
-                key[k] = s_keys[j];
-                if (kVals) val[k] = s_vals[j];
-            }
-        }
-        moved = true;
-        __syncthreads();
-    }
-    if (!moved) return;  // already in order
-#pragma unroll
-    for (int k = 0; k < kLocalItems; k++) {
-        const uint32_t j = (uint32_t)(j0 + 64 * k);
-        if (j < m) {
-            keys[base + first + j] = key[k];
-            if (kVals) vals[base + first + j] = val[k];
-        }
-    }
+    if (overflow) __hip_atomic_store(&ctrl[kCtrlSortFallback], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // dst = src when the fallback ran an odd number of passes (device-side test).
@@ -525,48 +406,58 @@ static void dispatch_pass(int cfg, hipStream_t s, uint32_t tiles, const K* src, 
     }
 }
 
-// Two top-digit passes, the LDS finish, then the full pass sequence guarded by
-// the fallback flag (launched always, returning at once when not needed).
+// Three top-digit passes (keys -> tmp -> keys -> tmp), the segment finish
+// (tmp -> keys), then the full pass sequence from tmp guarded by the
+// fallback flag (launched always, returning at once when not needed).
 template <class K>
 static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, uint32_t n, int startBit,
                        int live, uint32_t tiles, hipStream_t stream)
 {
     int rc;
+    const int keyBits = 8 * (int)sizeof(K);
     const int hg = hist_grid(ctx, n);
     if ((rc = ensure_partials(ctx, hg, live))) return rc;
-    // pass slots: 0, 1 = the two top digits; 2 .. 2+live-1 = the fallback sequence
-    k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, 2 + live, ctx->hist, ctx->status, ctx->statusWords);
+    // pass slots: 0..2 = the three top digits; 3 .. 3+live-1 = the fallback sequence
+    k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, 3 + live, ctx->hist, ctx->status, ctx->statusWords);
     k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials);
     launch_hist_reduce(ctx, hg, live, stream);
-    const int sA = startBit + 8 * (live - 2), sB = startBit + 8 * (live - 1);
     const uint32_t* guard = ctx->ctrl + kCtrlSortFallback;
-    if (vals) {
-        dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, keys, ktmp, vals, vtmp, n, sA, ctx->hist + 256 * (live - 2),
-                               ctx->status, ctx->ctrl, 0);
-        dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, ktmp, keys, vtmp, vals, n, sB, ctx->hist + 256 * (live - 1),
-                               ctx->status, ctx->ctrl, 1);
-    } else {
-        dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, keys, ktmp, nullptr, nullptr, n, sA,
-                                ctx->hist + 256 * (live - 2), ctx->status, ctx->ctrl, 0);
-        dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, ktmp, keys, nullptr, nullptr, n, sB,
-                                ctx->hist + 256 * (live - 1), ctx->status, ctx->ctrl, 1);
+    K* bufs[2] = {keys, ktmp};
+    uint32_t* vbufs[2] = {vals, vtmp};
+    for (int q = 0; q < 3; q++) {
+        const int p = live - 3 + q;
+        const int shift = startBit + 8 * p;
+        if (vals)
+            dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, bufs[q & 1], bufs[(q + 1) & 1], vbufs[q & 1],
+                                   vbufs[(q + 1) & 1], n, shift, ctx->hist + 256 * p, ctx->status, ctx->ctrl, q);
+        else
+            dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, bufs[q & 1], bufs[(q + 1) & 1], nullptr, nullptr, n,
+                                    shift, ctx->hist + 256 * p, ctx->status, ctx->ctrl, q);
     }
-    const unsigned lb = (unsigned)((n + kLocalChunk - 1) / kLocalChunk);
-    if (vals) k_local_sort<K, true><<<lb, kLocalBlock, 0, stream>>>(keys, vals, n, startBit, live, sA, ctx->ctrl);
-    else k_local_sort<K, false><<<lb, kLocalBlock, 0, stream>>>(keys, nullptr, n, startBit, live, sA, ctx->ctrl);
-
-    K* src = keys;
-    K* dst = ktmp;
-    uint32_t* vsrc = vals;
-    uint32_t* vdst = vtmp;
+    // sort key = bits [startBit, min(startBit + 8 * live, keyBits))
+    const int topBit = startBit + 8 * live < keyBits ? startBit + 8 * live : keyBits;
+    const K high = topBit == keyBits ? ~(K)0 : (((K)1 << topBit) - 1);
+    const K sortMask = high & ~(((K)1 << startBit) - 1);
+    const int prefixShift = startBit + 8 * (live - 3);
+    const unsigned fb = (unsigned)((n + kFinishChunk - 1) / kFinishChunk);
+    if (vals)
+        k_segment_finish<K, true><<<fb, kFinishBlock, 0, stream>>>(ktmp, keys, vtmp, vals, n, prefixShift, sortMask,
+                                                                   ctx->ctrl);
+    else
+        k_segment_finish<K, false><<<fb, kFinishBlock, 0, stream>>>(ktmp, keys, nullptr, nullptr, n, prefixShift,
+                                                                    sortMask, ctx->ctrl);
+    K* src = ktmp;
+    K* dst = keys;
+    uint32_t* vsrc = vtmp;
+    uint32_t* vdst = vals;
     for (int p = 0; p < live; p++) {
         const int shift = startBit + 8 * p;
         if (vals)
             dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, src, dst, vsrc, vdst, n, shift, ctx->hist + 256 * p,
-                                   ctx->status, ctx->ctrl, 2 + p, guard);
+                                   ctx->status, ctx->ctrl, 3 + p, guard);
         else
             dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, src, dst, nullptr, nullptr, n, shift,
-                                    ctx->hist + 256 * p, ctx->status, ctx->ctrl, 2 + p, guard);
+                                    ctx->hist + 256 * p, ctx->status, ctx->ctrl, 3 + p, guard);
         K* tk = src;
         src = dst;
         dst = tk;
@@ -610,7 +501,7 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
     uint32_t tiles = sort_tiles(ctx, n);
     if ((rc = ensure_status(ctx, tiles))) return rc;
 
-    if (!histReady && n >= kHybridMinKeys && live >= 3) return sort_hybrid(ctx, keys, vals, ktmp, vtmp, n, startBit, live,
+    if (!histReady && n >= kHybridMinKeys && live >= 4) return sort_hybrid(ctx, keys, vals, ktmp, vtmp, n, startBit, live,
                                                                              tiles, stream);
 
     if (!histReady) {
