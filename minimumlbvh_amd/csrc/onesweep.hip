@@ -292,7 +292,10 @@ __device__ inline uint32_t prefix24(K k, int shift)
     return shift < (int)(8 * sizeof(K)) ? (uint32_t)(k >> shift) & 0xFFFFFFu : 0u;
 }
 
-// sortMask selects the key bits [startBit, startBit + 8 * passes) that order keys.
+// sortMask selects the key bits [startBit, startBit + 8 * passes) that order
+// keys.  A wave covers 64 consecutive positions per step, so the segment
+// bounds of most keys come from two ballots (segment starts, segment ends);
+// a segment that crosses the wave's 64 positions is walked in LDS.
 template <class K, bool kVals>
 __global__ void __launch_bounds__(kFinishBlock)
     k_segment_finish(const K* __restrict__ src, K* __restrict__ dst, const uint32_t* __restrict__ vsrc,
@@ -300,39 +303,61 @@ __global__ void __launch_bounds__(kFinishBlock)
 {
     constexpr int kWin = kFinishChunk + 2 * kFinishHalo;
     __shared__ K s_k[kWin];
-    const int t = threadIdx.x;
+    __shared__ uint32_t s_pf[kWin];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t lo = (int64_t)blockIdx.x * kFinishChunk - kFinishHalo;  // window start (may be < 0)
     for (int i = t; i < kWin; i += kFinishBlock) {
         const int64_t pos = lo + i;
-        s_k[i] = (pos >= 0 && pos < (int64_t)n) ? src[pos] : (K)0;
+        const K k = (pos >= 0 && pos < (int64_t)n) ? src[pos] : (K)0;
+        s_k[i] = k;
+        s_pf[i] = prefix24(k, prefixShift);
     }
     __syncthreads();
-    const int64_t winLo = lo < 0 ? 0 : lo;  // first window position inside the array
-    const int64_t winHi = lo + kWin < (int64_t)n ? lo + kWin : (int64_t)n;
+    const int wLo = (int)((lo < 0 ? 0 : lo) - lo);                                   // first valid window index
+    const int wHi = (int)((lo + kWin < (int64_t)n ? lo + kWin : (int64_t)n) - lo);   // one past the last
+    const uint64_t atOrBelow = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const uint64_t atOrAbove = ~((1ull << lane) - 1);
     bool overflow = false;
 #pragma unroll
     for (int r = 0; r < kFinishItems; r++) {
-        const int w = kFinishHalo + t + r * kFinishBlock;  // window index of this thread's position
-        const int64_t pos = lo + w;
-        if (pos >= (int64_t)n) continue;
-        const K key = s_k[w];
-        const uint32_t pf = prefix24(key, prefixShift);
-        int a = w, e = w + 1;
-        while (lo + a > winLo && prefix24(s_k[a - 1], prefixShift) == pf) a--;
-        while (lo + e < winHi && prefix24(s_k[e], prefixShift) == pf) e++;
+        const int w0 = kFinishHalo + r * kFinishBlock + wave * 64;  // window index of lane 0
+        const int w = w0 + lane;
+        const bool in = w < wHi;
+        const uint32_t pf = s_pf[w];
+        const bool starts = in && (w == wLo || s_pf[w - 1] != pf);
+        const bool ends = in && (w + 1 == wHi || s_pf[w + 1] != pf);
+        const uint64_t sm = __ballot(starts), em = __ballot(ends);
+        if (!in) continue;
+        int a, e;
+        const uint64_t below = sm & atOrBelow;
+        if (below) {
+            a = w0 + 63 - __clzll(below);
+        } else {
+            a = w0;
+            while (a > wLo && s_pf[a - 1] == pf) a--;
+        }
+        const uint64_t above = em & atOrAbove;
+        if (above) {
+            e = w0 + __builtin_ctzll(above) + 1;
+        } else {
+            e = w0 + 64;
+            while (e < wHi && s_pf[e] == pf) e++;
+        }
         // a segment reaching the window edge inside the array is longer than the halo
-        if ((lo + a == winLo && winLo > 0) || (lo + e == winHi && winHi < (int64_t)n)) {
+        if ((a == wLo && lo + wLo > 0) || (e == wHi && lo + wHi < (int64_t)n)) {
             overflow = true;
             continue;
         }
-        const K mine = key & sortMask;
         int rank = 0;
-        for (int j = a; j < e; j++) {
-            const K other = s_k[j] & sortMask;
-            rank += (other < mine || (other == mine && j < w)) ? 1 : 0;
+        if (e - a > 1) {
+            const K mine = s_k[w] & sortMask;
+            for (int j = a; j < e; j++) {
+                const K other = s_k[j] & sortMask;
+                rank += (other < mine || (other == mine && j < w)) ? 1 : 0;
+            }
         }
-        dst[lo + a + rank] = key;
-        if (kVals) vdst[lo + a + rank] = vsrc[pos];
+        dst[lo + a + rank] = s_k[w];
+        if (kVals) vdst[lo + a + rank] = vsrc[lo + w];
     }
     if (overflow) __hip_atomic_store(&ctrl[kCtrlSortFallback], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
