@@ -306,11 +306,30 @@ __global__ void __launch_bounds__(kFinishBlock)
     __shared__ uint32_t s_pf[kWin];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t lo = (int64_t)blockIdx.x * kFinishChunk - kFinishHalo;  // window start (may be < 0)
-    for (int i = t; i < kWin; i += kFinishBlock) {
+    // all loads in flight before the first LDS store
+    constexpr int kLoads = (kWin + kFinishBlock - 1) / kFinishBlock;
+    K ld[kLoads];
+#pragma unroll
+    for (int u = 0; u < kLoads; u++) {
+        const int i = t + u * kFinishBlock;
         const int64_t pos = lo + i;
-        const K k = (pos >= 0 && pos < (int64_t)n) ? src[pos] : (K)0;
-        s_k[i] = k;
-        s_pf[i] = prefix24(k, prefixShift);
+        ld[u] = (i < kWin && pos >= 0 && pos < (int64_t)n) ? src[pos] : (K)0;
+    }
+    uint32_t val[kVals ? kFinishItems : 1];
+    if (kVals) {
+#pragma unroll
+        for (int r = 0; r < kFinishItems; r++) {
+            const int64_t pos = lo + kFinishHalo + r * kFinishBlock + t;
+            val[r] = pos < (int64_t)n ? vsrc[pos] : 0u;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kLoads; u++) {
+        const int i = t + u * kFinishBlock;
+        if (i < kWin) {
+            s_k[i] = ld[u];
+            s_pf[i] = prefix24(ld[u], prefixShift);
+        }
     }
     __syncthreads();
     const int wLo = (int)((lo < 0 ? 0 : lo) - lo);                                   // first valid window index
@@ -357,7 +376,7 @@ __global__ void __launch_bounds__(kFinishBlock)
             }
         }
         dst[lo + a + rank] = s_k[w];
-        if (kVals) vdst[lo + a + rank] = vsrc[lo + w];
+        if (kVals) vdst[lo + a + rank] = val[r];
     }
     if (overflow) __hip_atomic_store(&ctrl[kCtrlSortFallback], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
