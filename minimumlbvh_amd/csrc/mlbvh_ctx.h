@@ -114,9 +114,11 @@ uint32_t sort_tiles(mlbvh_ctx* ctx, uint32_t n);
 int sort_tile_keys(mlbvh_ctx* ctx);
 int ensure_status(mlbvh_ctx* ctx, uint32_t tiles);
 // Histogram partials of `blocks` blocks x nPass passes, and their reduction
-// into ctx->hist (one launch on `stream`).
+// into ctx->hist[firstHist ..] (one launch on `stream`; with a guard word, the
+// launch returns at once while the word is 0).
 int ensure_partials(mlbvh_ctx* ctx, int blocks, int nPass);
-void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream);
+void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream, int firstHist = 0,
+                        const uint32_t* guard = nullptr);
 
 // One plain build (the body of mlbvh_build_ex).
 int build_one(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_nodes, void* d_root,

@@ -70,15 +70,18 @@ __device__ inline uint32_t scan256(uint32_t v, uint32_t* s_tmp)
     return base + x - v;
 }
 
-__global__ void k_sort_init(uint32_t* ctrl, int nPass, uint32_t* hist, uint64_t* status, size_t statusWords)
+// nPass pass slots (tags, tickets); nHist <= 8 histograms zeroed.
+__global__ void k_sort_init(uint32_t* ctrl, int nPass, int nHist, uint32_t* hist, uint64_t* status, size_t statusWords)
 {
-    sort_state_init(ctrl, nPass, hist, nPass, status, statusWords);
+    sort_state_init(ctrl, nPass, hist, nHist, status, statusWords);
 }
 
 template <class K>
 __global__ void __launch_bounds__(kHistBlock) k_hist(const K* __restrict__ keys, uint32_t n, int startBit,
-                                                     int nPass, uint32_t* __restrict__ partials)
+                                                     int nPass, uint32_t* __restrict__ partials,
+                                                     const uint32_t* __restrict__ guard = nullptr)
 {
+    if (guard && *guard == 0) return;
     __shared__ uint32_t s_h[8][256];
     for (int p = 0; p < 8; p++) s_h[p][threadIdx.x] = 0;
     __syncthreads();
@@ -112,8 +115,9 @@ __global__ void __launch_bounds__(kHistBlock) k_hist(const K* __restrict__ keys,
 // unrolled loads and adds its sum atomically (hist starts zeroed).
 constexpr int kReduceChunk = 32;
 __global__ void __launch_bounds__(256) k_hist_reduce(const uint32_t* __restrict__ partials, int nBlocks, int nPass,
-                                                     uint32_t* __restrict__ hist)
+                                                     uint32_t* __restrict__ hist, const uint32_t* __restrict__ guard)
 {
+    if (guard && *guard == 0) return;
     const int p = blockIdx.x;
     const int b0 = blockIdx.y * kReduceChunk;
     const int b1 = min(b0 + kReduceChunk, nBlocks);
@@ -284,7 +288,15 @@ constexpr int kFinishBlock = 256;
 constexpr int kFinishItems = 8;                        // positions per thread
 constexpr int kFinishChunk = kFinishBlock * kFinishItems;
 constexpr int kFinishHalo = 64;                        // longest segment handled
+constexpr int kFinishNear = 8;                         // segments up to 9 keys are ranked with shuffles
 constexpr uint32_t kHybridMinKeys = 1u << 18;
+
+__device__ inline uint32_t shfl64(uint32_t v, int src) { return __shfl(v, src, 64); }
+__device__ inline uint64_t shfl64(uint64_t v, int src)
+{
+    const uint32_t lo = __shfl((uint32_t)v, src, 64), hi = __shfl((uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
 
 template <class K>
 __device__ inline uint32_t prefix24(K k, int shift)
@@ -345,37 +357,42 @@ __global__ void __launch_bounds__(kFinishBlock)
         const uint32_t pf = s_pf[w];
         const bool starts = in && (w == wLo || s_pf[w - 1] != pf);
         const bool ends = in && (w + 1 == wHi || s_pf[w + 1] != pf);
-        const uint64_t sm = __ballot(starts), em = __ballot(ends);
-        if (!in) continue;
-        int a, e;
-        const uint64_t below = sm & atOrBelow;
-        if (below) {
-            a = w0 + 63 - __clzll(below);
-        } else {
-            a = w0;
-            while (a > wLo && s_pf[a - 1] == pf) a--;
-        }
-        const uint64_t above = em & atOrAbove;
-        if (above) {
-            e = w0 + __builtin_ctzll(above) + 1;
-        } else {
-            e = w0 + 64;
-            while (e < wHi && s_pf[e] == pf) e++;
-        }
-        // a segment reaching the window edge inside the array is longer than the halo
-        if ((a == wLo && lo + wLo > 0) || (e == wHi && lo + wHi < (int64_t)n)) {
-            overflow = true;
-            continue;
-        }
+        const uint64_t below = __ballot(starts) & atOrBelow;
+        const uint64_t above = __ballot(ends) & atOrAbove;
+        // bounds when the segment starts and ends inside this wave's 64 positions
+        const int aFast = below ? w0 + 63 - __clzll(below) : w0;
+        const int eFast = above ? w0 + __builtin_ctzll(above) + 1 : w0 + 64;
+        const K key = s_k[w];
+        const K mine = key & sortMask;
+        // rank among neighbours within +-kFinishNear lanes (all lanes take part)
         int rank = 0;
-        if (e - a > 1) {
-            const K mine = s_k[w] & sortMask;
+#pragma unroll
+        for (int d = 1; d <= kFinishNear; d++) {
+            const K up = shfl64(mine, lane + d < 64 ? lane + d : 63);
+            const K dn = shfl64(mine, lane - d >= 0 ? lane - d : 0);
+            rank += (w + d < eFast && up < mine) ? 1 : 0;
+            rank += (w - d >= aFast && dn <= mine) ? 1 : 0;
+        }
+        if (!in) continue;
+        int a = aFast, e = eFast;
+        if (!below || !above || e - a > 1 + kFinishNear) {
+            // longer segment, or one crossing this wave's positions: walk and count
+            if (!below)
+                while (a > wLo && s_pf[a - 1] == pf) a--;
+            if (!above)
+                while (e < wHi && s_pf[e] == pf) e++;
+            // a segment reaching the window edge inside the array is longer than the halo
+            if ((a == wLo && lo + wLo > 0) || (e == wHi && lo + wHi < (int64_t)n)) {
+                overflow = true;
+                continue;
+            }
+            rank = 0;
             for (int j = a; j < e; j++) {
                 const K other = s_k[j] & sortMask;
                 rank += (other < mine || (other == mine && j < w)) ? 1 : 0;
             }
         }
-        dst[lo + a + rank] = s_k[w];
+        dst[lo + a + rank] = key;
         if (kVals) vdst[lo + a + rank] = val[r];
     }
     if (overflow) __hip_atomic_store(&ctrl[kCtrlSortFallback], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -462,9 +479,10 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
     const int hg = hist_grid(ctx, n);
     if ((rc = ensure_partials(ctx, hg, live))) return rc;
     // pass slots: 0..2 = the three top digits; 3 .. 3+live-1 = the fallback sequence
-    k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, 3 + live, ctx->hist, ctx->status, ctx->statusWords);
-    k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials);
-    launch_hist_reduce(ctx, hg, live, stream);
+    k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, 3 + live, live, ctx->hist, ctx->status, ctx->statusWords);
+    // the three top digits' histograms now; the rest only if the fallback runs
+    k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - 3), 3, ctx->partials);
+    launch_hist_reduce(ctx, hg, 3, stream, live - 3);
     const uint32_t* guard = ctx->ctrl + kCtrlSortFallback;
     K* bufs[2] = {keys, ktmp};
     uint32_t* vbufs[2] = {vals, vtmp};
@@ -490,6 +508,10 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
     else
         k_segment_finish<K, false><<<fb, kFinishBlock, 0, stream>>>(ktmp, keys, nullptr, nullptr, n, prefixShift,
                                                                     sortMask, ctx->ctrl);
+    // fallback: histograms of the lower digits (of the three-pass state, which
+    // the finish does not modify), then every pass from that state
+    k_hist<K><<<hg, kHistBlock, 0, stream>>>(ktmp, n, startBit, live - 3, ctx->partials, guard);
+    launch_hist_reduce(ctx, hg, live - 3, stream, 0, guard);
     K* src = ktmp;
     K* dst = keys;
     uint32_t* vsrc = vtmp;
@@ -551,7 +573,7 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
     if (!histReady) {
         int hg = hist_grid(ctx, n);
         if ((rc = ensure_partials(ctx, hg, live))) return rc;
-        k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, live, ctx->hist, ctx->status, ctx->statusWords);
+        k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, live, live, ctx->hist, ctx->status, ctx->statusWords);
         k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials);
         launch_hist_reduce(ctx, hg, live, stream);
     }
@@ -593,10 +615,11 @@ int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, 
     return sort_impl<uint64_t>(ctx, keys, nullptr, tmp, nullptr, n, startBit, endBit, histReady, stream);
 }
 
-void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream)
+void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream, int firstHist,
+                        const uint32_t* guard)
 {
     dim3 grid((unsigned)nPass, (unsigned)((blocks + kReduceChunk - 1) / kReduceChunk), 1);
-    k_hist_reduce<<<grid, 256, 0, stream>>>(ctx->partials, blocks, nPass, ctx->hist);
+    k_hist_reduce<<<grid, 256, 0, stream>>>(ctx->partials, blocks, nPass, ctx->hist + 256 * firstHist, guard);
 }
 
 }  // namespace mlbvh
