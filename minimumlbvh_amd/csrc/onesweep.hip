@@ -288,15 +288,8 @@ constexpr int kFinishBlock = 256;
 constexpr int kFinishItems = 8;                        // positions per thread
 constexpr int kFinishChunk = kFinishBlock * kFinishItems;
 constexpr int kFinishHalo = 64;                        // longest segment handled
-constexpr int kFinishNear = 8;                         // segments up to 9 keys are ranked with shuffles
+constexpr int kFinishNear = 8;                         // segments up to 9 keys take the fixed-count path
 constexpr uint32_t kHybridMinKeys = 1u << 18;
-
-__device__ inline uint32_t shfl64(uint32_t v, int src) { return __shfl(v, src, 64); }
-__device__ inline uint64_t shfl64(uint64_t v, int src)
-{
-    const uint32_t lo = __shfl((uint32_t)v, src, 64), hi = __shfl((uint32_t)(v >> 32), src, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
 
 template <class K>
 __device__ inline uint32_t prefix24(K k, int shift)
@@ -364,14 +357,14 @@ __global__ void __launch_bounds__(kFinishBlock)
         const int eFast = above ? w0 + __builtin_ctzll(above) + 1 : w0 + 64;
         const K key = s_k[w];
         const K mine = key & sortMask;
-        // rank among neighbours within +-kFinishNear lanes (all lanes take part)
+        // short segment inside this wave's positions: a fixed count of
+        // comparisons from its first key (a uniform loop, no per-lane trip count)
         int rank = 0;
 #pragma unroll
-        for (int d = 1; d <= kFinishNear; d++) {
-            const K up = shfl64(mine, lane + d < 64 ? lane + d : 63);
-            const K dn = shfl64(mine, lane - d >= 0 ? lane - d : 0);
-            rank += (w + d < eFast && up < mine) ? 1 : 0;
-            rank += (w - d >= aFast && dn <= mine) ? 1 : 0;
+        for (int j = 0; j <= kFinishNear; j++) {
+            const int idx = aFast + j;
+            const K other = s_k[idx < kWin ? idx : kWin - 1] & sortMask;
+            rank += (idx < eFast && (other < mine || (other == mine && idx < w))) ? 1 : 0;
         }
         if (!in) continue;
         int a = aFast, e = eFast;
