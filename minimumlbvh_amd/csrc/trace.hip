@@ -15,6 +15,7 @@ constexpr int kTraceBlock = 256;
 #define MLBVH_AO_REFILL 1
 #endif
 constexpr int kAoRefill = MLBVH_AO_REFILL;  // lanes waiting before a wave starts new AO rays
+constexpr uint32_t kAoStack = 16;           // per-lane stack entries of the AO-ray walk
 template <bool kAny, bool kCount>
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace(const Node* __restrict__ nodes, const Tri* __restrict__ tris, const uint32_t* __restrict__ rootPtr,
@@ -261,13 +262,14 @@ __global__ void __launch_bounds__(kTraceBlock)
     const bool hasHit = inImage && hit.t != kFltMax;
 
     // AO rays, "while-while": a lane whose ray ends takes its next sample
-    // without waiting for the rest of the wave.  Starting a ray (Sobol point,
-    // cosine-hemisphere direction with its cos/sin, basis change, three
-    // reciprocals) costs a few node steps of instructions and the whole wave
-    // executes it whenever any lane needs it, so lanes are refilled in batches:
-    // once kAoRefill lanes wait, or no lane has a ray in flight.  An any-hit
-    // ray's answer does not depend on when it starts or on the visit order of
-    // the other lanes, so results and counters are those of one ray at a time.
+    // without waiting for the rest of the wave.  An AO ray only asks whether
+    // anything lies within `radius`, and that answer does not depend on the
+    // visit order, so these rays walk with an explicit per-lane stack in LDS
+    // (near child first, far child pushed): every internal node is fetched
+    // once, where the stack-free walk fetches it again on the way back up.  A
+    // push past kAoStack entries restarts that ray with the stack-free walk.
+    // The oracle's orc_any_hit_stack is the same walk, counters included.
+    __shared__ uint32_t s_stack[kAoStack][kTraceBlock];
     f3 n = mk(0, 0, 1), xaxis = mk(1, 0, 0), zaxis = mk(0, 1, 0), org = mk(0, 0, 0);
     if (hasHit) {
         n = normalize(hit.ng);
@@ -279,8 +281,10 @@ __global__ void __launch_bounds__(kTraceBlock)
     int sample = 0, occluded = 0;
     bool live = false;                         // a ray is in flight
     bool waiting = hasHit && 0 < nSample;      // the next sample is due
+    bool stackFree = false;                    // this ray fell back to the stack-free walk
     f3 dir = mk(0, 0, 1), inv = mk(0, 0, 1);
-    uint32_t curr = kInvalidNode, prev = kInvalidNode, pend = 0;
+    uint32_t curr = kInvalidNode, prev = kInvalidNode, pend = 0, sp = 0;
+    const int t = threadIdx.x;
     uint32_t guard = kMaxTraversalSteps;
     while (guard-- != 0) {
         const uint64_t liveMask = __ballot(live);
@@ -301,6 +305,8 @@ __global__ void __launch_bounds__(kTraceBlock)
                 curr = root;
                 prev = kInvalidNode;
                 pend = 0;
+                sp = 0;
+                stackFree = false;
                 live = true;
                 waiting = false;
             }
@@ -310,24 +316,52 @@ __global__ void __launch_bounds__(kTraceBlock)
         if (curr & kLeafBit) {
             const Tri tri = tris[curr & ~kLeafBit];
             if (kCount) nTest++;
-            float t, u, v;
+            float tt, u, v;
             f3 ng;
-            if (isect_tri(&t, &u, &v, &ng, 0.0f, radius, org, dir, tri.v[0], tri.v[1], tri.v[2])) {
+            if (isect_tri(&tt, &u, &v, &ng, 0.0f, radius, org, dir, tri.v[0], tri.v[1], tri.v[2])) {
                 done = blocked = true;
-            } else {
+            } else if (stackFree) {
                 uint32_t tmp = curr;
                 curr = prev;
                 prev = tmp;
+            } else if (sp == 0) {
+                done = true;
+            } else {
+                curr = s_stack[--sp][t];
             }
         } else {
             const NodeView nd = load_node(nodes, curr);
             if (kCount) nFetch++;
-            StepOut s = stackfree_step(nd, prev, org, inv, radius, pend);
-            if (s.done || s.next == kInvalidNode) {
-                done = true;
+            if (stackFree) {
+                StepOut s = stackfree_step(nd, prev, org, inv, radius, pend);
+                if (s.done || s.next == kInvalidNode) {
+                    done = true;
+                } else {
+                    prev = curr;
+                    curr = s.next;
+                }
             } else {
-                prev = curr;
-                curr = s.next;
+                const float2 rl = slabs(org, inv, nd.lo0, nd.hi0, radius);
+                const float2 rr = slabs(org, inv, nd.lo1, nd.hi1, radius);
+                const bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y;
+                if (hitL && hitR) {
+                    const bool rFirst = rr.x < rl.x;
+                    if (sp == kAoStack) {  // stack full: restart this ray stack-free
+                        stackFree = true;
+                        curr = root;
+                        prev = kInvalidNode;
+                        pend = 0;
+                    } else {
+                        s_stack[sp++][t] = rFirst ? nd.c0 : nd.c1;
+                        curr = rFirst ? nd.c1 : nd.c0;
+                    }
+                } else if (hitL || hitR) {
+                    curr = hitL ? nd.c0 : nd.c1;
+                } else if (sp == 0) {
+                    done = true;
+                } else {
+                    curr = s_stack[--sp][t];
+                }
             }
         }
         if (done) {

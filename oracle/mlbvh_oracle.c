@@ -897,10 +897,91 @@ void orc_render_normal(uint32_t* pixels, int w, int h, const orc_raygen* g, uint
     }
 }
 
+/* Any-hit walk with an explicit stack of at most `cap` entries: at a node whose
+   two children are both entered the near child (smaller entry distance, the
+   left one on ties) is taken and the far child pushed; a leaf or a node with
+   no child entered pops.  A push past `cap` restarts the ray with the
+   stack-free walk with early exit.  No reference counterpart: the reference
+   traces its AO rays with intersect_stackfree (minimum_lbvh.h:981-1061); the
+   answer (occluded or not) does not depend on the visit order, and the tests
+   check it against that walk.  cnt: node fetches, triangle tests. */
+int orc_any_hit_stack(const orc_node* nodes, const orc_tri* tris, uint32_t root, orc_v3 ro, orc_v3 rd, orc_v3 inv,
+                      float tmax, uint64_t* cnt, int cap)
+{
+    uint32_t stack[64];
+    int sp = 0;
+    uint32_t node = root;
+    if (cap > 64) cap = 64;
+    for (;;) {
+        if (node & ORC_LEAF_BIT) {
+            float t, u, v;
+            orc_v3 ng;
+            const orc_tri* tri = &tris[node & ~ORC_LEAF_BIT];
+            cnt[1]++;
+            if (orc_intersect_tri(&t, &u, &v, &ng, 0.0f, tmax, ro, rd, tri->vs[0], tri->vs[1], tri->vs[2])) return 1;
+            if (sp == 0) return 0;
+            node = stack[--sp];
+            continue;
+        }
+        const orc_node* nd = &nodes[node];
+        cnt[0]++;
+        float lmin, lmax, rmin, rmax;
+        orc_slabs(ro, inv, nd->aabbs[0].lower, nd->aabbs[0].upper, tmax, &lmin, &lmax);
+        orc_slabs(ro, inv, nd->aabbs[1].lower, nd->aabbs[1].upper, tmax, &rmin, &rmax);
+        int hitL = lmin <= lmax;
+        int hitR = rmin <= rmax;
+        if (hitL && hitR) {
+            uint32_t nearN = nd->children[0], farN = nd->children[1];
+            if (rmin < lmin) {
+                nearN = nd->children[1];
+                farN = nd->children[0];
+            }
+            if (sp == cap) {
+                orc_hit h;
+                hit_reset(&h, tmax);
+                orc_intersect_stackfree_ex(&h, nodes, tris, root, ro, rd, inv, ORC_QUERY_ANY, cnt, 1);
+                return h.t != tmax;
+            }
+            stack[sp++] = farN;
+            node = nearN;
+        } else if (hitL || hitR) {
+            node = nd->children[hitL ? 0 : 1];
+        } else {
+            if (sp == 0) return 0;
+            node = stack[--sp];
+        }
+    }
+}
+
+/* orc_any_hit_stack over a batch of rays (8 floats each, tmax in slot 3). */
+void orc_any_hit_stack_batch(const orc_node* nodes, const orc_tri* tris, uint32_t root, const float* rays, int nRays,
+                             int cap, uint8_t* occluded, uint64_t* counters, int nthreads)
+{
+    int nt = pick_threads(nthreads);
+    uint64_t c0 = 0, c1 = 0;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt) reduction(+ : c0, c1)
+    for (int i = 0; i < nRays; i++) {
+        const float* r = rays + 8 * (size_t)i;
+        orc_v3 ro = V(r[0], r[1], r[2]), rd = V(r[4], r[5], r[6]);
+        uint64_t cnt[2] = {0, 0};
+        occluded[i] = (uint8_t)orc_any_hit_stack(nodes, tris, root, ro, rd, orc_inv_rd(rd), r[3], cnt, cap);
+        c0 += cnt[0];
+        c1 += cnt[1];
+    }
+    if (counters) {
+        counters[0] += c0;
+        counters[1] += c1;
+    }
+}
+
+/* walk: 0 = the reference stack-free walk for every ray; 1 = stack-free with
+   the early exit; 2 = stack-free with early exit for the primary ray and
+   orc_any_hit_stack (stack of ORC_AO_STACK entries) for the AO rays. */
 void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_raygen* g, uint32_t root,
                    const orc_node* nodes, const orc_tri* tris, int useSobol, int nSample, float radius, uint32_t seed,
-                   int row0, int row1, uint64_t* counters, int nthreads, int earlyExit) /* main_gpu.cu:111-177 */
+                   int row0, int row1, uint64_t* counters, int nthreads, int walk) /* main_gpu.cu:111-177 */
 {
+    const int earlyExit = walk != 0;
     int nt = pick_threads(nthreads);
     uint64_t cRays = 0, cFetch = 0, cTest = 0;
     if (row0 < 0) row0 = 0;
@@ -942,12 +1023,19 @@ void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_rayge
                 }
                 orc_v3 dl = hemisphere_cos(r0, r1);
                 orc_v3 dir = vadd(vadd(vmuls(xaxis, dl.x), vmuls(zaxis, dl.z)), vmuls(n, dl.y));
-                orc_hit h2;
-                hit_reset(&h2, radius);
-                orc_intersect_stackfree_ex(&h2, nodes, tris, root, ao_origin, dir, orc_inv_rd(dir), ORC_QUERY_ANY, cnt,
-                                           earlyExit);
+                int occluded;
+                if (walk == 2) {
+                    occluded = orc_any_hit_stack(nodes, tris, root, ao_origin, dir, orc_inv_rd(dir), radius, cnt,
+                                                 ORC_AO_STACK);
+                } else {
+                    orc_hit h2;
+                    hit_reset(&h2, radius);
+                    orc_intersect_stackfree_ex(&h2, nodes, tris, root, ao_origin, dir, orc_inv_rd(dir), ORC_QUERY_ANY,
+                                               cnt, earlyExit);
+                    occluded = h2.t != radius;
+                }
                 cRays++;
-                if (h2.t != radius) ao += 1.0f / (float)nSample;
+                if (occluded) ao += 1.0f / (float)nSample;
             }
             cFetch += cnt[0];
             cTest += cnt[1];

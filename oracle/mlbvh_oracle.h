@@ -118,9 +118,15 @@ void orc_render_normal(uint32_t* pixels, int w, int h, const orc_raygen* g, uint
                        const orc_node* nodes, const orc_tri* tris, int nthreads);
 /* aoOut (nullable): occlusion fraction per pixel, -1 for primary miss.
    rows [row0,row1) only (for bounded CPU baselines); counters: rays traced, node fetches, tri tests */
+/* walk: 0 reference stack-free walk, 1 stack-free with early exit, 2 early-exit primary + stack any-hit AO rays */
+#define ORC_AO_STACK 16
+int orc_any_hit_stack(const orc_node* nodes, const orc_tri* tris, uint32_t root, orc_v3 ro, orc_v3 rd, orc_v3 inv,
+                      float tmax, uint64_t* cnt, int cap);
+void orc_any_hit_stack_batch(const orc_node* nodes, const orc_tri* tris, uint32_t root, const float* rays, int nRays,
+                             int cap, uint8_t* occluded, uint64_t* counters, int nthreads);
 void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_raygen* g, uint32_t root,
                    const orc_node* nodes, const orc_tri* tris, int useSobol, int nSample, float radius,
-                   uint32_t seed, int row0, int row1, uint64_t* counters, int nthreads, int earlyExit);
+                   uint32_t seed, int row0, int row1, uint64_t* counters, int nthreads, int walk);
 void orc_render_pt(float* acc /* float4 per pixel */, int w, int h, const orc_raygen* g, uint32_t root,
                    const orc_node* nodes, const orc_tri* tris, const orc_attrib* attribs, int useSobol,
                    int maxSPP, int maxDepth, int nthreads);

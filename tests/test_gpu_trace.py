@@ -1,8 +1,9 @@
 """GPU stack-free traversal and render kernels vs the oracle.
 
 Bar: Hit records (t, uv, ng, triangleIndex) bit-identical to the oracle;
-node-fetch / triangle-test counters identical (the traversal order is the
-same).  Normal images bit-identical.  AO: primary hit/miss identical on every
+node-fetch / triangle-test counters identical to the oracle's walks of the
+same kind (early-exit stack-free for closest/any-hit rays, the stack any-hit
+walk for AO rays).  Normal images bit-identical.  AO: primary hit/miss identical on every
 pixel; the occlusion fraction may differ only where an AO direction, which
 goes through the device cosf/sinf instead of the host libm ones, grazes an
 edge -- at most 0.5% of hit pixels and by at most 1/N each.  PT: mean radiance
@@ -131,7 +132,9 @@ def test_render_ao_matches_oracle(gpu, orc, w, h, nsample, radius, seed):
     assert dpx.max() <= 1
     gc = cnt.cpu().numpy().view(np.uint64)
     assert gc[0] == rcnt[0]  # rays traced: 1 + N per hit pixel
-    _, eao, ecnt = orc.render_ao(og, w, h, root, nodes, tris, 1, nsample, radius, seed, early_exit=True)
+    # the kernel's walks: early-exit stack-free primary ray, stack any-hit AO rays
+    _, eao, ecnt = orc.render_ao(og, w, h, root, nodes, tris, 1, nsample, radius, seed, ao_stack=True)
+    assert np.array_equal(eao, rao)
     if (diff == 0).all():
         assert np.array_equal(gc, ecnt)
 

@@ -90,3 +90,39 @@ def test_early_exit_is_exact(orc, query, tmax):
         assert np.array_equal(ee.view(np.uint32), ref.view(np.uint32))
         assert cee[0] <= cref[0] and cee[1] == cref[1]
     assert cee[0] < cref[0]
+
+
+@pytest.mark.parametrize("cap", [16, 2, 1])
+def test_any_hit_stack_matches_stackfree(orc, cap):
+    """The AO rays' stack walk answers exactly what the stack-free any-hit walk
+    answers; cap 1-2 forces the restart path (stack overflow -> stack-free)."""
+    for n, seed, tmax in [(2, 4, 5.0), (700, 5, 0.3), (65536, 6, 0.05), (20000, 7, 2.0)]:
+        tris = scenes.random_triangles(n, seed, sigma=0.02)
+        nodes, root, _, _ = orc.build(tris)
+        rays = random_rays(20000, seed + 20, tmax=tmax)
+        ref, cref = orc.trace(nodes, tris, root, rays, query=1, method=0)
+        occ, cnt = orc.any_hit_stack(nodes, tris, root, rays, cap=cap)
+        assert np.array_equal(occ, ref["tri"] != 0xFFFFFFFF), (n, cap)
+        assert occ.sum() > 0
+    if cap == 16:
+        assert cnt[0] < cref[0]  # each internal node fetched once per visit
+
+
+def test_ao_walks_agree(orc):
+    """Occlusion per pixel is the same whichever walk the AO rays take.  All
+    three walks visit leaves in the same depth-first, near-child-first order,
+    so they also test the same triangles; they differ only in how often an
+    internal node is fetched again."""
+    tris = scenes.random_triangles(30000, 9)
+    nodes, root, _, _ = orc.build(tris)
+    eye, ctr, up, fov = scenes.CAMERA_SMALL
+    import math
+    g = orc.raygen_lookat(eye, ctr, up, math.radians(fov), 96, 80)
+    outs = [orc.render_ao(g, 96, 80, root, nodes, tris, 1, 8, r, 3, **kw) for r in (0.1, 1.0)
+            for kw in (dict(), dict(early_exit=True), dict(ao_stack=True))]
+    for k in (0, 3):
+        base_px, base_ao, base_cnt = outs[k]
+        for px, ao, cnt in outs[k + 1:k + 3]:
+            assert np.array_equal(px, base_px) and np.array_equal(np.nan_to_num(ao), np.nan_to_num(base_ao))
+            assert cnt[0] == base_cnt[0] and cnt[2] == base_cnt[2]
+        assert outs[k + 2][2][1] < outs[k + 1][2][1] < base_cnt[1]
