@@ -248,43 +248,38 @@ __global__ void __launch_bounds__(kTraceBlock)
     const uint32_t root = *rootPtr;
     const int pixel = xi + yi * w;
 
-    // Primary ray: closest hit, exactly the reference's visit order.
-    f3 ro = mk(0, 0, 0), rd = mk(0, 0, 1);
-    Hit hit;
-    hit.t = kFltMax;
-    hit.ng = mk(0, 0, 0);
-    hit.tri = 0xFFFFFFFFu;
-    if (inImage) {
-        shoot(cam, &ro, &rd, (float)xi / (float)w, (float)yi / (float)h);
-        traverse<false, kCount>(nodes, tris, root, ro, rd, inv_rd(rd), hit, nFetch, nTest);
-        nRays++;
-    }
-    const bool hasHit = inImage && hit.t != kFltMax;
-
-    // AO rays, "while-while": a lane whose ray ends takes its next sample
-    // without waiting for the rest of the wave.  An AO ray only asks whether
-    // anything lies within `radius`, and that answer does not depend on the
-    // visit order, so these rays walk with an explicit per-lane stack in LDS
-    // (near child first, far child pushed): every internal node is fetched
-    // once, where the stack-free walk fetches it again on the way back up.  A
-    // push past kAoStack entries restarts that ray with the stack-free walk.
-    // The oracle's orc_any_hit_stack is the same walk, counters included.
+    // All of a lane's rays run in one "while-while" loop: first its primary ray
+    // (closest hit, the reference's stack-free walk and visit order), then its
+    // AO rays.  A lane whose ray ends takes its next one without waiting for
+    // the rest of the wave, so a wave is not held up by its slowest primary
+    // ray before any AO ray starts.  An AO ray only asks whether anything lies
+    // within `radius`, and that answer does not depend on the visit order, so
+    // AO rays walk with an explicit per-lane stack in LDS (near child first,
+    // far child pushed): every internal node is fetched once, where the
+    // stack-free walk fetches it again on the way back up.  A push past
+    // kAoStack entries restarts that AO ray with the stack-free walk.  The
+    // oracle's orc_render_ao (walk 2) does the same, counters included.
+    // Node fetch, slab tests and triangle test are shared by both walks; only
+    // the choice of the next node differs.
     __shared__ uint32_t s_stack[kAoStack][kTraceBlock];
-    f3 n = mk(0, 0, 1), xaxis = mk(1, 0, 0), zaxis = mk(0, 1, 0), org = mk(0, 0, 0);
-    if (hasHit) {
-        n = normalize(hit.ng);
-        if (0.0f < dot(n, rd)) n = -n;
-        onb(n, &xaxis, &zaxis);
-        org = (ro + rd * hit.t) + n * 0.001f;
+    const int t = threadIdx.x;
+    f3 o = mk(0, 0, 0), d = mk(0, 0, 1), iv = mk(0, 0, 1);  // the lane's current ray
+    float tmax = kFltMax;                                    // primary: closest t so far; AO: radius
+    f3 hitNg = mk(0, 0, 0);
+    bool primary = inImage;       // the primary ray is in flight
+    bool live = inImage;          // a ray is in flight
+    bool waiting = false;         // the next AO sample is due
+    bool stackFree = true;        // the current ray walks stack-free
+    bool hasHit = false;
+    uint32_t curr = kInvalidNode, prev = kInvalidNode, pend = 0, sp = 0;
+    if (inImage) {
+        shoot(cam, &o, &d, (float)xi / (float)w, (float)yi / (float)h);
+        iv = inv_rd(d);
+        curr = root;
     }
+    f3 n = mk(0, 0, 1), xaxis = mk(1, 0, 0), zaxis = mk(0, 1, 0), org = mk(0, 0, 0);
     Pcg rng(hash_pcg((uint32_t)pixel), 0);
     int sample = 0, occluded = 0;
-    bool live = false;                         // a ray is in flight
-    bool waiting = hasHit && 0 < nSample;      // the next sample is due
-    bool stackFree = false;                    // this ray fell back to the stack-free walk
-    f3 dir = mk(0, 0, 1), inv = mk(0, 0, 1);
-    uint32_t curr = kInvalidNode, prev = kInvalidNode, pend = 0, sp = 0;
-    const int t = threadIdx.x;
     uint32_t guard = kMaxTraversalSteps;
     while (guard-- != 0) {
         const uint64_t liveMask = __ballot(live);
@@ -300,8 +295,10 @@ __global__ void __launch_bounds__(kTraceBlock)
                     r1 = rng.uniformf();
                 }
                 f3 dl = hemisphere_cos(r0, r1);
-                dir = ((xaxis * dl.x) + (zaxis * dl.z)) + (n * dl.y);
-                inv = inv_rd(dir);
+                o = org;
+                d = ((xaxis * dl.x) + (zaxis * dl.z)) + (n * dl.y);
+                iv = inv_rd(d);
+                tmax = radius;
                 curr = root;
                 prev = kInvalidNode;
                 pend = 0;
@@ -318,57 +315,73 @@ __global__ void __launch_bounds__(kTraceBlock)
             if (kCount) nTest++;
             float tt, u, v;
             f3 ng;
-            if (isect_tri(&tt, &u, &v, &ng, 0.0f, radius, org, dir, tri.v[0], tri.v[1], tri.v[2])) {
+            const bool hitTri = isect_tri(&tt, &u, &v, &ng, 0.0f, tmax, o, d, tri.v[0], tri.v[1], tri.v[2]);
+            if (hitTri && !primary) {
                 done = blocked = true;
-            } else if (stackFree) {
-                uint32_t tmp = curr;
-                curr = prev;
-                prev = tmp;
-            } else if (sp == 0) {
-                done = true;
             } else {
-                curr = s_stack[--sp][t];
-            }
-        } else {
-            const NodeView nd = load_node(nodes, curr);
-            if (kCount) nFetch++;
-            if (stackFree) {
-                StepOut s = stackfree_step(nd, prev, org, inv, radius, pend);
-                if (s.done || s.next == kInvalidNode) {
-                    done = true;
-                } else {
-                    prev = curr;
-                    curr = s.next;
+                if (hitTri) {  // primary: a closer hit (or an equal-t one visited later)
+                    tmax = tt;
+                    hitNg = ng;
                 }
-            } else {
-                const float2 rl = slabs(org, inv, nd.lo0, nd.hi0, radius);
-                const float2 rr = slabs(org, inv, nd.lo1, nd.hi1, radius);
-                const bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y;
-                if (hitL && hitR) {
-                    const bool rFirst = rr.x < rl.x;
-                    if (sp == kAoStack) {  // stack full: restart this ray stack-free
-                        stackFree = true;
-                        curr = root;
-                        prev = kInvalidNode;
-                        pend = 0;
-                    } else {
-                        s_stack[sp++][t] = rFirst ? nd.c0 : nd.c1;
-                        curr = rFirst ? nd.c1 : nd.c0;
-                    }
-                } else if (hitL || hitR) {
-                    curr = hitL ? nd.c0 : nd.c1;
+                if (stackFree) {
+                    uint32_t tmp = curr;
+                    curr = prev;
+                    prev = tmp;
                 } else if (sp == 0) {
                     done = true;
                 } else {
                     curr = s_stack[--sp][t];
                 }
             }
+        } else {
+            const NodeView nd = load_node(nodes, curr);
+            if (kCount) nFetch++;
+            const float2 rl = slabs(o, iv, nd.lo0, nd.hi0, tmax);
+            const float2 rr = slabs(o, iv, nd.lo1, nd.hi1, tmax);
+            const bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y, rFirst = rr.x < rl.x;
+            if (stackFree) {
+                const StepOut s = stackfree_decide(nd, prev, hitL, hitR, rFirst, pend);
+                if (s.done || s.next == kInvalidNode) {
+                    done = true;
+                } else {
+                    prev = curr;
+                    curr = s.next;
+                }
+            } else if (hitL && hitR) {
+                if (sp == kAoStack) {  // stack full: restart this AO ray stack-free
+                    stackFree = true;
+                    curr = root;
+                    prev = kInvalidNode;
+                    pend = 0;
+                } else {
+                    s_stack[sp++][t] = rFirst ? nd.c0 : nd.c1;
+                    curr = rFirst ? nd.c1 : nd.c0;
+                }
+            } else if (hitL || hitR) {
+                curr = hitL ? nd.c0 : nd.c1;
+            } else if (sp == 0) {
+                done = true;
+            } else {
+                curr = s_stack[--sp][t];
+            }
         }
         if (done) {
             nRays++;
-            occluded += blocked ? 1 : 0;
             live = false;
-            waiting = ++sample < nSample;
+            if (primary) {
+                primary = false;
+                hasHit = tmax != kFltMax;
+                if (hasHit) {
+                    n = normalize(hitNg);
+                    if (0.0f < dot(n, d)) n = -n;
+                    onb(n, &xaxis, &zaxis);
+                    org = (o + d * tmax) + n * 0.001f;
+                    waiting = 0 < nSample;
+                }
+            } else {
+                occluded += blocked ? 1 : 0;
+                waiting = ++sample < nSample;
+            }
         }
     }
 

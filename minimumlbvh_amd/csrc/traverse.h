@@ -102,16 +102,16 @@ struct StepOut {
     uint32_t next;
     bool done;
 };
-__device__ inline StepOut stackfree_step(const NodeView& nd, uint32_t prev, f3 ro, f3 inv, float tHit, uint32_t& pend)
+// The decision part of a step, given the two children's slab results
+// (hitL / hitR: the child is entered; rFirst: the right child is entered
+// first, which with both hit makes it the near child).
+__device__ inline StepOut stackfree_decide(const NodeView& nd, uint32_t prev, bool hitL, bool hitR, bool rFirst,
+                                           uint32_t& pend)
 {
-    float2 rl = slabs(ro, inv, nd.lo0, nd.hi0, tHit);
-    float2 rr = slabs(ro, inv, nd.lo1, nd.hi1, tHit);
-    bool hitL = rl.x <= rl.y;
-    bool hitR = rr.x <= rr.y;
     uint32_t nearN = nd.c0, farN = nd.c1;
     int nHits = 0;
     if (hitL && hitR) {
-        if (rr.x < rl.x) {
+        if (rFirst) {
             nearN = nd.c1;
             farN = nd.c0;
         }
@@ -131,6 +131,13 @@ __device__ inline StepOut stackfree_step(const NodeView& nd, uint32_t prev, f3 r
         next = nd.parent;
     }
     return StepOut{next, next == nd.parent && pend == 0};
+}
+
+__device__ inline StepOut stackfree_step(const NodeView& nd, uint32_t prev, f3 ro, f3 inv, float tHit, uint32_t& pend)
+{
+    const float2 rl = slabs(ro, inv, nd.lo0, nd.hi0, tHit);
+    const float2 rr = slabs(ro, inv, nd.lo1, nd.hi1, tHit);
+    return stackfree_decide(nd, prev, rl.x <= rl.y, rr.x <= rr.y, rr.x < rl.x, pend);
 }
 
 template <bool kAny, bool kCount>
