@@ -111,12 +111,45 @@ MLBVH_API int mlbvh_build_ex(mlbvh_ctx* ctx, const void* d_triangles, int n, uin
 MLBVH_API int mlbvh_refit(mlbvh_ctx* ctx, const void* d_triangles, int n, const uint64_t* d_sorted_records,
                 const uint8_t* d_deltas, void* d_nodes, void* d_root, void* stream);
 
+/* Indexed-mesh build: the triangles are index triples d_indices
+ * (nTriangles x 3 u32) into d_vertices (nVertices x 3 fp32 xyz).  No
+ * reference counterpart: BVHGPUBuilder::build (minimum_lbvh.h:783-871) takes
+ * a Triangle array, so a caller with an indexed mesh first writes the 36-byte
+ * de-indexed copy.  Nodes, root and the optional sorted records / deltas are
+ * bit-identical to mlbvh_build_ex on that copy; indices >= nVertices are
+ * clamped to the last vertex (memory-safe, results unspecified). */
+MLBVH_API int mlbvh_build_indexed(mlbvh_ctx* ctx, const float* d_vertices, int nVertices, const uint32_t* d_indices,
+                                  int nTriangles, uint32_t buildOption, void* d_nodes, void* d_root,
+                                  uint64_t* d_sorted_records, uint8_t* d_deltas, void* stream);
+
+/* mlbvh_refit for an indexed mesh (arguments as mlbvh_build_indexed). */
+MLBVH_API int mlbvh_refit_indexed(mlbvh_ctx* ctx, const float* d_vertices, int nVertices, const uint32_t* d_indices,
+                                  int nTriangles, const uint64_t* d_sorted_records, const uint8_t* d_deltas,
+                                  void* d_nodes, void* d_root, void* stream);
+
 /* Batch trace with intersect_stackfree (minimum_lbvh.h:981-1061).
  * d_rays: nRays x 32 B Ray; d_hits: nRays x 32 B Hit.  d_counters (nullable):
  * 2 x u64 accumulated [internal-node fetches, triangle tests]. */
 MLBVH_API int mlbvh_trace(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles, const void* d_root,
                 const float* d_rays, void* d_hits, int nRays, int rayQuery, unsigned long long* d_counters,
                 void* stream);
+
+/* Batch trace with intersect_escape_link (minimum_lbvh.h:1063-1137): the
+ * same arguments and records as mlbvh_trace.  Closest-hit t equals
+ * mlbvh_trace's; on equal-t ties the triangle can differ (this walk always
+ * takes the left child first).  d_counters[0] counts internal-node visits
+ * (each visit tests one child box). */
+/* mlbvh_trace over an indexed mesh (vertex / index buffers as
+ * mlbvh_build_indexed): Hit records bit-identical to mlbvh_trace on the
+ * de-indexed triangles.  No reference counterpart (the reference traverses a
+ * Triangle array). */
+MLBVH_API int mlbvh_trace_indexed(mlbvh_ctx* ctx, const void* d_nodes, const float* d_vertices, int nVertices,
+                                  const uint32_t* d_indices, const void* d_root, const float* d_rays, void* d_hits,
+                                  int nRays, int rayQuery, unsigned long long* d_counters, void* stream);
+
+MLBVH_API int mlbvh_trace_escape_link(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,
+                                      const void* d_root, const float* d_rays, void* d_hits, int nRays,
+                                      int rayQuery, unsigned long long* d_counters, void* stream);
 
 /* Batched build of many meshes in one pipeline.  No reference counterpart:
  * the reference builds one mesh per BVHGPUBuilder::build call

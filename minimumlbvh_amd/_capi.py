@@ -38,7 +38,11 @@ SIGNATURES = {
     "mlbvh_build": ([_P, _P, c_int, c_uint32, _P, _P, _P], c_int),
     "mlbvh_build_ex": ([_P, _P, c_int, c_uint32, _P, _P, _P, _P, _P], c_int),
     "mlbvh_refit": ([_P, _P, c_int, _P, _P, _P, _P, _P], c_int),
+    "mlbvh_build_indexed": ([_P, _P, c_int, _P, c_int, c_uint32, _P, _P, _P, _P, _P], c_int),
+    "mlbvh_refit_indexed": ([_P, _P, c_int, _P, c_int, _P, _P, _P, _P, _P], c_int),
     "mlbvh_trace": ([_P, _P, _P, _P, _P, _P, c_int, c_int, _P, _P], c_int),
+    "mlbvh_trace_escape_link": ([_P, _P, _P, _P, _P, _P, c_int, c_int, _P, _P], c_int),
+    "mlbvh_trace_indexed": ([_P, _P, _P, c_int, _P, _P, _P, _P, c_int, c_int, _P, _P], c_int),
     "mlbvh_build_batched": ([_P, _P, _P, c_int, c_uint32, c_uint32, c_uint32, _P, _P, _P], c_int),
     "mlbvh_trace_batched": ([_P, _P, _P, _P, _P, c_int, _P, _P, c_int, c_int, _P, _P], c_int),
     "mlbvh_build_trace_per_mesh": ([_P, _P, _P, c_int, c_uint32, _P, _P, _P, _P, _P, c_int, _P], c_int),

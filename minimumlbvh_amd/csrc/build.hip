@@ -32,8 +32,11 @@ __global__ void k_build_init(uint32_t* sceneBox, uint32_t* ctrl, uint32_t* hist,
 }
 
 // Scene bounds over all vertices: per-thread running min/max, wave and block
-// reduction, then 6 order-mapped integer atomics per block.
-__global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(const Tri* __restrict__ tris, int n, uint32_t* sceneBox)
+// reduction, then 6 order-mapped integer atomics per block.  TS, here and in
+// the kernels below, is the triangle source: `const Tri*` (a soup) or
+// IndexedTris.
+template <class TS>
+__global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(TS tris, int n, uint32_t* sceneBox)
 {
     f3 lo = mk(kFltMax, kFltMax, kFltMax), hi = mk(-kFltMax, -kFltMax, -kFltMax);
     for (int i = blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += gridDim.x * kBuildBlock) {
@@ -71,8 +74,9 @@ __global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(const Tri* __restric
 // Grid-stride over a few blocks per CU so that the histograms are merged into
 // global memory by few blocks (global atomics on 1024 shared counters are the
 // cost otherwise).
+template <class TS>
 __global__ void __launch_bounds__(kBuildBlock)
-    k_morton(const Tri* __restrict__ tris, int n, const uint32_t* __restrict__ sceneBox, uint32_t opt,
+    k_morton(TS tris, int n, const uint32_t* __restrict__ sceneBox, uint32_t opt,
              uint64_t* __restrict__ rec, uint32_t* __restrict__ partials, uint32_t* __restrict__ flags,
              uint32_t* __restrict__ ctrl)
 {
@@ -142,8 +146,9 @@ __global__ void __launch_bounds__(kBuildBlock)
 #endif
 constexpr int kLocalLeaves = MLBVH_CLIMB_LEAVES;
 
+template <class TS>
 __global__ void __launch_bounds__(kLocalLeaves)
-    k_climb_local(const Tri* __restrict__ tris, uint32_t n, const uint64_t* __restrict__ rec,
+    k_climb_local(TS tris, uint32_t n, const uint64_t* __restrict__ rec,
                   const uint8_t* __restrict__ deltas, Node* __restrict__ nodes, uint32_t* __restrict__ frontier,
                   uint32_t* __restrict__ ctrl, uint32_t* __restrict__ root)
 {
@@ -251,8 +256,9 @@ __global__ void __launch_bounds__(kLocalLeaves)
     for (uint32_t i = t; i < 3 * s_nOut; i += kLocalLeaves) frontier[3 * (size_t)s_base + i] = s_out[i];
 }
 
+template <class TS>
 __global__ void __launch_bounds__(kBuildBlock)
-    k_climb_upper(const Tri* __restrict__ tris, uint32_t n, const uint8_t* __restrict__ deltas, Node* nodes,
+    k_climb_upper(TS tris, uint32_t n, const uint8_t* __restrict__ deltas, Node* nodes,
                   uint32_t* flags, const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ ctrl,
                   uint32_t* root)
 {
@@ -281,7 +287,8 @@ __global__ void __launch_bounds__(kBuildBlock) k_climb_reset(uint32_t* flags, in
     if (i == 0) ctrl[kCtrlFrontier] = 0;
 }
 
-static int launch_climb(mlbvh_ctx* ctx, const Tri* tris, int n, const uint64_t* rec, const uint8_t* deltas,
+template <class TS>
+static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, const uint8_t* deltas,
                         Node* nodes, uint32_t* root, hipStream_t stream)
 {
     int rc = ensure(&ctx->frontier, &ctx->frontierCap, (size_t)n * 3, "hipMalloc(climb frontier)");
@@ -301,10 +308,11 @@ static int launch_climb(mlbvh_ctx* ctx, const Tri* tris, int n, const uint64_t* 
 
 static inline unsigned grid1(size_t n) { return (unsigned)((n + kBuildBlock - 1) / kBuildBlock); }
 
-int build_one(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_nodes, void* d_root,
-                      uint64_t* d_sorted, uint8_t* d_deltas, hipStream_t stream)
+template <class TS>
+static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_nodes, void* d_root, uint64_t* d_sorted,
+                      uint8_t* d_deltas, hipStream_t stream)
 {
-    if (!ctx || !d_tris || !d_nodes || !d_root) {
+    if (!ctx || !d_nodes || !d_root) {
         set_error("build: null argument");
         return MLBVH_E_INVALID_ARG;
     }
@@ -320,7 +328,6 @@ int build_one(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_n
     if ((rc = ensure(&ctx->deltas, &ctx->deltasCap, nn, "hipMalloc(deltas)"))) return rc;
     if ((rc = ensure(&ctx->flags, &ctx->flagsCap, nn, "hipMalloc(flags)"))) return rc;
 
-    const Tri* tris = (const Tri*)d_tris;
     Node* nodes = (Node*)d_nodes;
     uint32_t* root = (uint32_t*)d_root;
 
@@ -351,6 +358,54 @@ int build_one(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_n
     return MLBVH_OK;
 }
 
+int build_one(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_nodes, void* d_root,
+              uint64_t* d_sorted, uint8_t* d_deltas, hipStream_t stream)
+{
+    if (!d_tris) {
+        set_error("build: null argument");
+        return MLBVH_E_INVALID_ARG;
+    }
+    return build_impl(ctx, (const Tri*)d_tris, n, opt, d_nodes, d_root, d_sorted, d_deltas, stream);
+}
+
+// Indexed-mesh source from the C-ABI arguments, or an error.
+static int indexed_source(const float* d_vertices, int nVertices, const uint32_t* d_indices, const char* who,
+                          IndexedTris* out)
+{
+    if (!d_vertices || !d_indices || nVertices < 1) {
+        set_error("%s: null vertex/index buffer or nVertices = %d", who, nVertices);
+        return MLBVH_E_INVALID_ARG;
+    }
+    out->v = d_vertices;
+    out->f = d_indices;
+    out->vMax = (uint32_t)nVertices - 1u;
+    return MLBVH_OK;
+}
+
+template <class TS>
+static int refit_impl(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* d_sorted_records, const uint8_t* d_deltas,
+                      void* d_nodes, void* d_root, hipStream_t s)
+{
+    if (!ctx || !d_sorted_records || !d_deltas || !d_nodes || !d_root) {
+        set_error("refit: null argument");
+        return MLBVH_E_INVALID_ARG;
+    }
+    if (n < 2) {
+        set_error("refit: n = %d; at least 2 triangles are required", n);
+        return MLBVH_E_INVALID_ARG;
+    }
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = ensure(&ctx->flags, &ctx->flagsCap, (size_t)n, "hipMalloc(flags)"))) return rc;
+    unsigned g = grid1((size_t)n);
+    k_climb_reset<<<g, kBuildBlock, 0, s>>>(ctx->flags, n - 1, ctx->ctrl);
+    if ((rc = launch_climb(ctx, tris, n, d_sorted_records, d_deltas, (Node*)d_nodes, (uint32_t*)d_root, s)))
+        return rc;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_status(e, "refit kernel launch");
+    return MLBVH_OK;
+}
+
 }  // namespace mlbvh
 
 using namespace mlbvh;
@@ -371,24 +426,31 @@ extern "C" int mlbvh_build_ex(mlbvh_ctx* ctx, const void* d_triangles, int n, ui
 extern "C" int mlbvh_refit(mlbvh_ctx* ctx, const void* d_triangles, int n, const uint64_t* d_sorted_records,
                            const uint8_t* d_deltas, void* d_nodes, void* d_root, void* stream)
 {
-    if (!ctx || !d_triangles || !d_sorted_records || !d_deltas || !d_nodes || !d_root) {
+    if (!d_triangles) {
         set_error("refit: null argument");
         return MLBVH_E_INVALID_ARG;
     }
-    if (n < 2) {
-        set_error("refit: n = %d; at least 2 triangles are required", n);
-        return MLBVH_E_INVALID_ARG;
-    }
-    int rc = set_device(ctx);
+    return refit_impl(ctx, (const Tri*)d_triangles, n, d_sorted_records, d_deltas, d_nodes, d_root,
+                      (hipStream_t)stream);
+}
+
+extern "C" int mlbvh_build_indexed(mlbvh_ctx* ctx, const float* d_vertices, int nVertices, const uint32_t* d_indices,
+                                   int nTriangles, uint32_t buildOption, void* d_nodes, void* d_root,
+                                   uint64_t* d_sorted_records, uint8_t* d_deltas, void* stream)
+{
+    IndexedTris src;
+    int rc = indexed_source(d_vertices, nVertices, d_indices, "build_indexed", &src);
     if (rc) return rc;
-    if ((rc = ensure(&ctx->flags, &ctx->flagsCap, (size_t)n, "hipMalloc(flags)"))) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    unsigned g = grid1((size_t)n);
-    k_climb_reset<<<g, kBuildBlock, 0, s>>>(ctx->flags, n - 1, ctx->ctrl);
-    if ((rc = launch_climb(ctx, (const Tri*)d_triangles, n, d_sorted_records, d_deltas, (Node*)d_nodes,
-                           (uint32_t*)d_root, s)))
-        return rc;
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_status(e, "refit kernel launch");
-    return MLBVH_OK;
+    return build_impl(ctx, src, nTriangles, buildOption, d_nodes, d_root, d_sorted_records, d_deltas,
+                      (hipStream_t)stream);
+}
+
+extern "C" int mlbvh_refit_indexed(mlbvh_ctx* ctx, const float* d_vertices, int nVertices, const uint32_t* d_indices,
+                                   int nTriangles, const uint64_t* d_sorted_records, const uint8_t* d_deltas,
+                                   void* d_nodes, void* d_root, void* stream)
+{
+    IndexedTris src;
+    int rc = indexed_source(d_vertices, nVertices, d_indices, "refit_indexed", &src);
+    if (rc) return rc;
+    return refit_impl(ctx, src, nTriangles, d_sorted_records, d_deltas, d_nodes, d_root, (hipStream_t)stream);
 }

@@ -182,6 +182,27 @@ __device__ inline void wait_vector_memory()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// An indexed mesh as a triangle source: u32 index triples into fp32 xyz
+// vertices.  `tris[i]` yields the same Triangle a de-indexed soup holds at i,
+// so every kernel templated on its triangle source (Tri* or IndexedTris)
+// computes bit-identical results from either form.  An index past the last
+// vertex is clamped to it, so a malformed index array cannot read outside the
+// vertex buffer.
+struct IndexedTris {
+    const float* __restrict__ v;
+    const uint32_t* __restrict__ f;
+    uint32_t vMax;  // nVertices - 1
+    __device__ inline Tri operator[](uint32_t i) const
+    {
+        const uint32_t a = f[3 * (size_t)i], b = f[3 * (size_t)i + 1], c = f[3 * (size_t)i + 2];
+        const uint32_t j[3] = {a < vMax ? a : vMax, b < vMax ? b : vMax, c < vMax ? c : vMax};
+        Tri r;
+#pragma unroll
+        for (int k = 0; k < 3; k++) r.v[k] = mk(v[3 * (size_t)j[k]], v[3 * (size_t)j[k] + 1], v[3 * (size_t)j[k] + 2]);
+        return r;
+    }
+};
+
 __device__ inline Box tri_box(const Tri& tri)
 {
     Box box;

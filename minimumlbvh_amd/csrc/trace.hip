@@ -16,9 +16,12 @@ constexpr int kTraceBlock = 256;
 #endif
 constexpr int kAoRefill = MLBVH_AO_REFILL;  // lanes waiting before a wave starts new AO rays
 constexpr uint32_t kAoStack = 16;           // per-lane stack entries of the AO-ray walk
-template <bool kAny, bool kCount>
+// kEscape selects the walk: false the stack-free parent-pointer walk, true
+// the escape-link walk.  TS is the triangle source (`const Tri*` or
+// IndexedTris).
+template <bool kAny, bool kCount, bool kEscape, class TS>
 __global__ void __launch_bounds__(kTraceBlock)
-    k_trace(const Node* __restrict__ nodes, const Tri* __restrict__ tris, const uint32_t* __restrict__ rootPtr,
+    k_trace(const Node* __restrict__ nodes, TS tris, const uint32_t* __restrict__ rootPtr,
             const float4* __restrict__ rays, HitRec* __restrict__ hits, int nRays, unsigned long long* counters)
 {
     int i = blockIdx.x * kTraceBlock + threadIdx.x;
@@ -31,7 +34,10 @@ __global__ void __launch_bounds__(kTraceBlock)
         h.u = h.v = 0.0f;
         h.ng = mk(0, 0, 0);
         h.tri = 0xFFFFFFFFu;
-        traverse<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
+        if (kEscape)
+            traverse_escape<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
+        else
+            traverse<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
         HitRec r;
         r.t = h.t;
         r.pad = 0;
@@ -509,35 +515,81 @@ static dim3 image_grid(int w, int h) { return dim3((unsigned)((w + 15) / 16), (u
 
 using namespace mlbvh;
 
-extern "C" int mlbvh_trace(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles, const void* d_root,
-                           const float* d_rays, void* d_hits, int nRays, int rayQuery, unsigned long long* d_counters,
-                           void* stream)
+template <bool kEscape, class TS>
+static int trace_impl(mlbvh_ctx* ctx, const void* d_nodes, TS tris, const void* d_root, const float* d_rays,
+                      void* d_hits, int nRays, int rayQuery, unsigned long long* d_counters, void* stream,
+                      const char* who)
 {
-    int rc = check_scene(ctx, d_root, d_nodes, d_triangles, "trace");
+    // the callers have checked the triangle source
+    if (!ctx || !d_root || !d_nodes) {
+        set_error("%s: null argument", who);
+        return MLBVH_E_INVALID_ARG;
+    }
+    int rc = set_device(ctx);
     if (rc) return rc;
     if (nRays < 0 || (nRays > 0 && (!d_rays || !d_hits)) ||
         (rayQuery != MLBVH_RAY_QUERY_CLOSEST && rayQuery != MLBVH_RAY_QUERY_ANY)) {
-        set_error("trace: invalid rays/hits/query");
+        set_error("%s: invalid rays/hits/query", who);
         return MLBVH_E_INVALID_ARG;
     }
     if (nRays == 0) return MLBVH_OK;
     hipStream_t s = (hipStream_t)stream;
     unsigned g = (unsigned)((nRays + kTraceBlock - 1) / kTraceBlock);
     const Node* nodes = (const Node*)d_nodes;
-    const Tri* tris = (const Tri*)d_triangles;
     const uint32_t* root = (const uint32_t*)d_root;
     const float4* rays = (const float4*)d_rays;
     HitRec* hits = (HitRec*)d_hits;
     bool any = rayQuery == MLBVH_RAY_QUERY_ANY;
     if (d_counters) {
-        if (any) k_trace<true, true><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
-        else k_trace<false, true><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
+        if (any)
+            k_trace<true, true, kEscape><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
+        else
+            k_trace<false, true, kEscape><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
     } else {
-        if (any) k_trace<true, false><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
-        else k_trace<false, false><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
+        if (any)
+            k_trace<true, false, kEscape><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
+        else
+            k_trace<false, false, kEscape><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
     }
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? MLBVH_OK : hip_status(e, "trace launch");
+    return e == hipSuccess ? MLBVH_OK : hip_status(e, who);
+}
+
+extern "C" int mlbvh_trace(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles, const void* d_root,
+                           const float* d_rays, void* d_hits, int nRays, int rayQuery, unsigned long long* d_counters,
+                           void* stream)
+{
+    if (!d_triangles) {
+        set_error("trace: null argument");
+        return MLBVH_E_INVALID_ARG;
+    }
+    return trace_impl<false>(ctx, d_nodes, (const Tri*)d_triangles, d_root, d_rays, d_hits, nRays, rayQuery,
+                             d_counters, stream, "trace");
+}
+
+extern "C" int mlbvh_trace_escape_link(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,
+                                       const void* d_root, const float* d_rays, void* d_hits, int nRays,
+                                       int rayQuery, unsigned long long* d_counters, void* stream)
+{
+    if (!d_triangles) {
+        set_error("trace_escape_link: null argument");
+        return MLBVH_E_INVALID_ARG;
+    }
+    return trace_impl<true>(ctx, d_nodes, (const Tri*)d_triangles, d_root, d_rays, d_hits, nRays, rayQuery,
+                            d_counters, stream, "trace_escape_link");
+}
+
+extern "C" int mlbvh_trace_indexed(mlbvh_ctx* ctx, const void* d_nodes, const float* d_vertices, int nVertices,
+                                   const uint32_t* d_indices, const void* d_root, const float* d_rays, void* d_hits,
+                                   int nRays, int rayQuery, unsigned long long* d_counters, void* stream)
+{
+    if (!d_vertices || !d_indices || nVertices < 1) {
+        set_error("trace_indexed: null vertex/index buffer or nVertices = %d", nVertices);
+        return MLBVH_E_INVALID_ARG;
+    }
+    IndexedTris src{d_vertices, d_indices, (uint32_t)nVertices - 1u};
+    return trace_impl<false>(ctx, d_nodes, src, d_root, d_rays, d_hits, nRays, rayQuery, d_counters, stream,
+                             "trace_indexed");
 }
 
 extern "C" int mlbvh_trace_batched(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,

@@ -140,9 +140,10 @@ __device__ inline StepOut stackfree_step(const NodeView& nd, uint32_t prev, f3 r
     return stackfree_decide(nd, prev, rl.x <= rl.y, rr.x <= rr.y, rr.x < rl.x, pend);
 }
 
-template <bool kAny, bool kCount>
-__device__ inline void traverse(const Node* __restrict__ nodes, const Tri* __restrict__ tris, uint32_t root, f3 ro,
-                                f3 rd, f3 inv, Hit& hit, uint32_t& nFetch, uint32_t& nTest)
+// TS is the triangle source: `const Tri*` (a soup) or IndexedTris.
+template <bool kAny, bool kCount, class TS>
+__device__ inline void traverse(const Node* __restrict__ nodes, TS tris, uint32_t root, f3 ro, f3 rd, f3 inv,
+                                Hit& hit, uint32_t& nFetch, uint32_t& nTest)
 {
     uint32_t curr = root;
     uint32_t prev = kInvalidNode;
@@ -175,6 +176,73 @@ __device__ inline void traverse(const Node* __restrict__ nodes, const Tri* __res
         if (s.done) break;
         prev = curr;
         curr = s.next;
+    }
+}
+
+// Escape-link walk (intersect_escape_link, minimum_lbvh.h:1063-1137).  An
+// internal-node visit tests ONE child box: the left one when arriving from
+// the parent, else the right one.  A missed left box re-enters the same node
+// (which then tests its right box); a missed right box follows the node's
+// `context` link, the node to resume at once its subtree is done (~0: none).
+// After a leaf, the walk returns to the parent when the leaf was its left
+// child and follows the parent's link otherwise.  Children are always taken
+// left first, so closest-hit ties can resolve differently from the stack-free
+// walk; hit.t is the same.  nFetch counts internal-node visits.
+template <bool kAny, bool kCount, class TS>
+__device__ inline void traverse_escape(const Node* __restrict__ nodes, TS tris, uint32_t root, f3 ro, f3 rd, f3 inv,
+                                       Hit& hit, uint32_t& nFetch, uint32_t& nTest)
+{
+    uint32_t curr = root;
+    uint32_t prev = kInvalidNode;
+    uint32_t guard = kMaxTraversalSteps;
+    while (curr != kInvalidNode && guard-- != 0) {
+        if (curr & kLeafBit) {
+            const Tri tri = tris[curr & ~kLeafBit];
+            if (kCount) nTest++;
+            float t, u, v;
+            f3 ng;
+            if (isect_tri(&t, &u, &v, &ng, 0.0f, hit.t, ro, rd, tri.v[0], tri.v[1], tri.v[2])) {
+                hit.t = t;
+                hit.u = u;
+                hit.v = v;
+                hit.ng = ng;
+                hit.tri = curr & ~kLeafBit;
+                if (kAny) break;
+            }
+            if (prev == kInvalidNode) break;  // a leaf root: nothing else to visit
+            // prev is the leaf's parent: the walk only enters a leaf from it
+            const uint32_t* pw = reinterpret_cast<const uint32_t*>(nodes + prev);
+            if (pw[0] == curr) {
+                const uint32_t tmp = curr;
+                curr = prev;
+                prev = tmp;
+                continue;
+            }
+            const uint32_t link = pw[15];
+            prev = curr;
+            curr = link != 0xFFFFFFFFu ? link : kInvalidNode;
+            continue;
+        }
+        const uint4* p = reinterpret_cast<const uint4*>(nodes + curr);
+        const uint4 a = p[0], b = p[1], c = p[2], d = p[3];
+        if (kCount) nFetch++;
+        const bool descent = prev == d.z;
+        f3 lo, hi;
+        uint32_t hitChild, miss;
+        if (descent) {
+            lo = mk(__uint_as_float(a.z), __uint_as_float(a.w), __uint_as_float(b.x));
+            hi = mk(__uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w));
+            hitChild = a.x;
+            miss = curr;
+        } else {
+            lo = mk(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z));
+            hi = mk(__uint_as_float(c.w), __uint_as_float(d.x), __uint_as_float(d.y));
+            hitChild = a.y;
+            miss = d.w != 0xFFFFFFFFu ? d.w : kInvalidNode;
+        }
+        const float2 r = slabs(ro, inv, lo, hi, hit.t);
+        prev = curr;
+        curr = r.x <= r.y ? hitChild : miss;
     }
 }
 

@@ -136,8 +136,48 @@ class BVHGPUBuilder:
                                        _ptr(self.m_sortedRecords), _ptr(self.m_deltas), _ptr(self.m_internals),
                                        _ptr(self.m_rootNode), _stream(stream)))
 
+    def build_indexed(self, vertices: torch.Tensor, indices: torch.Tensor, buildOption: int = 0, stream=None,
+                      keep_topology: bool = False) -> None:
+        """Build from an indexed mesh (mlbvh_build_indexed): ``vertices`` (V, 3)
+        float32, ``indices`` (T, 3) int32 holding u32 vertex indices.  The nodes
+        equal build() on the de-indexed triangles."""
+        nV, nT = _check_indexed(vertices, indices)
+        if nT < 2:
+            raise ValueError("at least 2 triangles are required")
+        dev = vertices.device
+        if self.m_internals is None or self.m_internals.shape[0] != nT - 1:
+            self.m_internals = torch.empty((nT - 1, 16), dtype=torch.int32, device=dev)
+        self.m_nTriangles = nT
+        if keep_topology:
+            if self.m_sortedRecords is None or self.m_sortedRecords.numel() != nT:
+                self.m_sortedRecords = torch.empty(nT, dtype=torch.int64, device=dev)
+                self.m_deltas = torch.empty(nT - 1, dtype=torch.uint8, device=dev)
+        rec = self.m_sortedRecords if keep_topology else None
+        dl = self.m_deltas if keep_topology else None
+        check(self.ctx.lib.mlbvh_build_indexed(self.ctx.handle, _ptr(vertices), nV, _ptr(indices), nT, buildOption,
+                                               _ptr(self.m_internals), _ptr(self.m_rootNode), _ptr(rec), _ptr(dl),
+                                               _stream(stream)))
+
+    def refit_indexed(self, vertices: torch.Tensor, indices: torch.Tensor, stream=None) -> None:
+        if self.m_sortedRecords is None:
+            raise RuntimeError("refit needs a build(..., keep_topology=True) first")
+        nV, nT = _check_indexed(vertices, indices)
+        if nT != self.m_nTriangles:
+            raise ValueError("triangle count differs from the build")
+        check(self.ctx.lib.mlbvh_refit_indexed(self.ctx.handle, _ptr(vertices), nV, _ptr(indices), nT,
+                                               _ptr(self.m_sortedRecords), _ptr(self.m_deltas),
+                                               _ptr(self.m_internals), _ptr(self.m_rootNode), _stream(stream)))
+
     def empty(self) -> bool:
         return self.m_nTriangles == 0
+
+
+def _check_indexed(vertices: torch.Tensor, indices: torch.Tensor):
+    if vertices.dtype != torch.float32 or vertices.dim() != 2 or vertices.shape[1] != 3:
+        raise ValueError("vertices must be (V, 3) float32")
+    if indices.dtype != torch.int32 or indices.dim() != 2 or indices.shape[1] != 3:
+        raise ValueError("indices must be (T, 3) int32 (u32 vertex indices)")
+    return vertices.shape[0], indices.shape[0]
 
 
 class RayGenerator:
@@ -158,9 +198,12 @@ class RayGenerator:
 
 def trace(builder_or_nodes, triangles: torch.Tensor, rays: torch.Tensor, query: int = RAY_QUERY_CLOSEST,
           root: Optional[torch.Tensor] = None, counters: Optional[torch.Tensor] = None, stream=None,
-          device: int = 0) -> torch.Tensor:
+          device: int = 0, walk: str = "stackfree") -> torch.Tensor:
     """Trace rays (n, 8) float32 [ox,oy,oz,tmax,dx,dy,dz,pad]; returns hits as an
-    (n, 8) int32 tensor holding the 32-byte Hit records."""
+    (n, 8) int32 tensor holding the 32-byte Hit records.  ``walk`` is
+    "stackfree" (intersect_stackfree) or "escape_link" (intersect_escape_link)."""
+    if walk not in ("stackfree", "escape_link"):
+        raise ValueError(f"walk must be 'stackfree' or 'escape_link', not {walk!r}")
     if isinstance(builder_or_nodes, BVHGPUBuilder):
         nodes, root = builder_or_nodes.m_internals, builder_or_nodes.m_rootNode
     else:
@@ -168,8 +211,26 @@ def trace(builder_or_nodes, triangles: torch.Tensor, rays: torch.Tensor, query: 
     n = rays.shape[0]
     hits = torch.empty((n, 8), dtype=torch.int32, device=rays.device)
     ctx = context(device)
-    check(ctx.lib.mlbvh_trace(ctx.handle, _ptr(nodes), _ptr(triangles), _ptr(root), _ptr(rays), _ptr(hits), n, query,
-                              _ptr(counters), _stream(stream)))
+    fn = ctx.lib.mlbvh_trace if walk == "stackfree" else ctx.lib.mlbvh_trace_escape_link
+    check(fn(ctx.handle, _ptr(nodes), _ptr(triangles), _ptr(root), _ptr(rays), _ptr(hits), n, query, _ptr(counters),
+             _stream(stream)))
+    return hits
+
+
+def trace_indexed(builder_or_nodes, vertices: torch.Tensor, indices: torch.Tensor, rays: torch.Tensor,
+                  query: int = RAY_QUERY_CLOSEST, root: Optional[torch.Tensor] = None,
+                  counters: Optional[torch.Tensor] = None, stream=None, device: int = 0) -> torch.Tensor:
+    """trace() over an indexed mesh (mlbvh_trace_indexed)."""
+    if isinstance(builder_or_nodes, BVHGPUBuilder):
+        nodes, root = builder_or_nodes.m_internals, builder_or_nodes.m_rootNode
+    else:
+        nodes = builder_or_nodes
+    nV, _ = _check_indexed(vertices, indices)
+    n = rays.shape[0]
+    hits = torch.empty((n, 8), dtype=torch.int32, device=rays.device)
+    ctx = context(device)
+    check(ctx.lib.mlbvh_trace_indexed(ctx.handle, _ptr(nodes), _ptr(vertices), nV, _ptr(indices), _ptr(root),
+                                      _ptr(rays), _ptr(hits), n, query, _ptr(counters), _stream(stream)))
     return hits
 
 

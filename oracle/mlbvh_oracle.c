@@ -607,9 +607,10 @@ void orc_intersect_stackfull(orc_hit* hit, const orc_node* nodes, const orc_tri*
 }
 
 /* Escape-link traversal using `context` as the miss link,
-   minimum_lbvh.h:1063-1137. */
+   minimum_lbvh.h:1063-1137.  counters (may be NULL): [0] internal-node
+   visits, [1] triangle tests. */
 void orc_intersect_escape_link(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root, orc_v3 ro,
-                               orc_v3 rd, orc_v3 inv, int query)
+                               orc_v3 rd, orc_v3 inv, int query, uint64_t* counters)
 {
     uint32_t curr = root;
     uint32_t prev = ORC_NODE_INVALID;
@@ -618,6 +619,7 @@ void orc_intersect_escape_link(orc_hit* hit, const orc_node* nodes, const orc_tr
             float t, u, v;
             orc_v3 ng;
             const orc_tri* tri = &tris[curr & ~ORC_LEAF_BIT];
+            if (counters) counters[1]++;
             if (orc_intersect_tri(&t, &u, &v, &ng, 0.0f, hit->t, ro, rd, tri->vs[0], tri->vs[1], tri->vs[2])) {
                 hit->t = t;
                 hit->u = u;
@@ -641,6 +643,7 @@ void orc_intersect_escape_link(orc_hit* hit, const orc_node* nodes, const orc_tr
             continue;
         }
         const orc_node* nd = &nodes[curr];
+        if (counters) counters[0]++;
         uint32_t parent_node = nd->parent;
         orc_aabb box;
         uint32_t hitChild, miss;
@@ -684,7 +687,7 @@ void orc_trace(const orc_node* nodes, const orc_tri* tris, uint32_t root, const 
             else if (method == 1)
                 orc_intersect_stackfull(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), stack);
             else
-                orc_intersect_escape_link(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query);
+                orc_intersect_escape_link(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt);
             hits[i] = h;
             c0 += cnt[0];
             c1 += cnt[1];

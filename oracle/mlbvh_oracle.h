@@ -99,7 +99,7 @@ void orc_intersect_stackfree_ex(orc_hit* hit, const orc_node* nodes, const orc_t
 void orc_intersect_stackfull(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root,
                              orc_v3 ro, orc_v3 rd, orc_v3 inv, uint32_t* stack);
 void orc_intersect_escape_link(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root,
-                               orc_v3 ro, orc_v3 rd, orc_v3 inv, int query);
+                               orc_v3 ro, orc_v3 rd, orc_v3 inv, int query, uint64_t* counters);
 /* rays: 8 floats each (ox,oy,oz,tmax,dx,dy,dz,pad); method 0 stackfree, 1 stackfull, 2 escape link,
    3 stackfree with the pending-count early exit */
 void orc_trace(const orc_node* nodes, const orc_tri* tris, uint32_t root, const float* rays, int nRays,

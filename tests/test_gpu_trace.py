@@ -79,6 +79,44 @@ def test_trace_on_gpu_built_bvh(gpu, orc):
     assert np.array_equal(cnt.cpu().numpy().view(np.uint64), oc)
 
 
+@pytest.mark.parametrize("query", [0, 1])
+@pytest.mark.parametrize("n", [2, 1000, 65536])
+def test_trace_escape_link_bit_exact(gpu, orc, n, query):
+    """mlbvh_trace_escape_link against the oracle's escape-link walk: Hit
+    records and visit counters bit-identical; closest-hit t equal to the
+    stack-free walk's."""
+    print(f"escape-link trace n={n} query={query}", flush=True)
+    tris = scenes.random_triangles(n, 70 + n, sigma=0.02)
+    nodes, root, _, _ = orc.build(tris)
+    rays = random_rays(50_000, 3 * n, tmax=None if query == 0 else 0.25)
+    oh, oc = orc.trace(nodes, tris, root, rays, query=query, method=2)
+    nt, rt = upload_bvh(nodes, root)
+    r = torch.from_numpy(rays).cuda()
+    cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+    gh = gpu.trace(nt, torch.from_numpy(tris).cuda(), r, query=query, root=rt, counters=cnt, walk="escape_link")
+    torch.cuda.synchronize()
+    assert np.array_equal(gh.cpu().numpy().view(np.uint32).reshape(-1, 8), oh.view(np.uint32).reshape(-1, 8))
+    assert np.array_equal(cnt.cpu().numpy().view(np.uint64), oc)
+    if query == 0:
+        sf, _ = orc.trace(nodes, tris, root, rays, query=0, method=0)
+        assert np.array_equal(sf["t"], oh["t"])
+
+
+def test_trace_escape_link_on_gpu_built_bvh(gpu, orc):
+    tris = scenes.random_triangles(200_000, 9, sigma=0.01)
+    t = torch.from_numpy(tris).cuda()
+    b = gpu.BVHGPUBuilder()
+    b.build(t, tris.shape[0], 0, gpu.OnesweepSort())
+    rays = random_rays(100_000, 10)
+    gh = gpu.trace(b, t, torch.from_numpy(rays).cuda(), walk="escape_link")
+    torch.cuda.synchronize()
+    nodes, root, _, _ = orc.build(tris)
+    oh, _ = orc.trace(nodes, tris, root, rays, method=2)
+    assert np.array_equal(gh.cpu().numpy().view(np.uint32).reshape(-1, 8), oh.view(np.uint32).reshape(-1, 8))
+    with pytest.raises(ValueError):
+        gpu.trace(b, t, torch.from_numpy(rays[:10]).cuda(), walk="stackful")
+
+
 def small_scene(orc):
     tris = scenes.random_triangles(65536, 0)
     nodes, root, _, _ = orc.build(tris)

@@ -126,3 +126,63 @@ def test_ao_walks_agree(orc):
             assert np.array_equal(px, base_px) and np.array_equal(np.nan_to_num(ao), np.nan_to_num(base_ao))
             assert cnt[0] == base_cnt[0] and cnt[2] == base_cnt[2]
         assert outs[k + 2][2][1] < outs[k + 1][2][1] < base_cnt[1]
+
+
+def fan_triangles(n, seed):
+    """Triangles whose bounding boxes all contain the origin while none of them
+    meets the +z ray from it: a fixed triangle whose xy shadow misses (0, 0)
+    and whose plane crosses the z axis at negative t, under a per-triangle
+    quarter turn about z, an optional mirror in x and per-axis scales (linear
+    maps that fix the origin, so both properties survive)."""
+    base = np.array([[-1.0, -1.0, -1.0], [1.0, -1.0, 1.0], [1.0, 0.9, 0.0]])
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, 4, n)
+    c, s = np.round(np.cos(k * np.pi / 2)), np.round(np.sin(k * np.pi / 2))
+    m = np.zeros((n, 3, 3))
+    m[:, 0, 0], m[:, 0, 1], m[:, 1, 0], m[:, 1, 1], m[:, 2, 2] = c, -s, s, c, 1
+    m[:, 0, :] *= np.where(rng.random(n) < 0.5, -1.0, 1.0)[:, None]
+    m *= rng.uniform(0.5, 2.0, (n, 3, 1))
+    return np.einsum("nij,vj->nvi", m, base).astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [2, 3, 100, 4097])
+def test_walk_counts_closed_form(orc, n):
+    """A ray that enters every box and hits no triangle walks the whole tree:
+    the stack-free walk visits each of the n-1 internal nodes three times
+    (down, back from the near child, back from the far child), the
+    escape-link walk twice (testing the left box, then the right box), and
+    both test each of the n triangles once."""
+    tris = fan_triangles(n, n)
+    lo, hi = tris.min(axis=1), tris.max(axis=1)
+    assert np.all(lo < 0) and np.all(hi > 0)
+    nodes, root, _, _ = orc.build(tris)
+    ray = np.zeros((1, 8), np.float32)
+    ray[0, 3] = np.float32(3.402823466e38)
+    ray[0, 6] = 1.0
+    h0, c0 = orc.trace(nodes, tris, root, ray, method=0)
+    h2, c2 = orc.trace(nodes, tris, root, ray, method=2)
+    assert h0["tri"][0] == 0xFFFFFFFF and h2["tri"][0] == 0xFFFFFFFF
+    assert c0.tolist() == [3 * (n - 1), n]
+    assert c2.tolist() == [2 * (n - 1), n]
+    # a ray pointing away from every box: the root's two boxes, nothing more
+    ray[0, 0:3] = 50.0
+    _, c2 = orc.trace(nodes, tris, root, ray, method=2)
+    assert c2.tolist() == [2, 0]
+
+
+@pytest.mark.parametrize("query,tmax", [(0, None), (1, 0.2)])
+def test_escape_link_bounds_and_occlusion(orc, query, tmax):
+    """The escape-link walk tests each triangle at most once and visits each
+    internal node at most twice per ray; its any-hit occlusion equals the
+    closest-hit query's."""
+    n = 3000
+    tris = scenes.random_triangles(n, 21, sigma=0.03)
+    nodes, root, _, _ = orc.build(tris)
+    rays = random_rays(2000, 22, tmax=tmax)
+    for i in range(0, 2000, 400):
+        _, c = orc.trace(nodes, tris, root, rays[i:i + 1], query=query, method=2)
+        assert c[0] <= 2 * (n - 1) and c[1] <= n
+    h, _ = orc.trace(nodes, tris, root, rays, query=query, method=2)
+    ref, _ = orc.trace(nodes, tris, root, rays, query=0, method=0)
+    assert np.array_equal(h["tri"] != 0xFFFFFFFF, ref["tri"] != 0xFFFFFFFF)
+    assert (h["tri"] != 0xFFFFFFFF).sum() > 50
