@@ -12,6 +12,9 @@ rays traced per second over the K timed steps.  The same run also times:
     uniform u64 keys, each against the 8 TB/s HBM roofline;
   * the 4096-mesh batched build + persistent trace, eager and replayed from a
     captured graph, against the plain per-mesh launch loop;
+  * on the same mesh, 4M closest-hit rays through the stack-free, the
+    escape-link and the indexed-mesh traversal, and the build from the
+    index buffer and with 63-bit keys;
   * 200 Adam steps of the differentiable primary render (level-7 sphere,
     512x512) against its forward-only time;
   * the CPU oracle (full build + the whole AO frame) on this host's cores.
@@ -150,6 +153,10 @@ def main():
         log("[bench] batched build + trace (4096 meshes)")
         workloads.run_batched(max(args.steps, 3), batched)
         torch.cuda.empty_cache()
+    walks = {}
+    if args.level == 9:
+        log("[bench] escape-link / indexed traversal, indexed and 63-bit builds")
+        workloads.run_walks(max(args.steps, 3), walks, sc)
     diffres = {}
     if not args.no_diff:
         log("[bench] differentiable render: 200 Adam steps")
@@ -226,6 +233,7 @@ def main():
             "tile": "x".join(str(v) for v in autotune.SORT_CONFIGS[best_cfg]),
         },
         "batched": batched or None,
+        "walks": walks or None,
         "diff": diffres or None,
         "tuning": autotune.get_tuning(),
     }

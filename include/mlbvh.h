@@ -54,6 +54,13 @@ extern "C" {
 /* build options, minimum_lbvh.h:42-46 */
 #define MLBVH_BUILD_OPTION_USE_NORMAL 1u
 #define MLBVH_BUILD_OPTION_CPU_PARALLEL 2u /* accepted and ignored on the GPU */
+/* No reference counterpart: key every triangle by the full 63-bit Morton code
+ * of its centroid instead of its top 32 bits (minimum_lbvh.h:550), so that
+ * centroids closer than 1/2^10 of the scene box still get distinct keys at
+ * scale.  Deltas become 64 - clz64; sorted records keep the 32-bit layout
+ * (index high, code >> 31 low), so mlbvh_refit works unchanged.  Not
+ * combinable with MLBVH_BUILD_OPTION_USE_NORMAL. */
+#define MLBVH_BUILD_OPTION_MORTON64 4u
 
 /* ray queries, minimum_lbvh.h:975-979 */
 #define MLBVH_RAY_QUERY_CLOSEST 0

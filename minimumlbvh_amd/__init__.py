@@ -8,6 +8,7 @@ and raises at import of the compute classes if it is missing.
 """
 from ._capi import (  # noqa: F401
     BUILD_OPTION_CPU_PARALLEL,
+    BUILD_OPTION_MORTON64,
     BUILD_OPTION_USE_NORMAL,
     RAY_QUERY_ANY,
     RAY_QUERY_CLOSEST,
@@ -17,6 +18,7 @@ from ._capi import (  # noqa: F401
 
 __all__ = [
     "BUILD_OPTION_CPU_PARALLEL",
+    "BUILD_OPTION_MORTON64",
     "BUILD_OPTION_USE_NORMAL",
     "RAY_QUERY_ANY",
     "RAY_QUERY_CLOSEST",

@@ -23,6 +23,7 @@ MLBVH_E_NO_DEVICE = -3
 
 BUILD_OPTION_USE_NORMAL = 1
 BUILD_OPTION_CPU_PARALLEL = 2
+BUILD_OPTION_MORTON64 = 4
 RAY_QUERY_CLOSEST = 0
 RAY_QUERY_ANY = 1
 

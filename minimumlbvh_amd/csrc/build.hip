@@ -108,6 +108,44 @@ __global__ void __launch_bounds__(kBuildBlock)
     for (int p = 0; p < 4; p++) partials[((size_t)blockIdx.x * 4 + p) * 256 + threadIdx.x] = s_h[p][threadIdx.x];
 }
 
+// The 63-bit key variant (MLBVH_BUILD_OPTION_MORTON64, no reference
+// counterpart): the full Morton code of the centroid as a u64 key beside the
+// triangle index, for a key-value sort over bits [0, 63).  Also opens the
+// climb's arrival words, as k_morton does.
+template <class TS>
+__global__ void __launch_bounds__(kBuildBlock)
+    k_morton64(TS tris, int n, const uint32_t* __restrict__ sceneBox, uint64_t* __restrict__ keys,
+               uint32_t* __restrict__ idx, uint32_t* __restrict__ flags, uint32_t* __restrict__ ctrl)
+{
+    const f3 lo = mk(ord2f(sceneBox[0]), ord2f(sceneBox[1]), ord2f(sceneBox[2]));
+    const f3 hi = mk(ord2f(sceneBox[3]), ord2f(sceneBox[4]), ord2f(sceneBox[5]));
+    for (int i = blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += gridDim.x * kBuildBlock) {
+        Tri tri = tris[i];
+        f3 center = ((tri.v[0] + tri.v[1]) + tri.v[2]) / 3.0f;
+        keys[i] = morton_in_box(lo, hi, center);
+        idx[i] = (uint32_t)i;
+        if (i < n - 1) flags[i] = 0xFFFFFFFFu;
+        if (i == 0) ctrl[kCtrlFrontier] = 0;
+    }
+}
+
+// Deltas 64 - clz64(a ^ b) of the sorted 63-bit keys (0 for equal keys), and
+// the records the climb and mlbvh_refit read: index high, code >> 31 low.
+__global__ void __launch_bounds__(kBuildBlock)
+    k_deltas64(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx, int n,
+               uint8_t* __restrict__ deltas, uint64_t* __restrict__ rec)
+{
+    int i = blockIdx.x * kBuildBlock + threadIdx.x;
+    if (i < n) {
+        const uint64_t k = keys[i];
+        rec[i] = (uint64_t)(uint32_t)(k >> 31) | ((uint64_t)idx[i] << 32);
+        if (i < n - 1) {
+            const uint64_t x = k ^ keys[i + 1];
+            deltas[i] = (uint8_t)(x ? 64 - __clzll(x) : 0);
+        }
+    }
+}
+
 // delta(a, b) = 32 - clz(a ^ b) of neighbouring sorted keys
 // (minimum_lbvh.cu:94-102).
 __global__ void __launch_bounds__(kBuildBlock)
@@ -320,6 +358,11 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         set_error("build: n = %d; at least 2 triangles are required", n);
         return MLBVH_E_INVALID_ARG;
     }
+    const bool wide = (opt & MLBVH_BUILD_OPTION_MORTON64) != 0;
+    if (wide && (opt & MLBVH_BUILD_OPTION_USE_NORMAL)) {
+        set_error("build: MORTON64 and USE_NORMAL cannot be combined");
+        return MLBVH_E_INVALID_ARG;
+    }
     int rc = set_device(ctx);
     if (rc) return rc;
     size_t nn = (size_t)n;
@@ -337,13 +380,24 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
     unsigned gAabb = g < (unsigned)ctx->cuCount * 8 ? g : (unsigned)ctx->cuCount * 8;
     k_scene_aabb<<<gAabb, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox);
     unsigned gMorton = g < (unsigned)ctx->cuCount * 4 ? g : (unsigned)ctx->cuCount * 4;
-    if ((rc = ensure_partials(ctx, (int)gMorton, 4))) return rc;
-    k_morton<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials, ctx->flags,
-                                                  ctx->ctrl);
-    launch_hist_reduce(ctx, (int)gMorton, 4, stream);
-    rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream);
-    if (rc) return rc;
-    k_deltas<<<g, kBuildBlock, 0, stream>>>(ctx->recA, n, ctx->deltas);
+    if (wide) {
+        if ((rc = ensure(&ctx->idxA, &ctx->idxCap, nn, "hipMalloc(indices)"))) return rc;
+        if ((rc = ensure(&ctx->idxB, &ctx->idxBCap, nn, "hipMalloc(indices tmp)"))) return rc;
+        k_morton64<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, ctx->recB, ctx->idxA, ctx->flags,
+                                                        ctx->ctrl);
+        // keys in recB (tmp: recA), records written to recA afterwards
+        rc = sort_pairs_u64(ctx, ctx->recB, ctx->idxA, ctx->recA, ctx->idxB, (uint32_t)n, 0, 63, stream);
+        if (rc) return rc;
+        k_deltas64<<<g, kBuildBlock, 0, stream>>>(ctx->recB, ctx->idxA, n, ctx->deltas, ctx->recA);
+    } else {
+        if ((rc = ensure_partials(ctx, (int)gMorton, 4))) return rc;
+        k_morton<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials,
+                                                      ctx->flags, ctx->ctrl);
+        launch_hist_reduce(ctx, (int)gMorton, 4, stream);
+        rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream);
+        if (rc) return rc;
+        k_deltas<<<g, kBuildBlock, 0, stream>>>(ctx->recA, n, ctx->deltas);
+    }
     if ((rc = launch_climb(ctx, tris, n, ctx->recA, ctx->deltas, nodes, root, stream))) return rc;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_status(e, "build kernel launch");

@@ -93,7 +93,8 @@ extern "C" int mlbvh_ctx_destroy(mlbvh_ctx* ctx)
     if (!ctx) return MLBVH_OK;
     hipSetDevice(ctx->device);
     void* bufs[] = {ctx->status, ctx->hist,     ctx->ctrl,     ctx->recA,    ctx->recB,
-                    ctx->deltas, ctx->flags,    ctx->sceneBox, ctx->meshBox, ctx->partials, ctx->frontier};
+                    ctx->deltas, ctx->flags,    ctx->sceneBox, ctx->meshBox, ctx->partials, ctx->frontier,
+                    ctx->idxA,   ctx->idxB};
     int rc = MLBVH_OK;
     for (void* p : bufs) {
         if (!p) continue;

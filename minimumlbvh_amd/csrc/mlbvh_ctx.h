@@ -25,6 +25,10 @@ struct mlbvh_ctx {
     size_t recCap = 0;
     uint64_t* recB = nullptr;  // (pong)
     size_t recBCap = 0;
+    uint32_t* idxA = nullptr;  // triangle indices beside 63-bit keys (MORTON64 build), ping
+    size_t idxCap = 0;
+    uint32_t* idxB = nullptr;  // (pong)
+    size_t idxBCap = 0;
     uint8_t* deltas = nullptr;
     size_t deltasCap = 0;
     uint32_t* flags = nullptr;  // per-internal-node arrival word of the climb
@@ -110,6 +114,10 @@ int set_device(mlbvh_ctx* ctx);
 // has called ensure_status(ctx, sort_tiles(n)).
 int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, int startBit, int endBit,
                      bool histReady, hipStream_t stream);
+// Key-value sort (u64 keys, u32 values) with its own histograms, as
+// mlbvh_sort_u64.
+int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,
+                   int startBit, int endBit, hipStream_t stream);
 uint32_t sort_tiles(mlbvh_ctx* ctx, uint32_t n);
 int sort_tile_keys(mlbvh_ctx* ctx);
 int ensure_status(mlbvh_ctx* ctx, uint32_t tiles);

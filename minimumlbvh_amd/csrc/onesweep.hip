@@ -608,6 +608,12 @@ int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, 
     return sort_impl<uint64_t>(ctx, keys, nullptr, tmp, nullptr, n, startBit, endBit, histReady, stream);
 }
 
+int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,
+                   int startBit, int endBit, hipStream_t stream)
+{
+    return sort_impl<uint64_t>(ctx, keys, vals, ktmp, vtmp, n, startBit, endBit, false, stream);
+}
+
 void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream, int firstHist,
                         const uint32_t* guard)
 {

@@ -16,6 +16,7 @@ constexpr int kTraceBlock = 256;
 #endif
 constexpr int kAoRefill = MLBVH_AO_REFILL;  // lanes waiting before a wave starts new AO rays
 constexpr uint32_t kAoStack = 16;           // per-lane stack entries of the AO-ray walk
+
 // kEscape selects the walk: false the stack-free parent-pointer walk, true
 // the escape-link walk.  TS is the triangle source (`const Tri*` or
 // IndexedTris).

@@ -15,7 +15,7 @@ from typing import Optional
 import torch
 
 from . import _capi
-from ._capi import RAY_QUERY_ANY, RAY_QUERY_CLOSEST, RayGen, check
+from ._capi import BUILD_OPTION_MORTON64, BUILD_OPTION_USE_NORMAL, RAY_QUERY_ANY, RAY_QUERY_CLOSEST, RayGen, check
 
 NODE_BYTES = 64
 TRI_BYTES = 36

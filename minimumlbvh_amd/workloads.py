@@ -6,6 +6,7 @@ profiler can wrap exactly one of them:
     python -m minimumlbvh_amd.workloads sort     [--reps K]   # 2^25 uniform u64 keys, 64 bits
     python -m minimumlbvh_amd.workloads batched  [--reps K]   # 4096 meshes, one build + trace
     python -m minimumlbvh_amd.workloads diff     [--reps K]   # level-7 sphere, 512^2, fwd+bwd steps
+    python -m minimumlbvh_amd.workloads walks    [--reps K]   # escape-link / indexed trace, indexed / 63-bit build
     python -m minimumlbvh_amd.workloads copy     [--reps K]   # 1 GiB device copy (counter calibration)
 
 Scenes:
@@ -209,6 +210,60 @@ def run_batched(reps: int, out: dict, graph: bool = True):
     return sc
 
 
+def sphere_rays(n: int, seed: int = 0, tmax: float = FLT_MAX, device: str = "cuda:0") -> torch.Tensor:
+    """n rays from the sphere of radius 3 about the origin towards points
+    uniform in the unit ball (the level-9 sphere has radius about 1)."""
+    g = torch.Generator(device=device).manual_seed(seed)
+    d = torch.randn((n, 3), generator=g, device=device, dtype=torch.float64)
+    o = 3.0 * d / d.norm(dim=1, keepdim=True)
+    t = torch.randn((n, 3), generator=g, device=device, dtype=torch.float64)
+    t = t / t.norm(dim=1, keepdim=True) * torch.rand((n, 1), generator=g, device=device, dtype=torch.float64) ** (1 / 3)
+    rays = torch.zeros((n, 8), dtype=torch.float32, device=device)
+    rays[:, 0:3] = o.float()
+    rays[:, 3] = tmax
+    rays[:, 4:7] = (t - o).float()
+    return rays.contiguous()
+
+
+def run_walks(reps: int, out: dict, sc: "AOScene | None" = None):
+    """The alternatives beside the headline path, on the level-9 sphere:
+    stack-free vs escape-link traversal and the indexed-mesh trace on 4M
+    closest-hit rays, and the build from the index buffer and with 63-bit
+    keys."""
+    sc = sc or AOScene()
+    sc.build()
+    n = 1 << 22
+    rays = sphere_rays(n)
+    vt = torch.from_numpy(sc.verts).to(rays.device)
+    ft = torch.from_numpy(sc.faces.astype(np.int32)).to(rays.device)
+    walks = {}
+    hits = {}
+    for name, fn in (("stackfree", lambda c=None: lbvh.trace(sc.builder, sc.tris, rays, counters=c)),
+                     ("escape_link", lambda c=None: lbvh.trace(sc.builder, sc.tris, rays, counters=c,
+                                                               walk="escape_link")),
+                     ("indexed", lambda c=None: lbvh.trace_indexed(sc.builder, vt, ft, rays, counters=c))):
+        cnt = torch.zeros(2, dtype=torch.int64, device=rays.device)
+        hits[name] = fn(cnt)
+        ms = events_ms(fn, reps)
+        c = cnt.cpu().numpy().view(np.uint64)
+        walks[name] = {"ms": round(ms, 4), "mrays_per_s": round(n / ms / 1e3, 1), "node_visits": int(c[0]),
+                       "triangle_tests": int(c[1])}
+    torch.cuda.synchronize()
+    out["rays"] = n
+    out["trace"] = walks
+    out["escape_t_equal"] = bool(torch.equal(hits["stackfree"][:, 0], hits["escape_link"][:, 0]))
+    out["indexed_hits_equal"] = bool(torch.equal(hits["stackfree"], hits["indexed"]))
+    ref = sc.builder.m_internals.clone()
+    b = lbvh.BVHGPUBuilder()
+    b.build_indexed(vt, ft)
+    torch.cuda.synchronize()
+    out["indexed_build_nodes_equal"] = bool(torch.equal(b.m_internals, ref))
+    out["build_ms"] = round(events_ms(sc.build, reps), 4)
+    out["indexed_build_ms"] = round(events_ms(lambda: b.build_indexed(vt, ft), reps), 4)
+    out["morton64_build_ms"] = round(events_ms(lambda: b.build(sc.tris, sc.n, lbvh.BUILD_OPTION_MORTON64), reps), 4)
+    return sc
+
+
 def run_diff(steps: int, out: dict):
     """C4: Adam steps (fwd + bwd + update) against forward-only time."""
     sc = DiffScene()
@@ -232,7 +287,7 @@ def run_diff(steps: int, out: dict):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("workload", choices=["ao", "build", "sort", "batched", "diff", "copy"])
+    ap.add_argument("workload", choices=["ao", "build", "sort", "batched", "diff", "walks", "copy"])
     ap.add_argument("--reps", type=int, default=3)
     args = ap.parse_args(argv)
     res = {"workload": args.workload}
@@ -258,6 +313,8 @@ def main(argv=None):
         run_batched(args.reps, res, graph=False)
     elif args.workload == "diff":
         run_diff(max(args.reps, 1), res)
+    elif args.workload == "walks":
+        run_walks(args.reps, res)
     else:
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
         b = torch.empty_like(a)

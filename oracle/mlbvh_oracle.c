@@ -305,12 +305,42 @@ void orc_build_leaf(uint32_t* root, orc_node* nodes, const orc_tri* tris, int n,
     }
 }
 
+/* The 63-bit key variant (build option ORC_OPT_MORTON64, no reference
+   counterpart).  The reference keeps the top 32 of the 63 Morton bits
+   (minimum_lbvh.h:550), so at scale many centroids share a key.  Here the
+   full code of aabb.encodeMortonCode(center) is the key: pairs (code, index)
+   sorted stably by code over bits [0, 63) (8 digit passes), deltas
+   64 - clz64(a ^ b) (0 for equal codes, at most 63).  The exported records
+   keep the layout of the 32-bit variant: high word the triangle index, low
+   word code >> 31. */
+static void orc_morton64_sorted(const orc_tri* tris, int n, const orc_aabb* scene, uint64_t* rec, uint8_t* deltas)
+{
+    uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n);
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        orc_tri tri = tris[i];
+        orc_v3 center = vdivs(vadd(vadd(tri.vs[0], tri.vs[1]), tri.vs[2]), 3.0f);
+        keys[i] = orc_aabb_encode_morton(scene, center);
+        idx[i] = (uint32_t)i;
+    }
+    orc_radix_sort_u64(keys, idx, (uint32_t)n, 0, 63);
+    for (int i = 0; i + 1 < n; i++) {
+        uint64_t x = keys[i] ^ keys[i + 1];
+        deltas[i] = (uint8_t)(x ? 64 - __builtin_clzll(x) : 0);
+    }
+    for (int i = 0; i < n; i++) rec[i] = (uint64_t)(uint32_t)(keys[i] >> 31) | ((uint64_t)idx[i] << 32);
+    free(keys);
+    free(idx);
+}
+
 /* BVHCPUBuilder::build, minimum_lbvh.h:512-592.  n < 2 is rejected (the
-   reference writes out of bounds for n == 1 at :359). */
+   reference writes out of bounds for n == 1 at :359), and so is
+   ORC_OPT_MORTON64 together with ORC_OPT_USE_NORMAL. */
 int orc_build(const orc_tri* tris, int n, uint32_t opt, orc_node* nodes, uint32_t* root, uint64_t* sorted_out,
               uint8_t* deltas_out, int nthreads)
 {
     if (n < 2) return -1;
+    if ((opt & ORC_OPT_MORTON64) && (opt & ORC_OPT_USE_NORMAL)) return -1;
     for (int i = 0; i < n - 1; i++) {
         memset(&nodes[i], 0, sizeof(orc_node));
         nodes[i].context = 0xFFFFFFFFu;
@@ -319,9 +349,13 @@ int orc_build(const orc_tri* tris, int n, uint32_t opt, orc_node* nodes, uint32_
     orc_scene_aabb(tris, n, &scene);
     uint64_t* rec = sorted_out ? sorted_out : (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n);
     uint8_t* deltas = deltas_out ? deltas_out : (uint8_t*)malloc((size_t)n);
-    orc_morton_records(tris, n, &scene, opt, rec);
-    orc_sort_records(rec, n);
-    orc_deltas(rec, n, deltas);
+    if (opt & ORC_OPT_MORTON64) {
+        orc_morton64_sorted(tris, n, &scene, rec, deltas);
+    } else {
+        orc_morton_records(tris, n, &scene, opt, rec);
+        orc_sort_records(rec, n);
+        orc_deltas(rec, n, deltas);
+    }
     if (opt & ORC_OPT_CPU_PARALLEL) {
         int nt = pick_threads(nthreads);
 #pragma omp parallel for schedule(static) num_threads(nt)

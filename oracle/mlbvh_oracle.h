@@ -52,6 +52,7 @@ typedef struct {                                          /* main_gpu.cu:10-15 *
 #define ORC_LEAF_BIT 0x80000000u
 #define ORC_OPT_USE_NORMAL 1u
 #define ORC_OPT_CPU_PARALLEL 2u
+#define ORC_OPT_MORTON64 4u /* no reference counterpart: full 63-bit keys */
 #define ORC_QUERY_CLOSEST 0
 #define ORC_QUERY_ANY 1
 

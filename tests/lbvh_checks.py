@@ -19,8 +19,9 @@ INVALID = np.uint32(0x7FFFFFFF)
 FLT_MAX = np.float32(3.402823466e38)
 
 
-def numpy_morton_keys(tris: np.ndarray, use_normal: bool = False) -> np.ndarray:
-    """32-bit Morton keys restated with numpy float32 arithmetic."""
+def numpy_morton_keys(tris: np.ndarray, use_normal: bool = False, full: bool = False) -> np.ndarray:
+    """32-bit Morton keys restated with numpy float32 arithmetic; with
+    ``full`` the whole 63-bit code (uint64) of the centroid instead."""
     t = np.ascontiguousarray(tris, np.float32).reshape(-1, 3, 3)
     lo = t.reshape(-1, 3).min(axis=0)
     hi = t.reshape(-1, 3).max(axis=0)
@@ -40,6 +41,8 @@ def numpy_morton_keys(tris: np.ndarray, use_normal: bool = False) -> np.ndarray:
                 out |= ((q[:, a] >> np.uint64(i)) & np.uint64(1)) << np.uint64(3 * i + a)
         return out
 
+    if full:
+        return code(quantise(c, lo, hi))
     key = (code(quantise(c, lo, hi)) >> np.uint64(31)).astype(np.uint32)
     if use_normal:
         e1 = t[:, 1] - t[:, 0]
@@ -60,10 +63,14 @@ def stable_order(keys: np.ndarray) -> np.ndarray:
 
 
 def deltas_of(sorted_keys: np.ndarray) -> np.ndarray:
+    """Bit length of the xor of neighbouring keys (exact for 64-bit keys)."""
     x = (sorted_keys[:-1] ^ sorted_keys[1:]).astype(np.uint64)
     d = np.zeros(x.shape[0], np.uint8)
-    nz = x != 0
-    d[nz] = (np.floor(np.log2(x[nz].astype(np.float64))) + 1).astype(np.uint8)
+    for s in (32, 16, 8, 4, 2, 1):
+        big = x >= (np.uint64(1) << np.uint64(s))
+        d[big] += np.uint8(s)
+        x = np.where(big, x >> np.uint64(s), x)
+    d[x != 0] += np.uint8(1)
     return d
 
 

@@ -100,3 +100,43 @@ def test_build_every_sort_config(gpu, orc, cfg):
         gpu.set_tuning(2, 8, 64)
     on, oroot, _, _ = orc.build(tris)
     assert groot == oroot and np.array_equal(gn, on)
+
+
+MORTON64 = 4
+
+
+@pytest.mark.parametrize("n", [2, 1000, 300_001])
+def test_build_morton64_bit_exact(gpu, orc, n):
+    """63-bit keys: nodes, sorted records and deltas bit-identical to the
+    oracle, on a scene with a dense clump (many shared 32-bit keys)."""
+    print(f"build morton64 n={n}", flush=True)
+    tris = scenes.random_triangles(n, 500 + n, sigma=0.0005)
+    tris[: n // 2] = tris[: n // 2] * np.float32(0.002) + np.float32(0.5)
+    gn, groot, b, _ = gpu_build(gpu, tris, opt=MORTON64, keep=True)
+    on, oroot, rec, d = orc.build(tris, opt=MORTON64)
+    assert groot == oroot and np.array_equal(gn, on)
+    assert np.array_equal(b.m_sortedRecords.cpu().numpy().view(np.uint64), rec)
+    assert np.array_equal(b.m_deltas.cpu().numpy(), d)
+
+
+def test_build_morton64_level9_full_size(gpu, orc):
+    """5,242,880 triangles with 63-bit keys: bit-exact vs the oracle, a valid
+    tree, and fewer equal-key runs than the 32-bit keys."""
+    v, f = scenes.displaced_icosphere(9, seed=0)
+    tris = scenes.deindex(v, f)
+    gn, groot, b, _ = gpu_build(gpu, tris, opt=MORTON64, keep=True)
+    gd = b.m_deltas.cpu().numpy()
+    on, oroot, rec, d = orc.build(tris, opt=MORTON64)
+    assert groot == oroot and np.array_equal(gn, on) and np.array_equal(gd, d)
+    C.structural_checks(gn, groot, tris.shape[0])
+    assert orc.validate(groot, gn, d, 64)
+    _, _, _, d32 = orc.build(tris)
+    assert (d == 0).sum() < (d32 == 0).sum()
+
+
+def test_build_morton64_rejects_use_normal(gpu):
+    from minimumlbvh_amd import MlbvhError
+
+    t = torch.from_numpy(scenes.random_triangles(100, 1)).cuda()
+    with pytest.raises(MlbvhError):
+        gpu.BVHGPUBuilder().build(t, 100, MORTON64 | 1)

@@ -147,3 +147,47 @@ def test_validate_detects_violation(orc):
     bad = d.copy()
     bad[root] = 0  # the root must hold the largest delta
     assert not orc.validate(root, nodes, bad)
+
+
+MORTON64 = 4
+
+
+def test_morton64_build_matches_characterisation(orc):
+    """63-bit keys on a clustered scene (many triangles share a 32-bit key):
+    sorted order = stable argsort of the numpy 63-bit codes, deltas = their
+    bit lengths, topology and boxes = the Cartesian-tree characterisation."""
+    tris = scenes.random_triangles(20000, 30, sigma=0.0005)
+    tris[:12000] = tris[:12000] * np.float32(0.002) + np.float32(0.5)  # a dense clump
+    keys32 = C.numpy_morton_keys(tris)
+    assert np.unique(keys32).size < 0.7 * tris.shape[0]
+    nodes, root, rec, d = orc.build(tris, opt=MORTON64)
+    keys = C.numpy_morton_keys(tris, full=True)
+    order = C.stable_order(keys)
+    assert np.array_equal((rec >> np.uint64(32)).astype(np.uint32), order)
+    assert np.array_equal((rec & np.uint64(0xFFFFFFFF)), keys[order] >> np.uint64(31))
+    assert np.array_equal(d, C.deltas_of(keys[order]))
+    assert d.max() <= 63 and np.unique(keys).size > np.unique(keys32).size
+    exp, eroot = C.expected_tree(tris, order, d)
+    assert root == eroot and np.array_equal(nodes, exp)
+    C.structural_checks(nodes, root, tris.shape[0])
+    assert orc.validate(root, nodes, d, 64)
+
+
+def test_morton64_equals_32bit_build_when_keys_are_distinct(orc):
+    """With every 32-bit key distinct the two variants sort alike and each
+    63-bit delta is the 32-bit delta + 31 (the highest differing bit lies in
+    the top 32 bits), a monotone shift: the climb takes the same decisions and
+    the node arrays are identical."""
+    tris = scenes.random_triangles(3000, 5)
+    keys32 = C.numpy_morton_keys(tris)
+    assert np.unique(keys32).size == keys32.size
+    a_nodes, a_root, a_rec, a_d = orc.build(tris)
+    b_nodes, b_root, b_rec, b_d = orc.build(tris, opt=MORTON64)
+    assert np.array_equal(b_rec, a_rec)
+    assert np.array_equal(b_d.astype(int), a_d.astype(int) + 31)
+    assert b_root == a_root and np.array_equal(b_nodes, a_nodes)
+
+
+def test_morton64_rejects_use_normal(orc):
+    with pytest.raises(ValueError):
+        orc.build(scenes.random_triangles(100, 1), opt=MORTON64 | 1)
