@@ -134,12 +134,24 @@ MLBVH_API int mlbvh_refit_indexed(mlbvh_ctx* ctx, const float* d_vertices, int n
                                   int nTriangles, const uint64_t* d_sorted_records, const uint8_t* d_deltas,
                                   void* d_nodes, void* d_root, void* stream);
 
-/* Batch trace with intersect_stackfree (minimum_lbvh.h:981-1061).
- * d_rays: nRays x 32 B Ray; d_hits: nRays x 32 B Hit.  d_counters (nullable):
- * 2 x u64 accumulated [internal-node fetches, triangle tests]. */
+/* Batch trace replacing per-ray intersect_stackfree (minimum_lbvh.h:981-1061).
+ * d_rays: nRays x 32 B Ray; d_hits: nRays x 32 B Hit, bit-identical to
+ * intersect_stackfree's (ties included).  The walk keeps a 16-entry stack of
+ * far children per lane in LDS and re-tests each popped entry's entry
+ * distance against the current hit, which visits the leaves in the stack-free
+ * walk's order while fetching each internal node once per visit (a deeper
+ * stack finishes the ray stack-free).  d_counters (nullable): 2 x u64
+ * accumulated [internal-node fetches, triangle tests] of this walk. */
 MLBVH_API int mlbvh_trace(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles, const void* d_root,
                 const float* d_rays, void* d_hits, int nRays, int rayQuery, unsigned long long* d_counters,
                 void* stream);
+
+/* Batch trace with intersect_stackfree's own parent-pointer walk (with an
+ * exact early exit once no far child is owed): the same arguments and Hit
+ * records as mlbvh_trace; d_counters count this walk's fetches. */
+MLBVH_API int mlbvh_trace_stackfree(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,
+                                    const void* d_root, const float* d_rays, void* d_hits, int nRays, int rayQuery,
+                                    unsigned long long* d_counters, void* stream);
 
 /* Batch trace with intersect_escape_link (minimum_lbvh.h:1063-1137): the
  * same arguments and records as mlbvh_trace.  Closest-hit t equals

@@ -43,6 +43,7 @@ SIGNATURES = {
     "mlbvh_refit_indexed": ([_P, _P, c_int, _P, c_int, _P, _P, _P, _P, _P], c_int),
     "mlbvh_trace": ([_P, _P, _P, _P, _P, _P, c_int, c_int, _P, _P], c_int),
     "mlbvh_trace_escape_link": ([_P, _P, _P, _P, _P, _P, c_int, c_int, _P, _P], c_int),
+    "mlbvh_trace_stackfree": ([_P, _P, _P, _P, _P, _P, c_int, c_int, _P, _P], c_int),
     "mlbvh_trace_indexed": ([_P, _P, _P, c_int, _P, _P, _P, _P, c_int, c_int, _P, _P], c_int),
     "mlbvh_build_batched": ([_P, _P, _P, c_int, c_uint32, c_uint32, c_uint32, _P, _P, _P], c_int),
     "mlbvh_trace_batched": ([_P, _P, _P, _P, _P, c_int, _P, _P, c_int, c_int, _P, _P], c_int),

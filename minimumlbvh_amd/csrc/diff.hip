@@ -56,7 +56,10 @@ __global__ void __launch_bounds__(kDiffBlock)
     hit.ng = mk(0, 0, 0);
     hit.tri = 0xFFFFFFFFu;
     uint32_t nFetch = 0, nTest = 0;
-    traverse<false, false>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), hit, nFetch, nTest);
+    __shared__ uint32_t s_stack[kLaneStack][kDiffBlock];
+    __shared__ float s_entry[kLaneStack][kDiffBlock];
+    traverse_stack<false, false, kDiffBlock>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), hit, nFetch, nTest,
+                                             &s_stack[0][threadIdx.x], &s_entry[0][threadIdx.x]);
     const float t = hit.t;
     hitTri[p] = hit.tri;
     if (hit.tri == 0xFFFFFFFFu) {

@@ -1,10 +1,15 @@
-// Stack-free parent-pointer traversal (intersect_stackfree,
-// minimum_lbvh.h:981-1061) and the kernels it drives: batch trace and the
-// normal / AO / path-tracing / pack kernels of main_gpu.cu.
+// Ray traversal kernels: batch trace (the replacement for calling
+// intersect_stackfree, minimum_lbvh.h:981-1061, per ray) and the normal / AO /
+// path-tracing / pack kernels of main_gpu.cu.
 //
 // One ray per lane.  The image kernels give each wave64 an 8x8 pixel tile
 // (a 256-thread block covers 16x16 pixels) so neighbouring lanes walk the
-// same part of the tree.  The walk itself is in traverse.h.
+// same part of the tree.  The walks are in traverse.h: every kernel finds its
+// hits with the per-lane LDS stack walk, which tests the leaves of the
+// stack-free walk in the same order (so the Hit records are identical) but
+// fetches each internal node once per visit; the stack-free and escape-link
+// walks stay available through mlbvh_trace_stackfree /
+// mlbvh_trace_escape_link.
 #include "mlbvh_ctx.h"
 #include "traverse.h"
 
@@ -12,19 +17,26 @@ namespace mlbvh {
 
 constexpr int kTraceBlock = 256;
 #ifndef MLBVH_AO_REFILL
-#define MLBVH_AO_REFILL 1
+#define MLBVH_AO_REFILL 8
 #endif
 constexpr int kAoRefill = MLBVH_AO_REFILL;  // lanes waiting before a wave starts new AO rays
 constexpr uint32_t kAoStack = 16;           // per-lane stack entries of the AO-ray walk
 
-// kEscape selects the walk: false the stack-free parent-pointer walk, true
-// the escape-link walk.  TS is the triangle source (`const Tri*` or
-// IndexedTris).
-template <bool kAny, bool kCount, bool kEscape, class TS>
+
+// Walks of k_trace.  kWalkStack: traverse_stack, the hits of
+// intersect_stackfree with each internal node fetched once per visit (the
+// default); kWalkStackFree: the parent-pointer walk itself (traverse);
+// kWalkEscape: the escape-link walk (traverse_escape).
+enum : int { kWalkStack = 0, kWalkStackFree = 1, kWalkEscape = 2 };
+
+// One ray per lane.  TS is the triangle source (`const Tri*` or IndexedTris).
+template <bool kAny, bool kCount, int kWalk, class TS>
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace(const Node* __restrict__ nodes, TS tris, const uint32_t* __restrict__ rootPtr,
             const float4* __restrict__ rays, HitRec* __restrict__ hits, int nRays, unsigned long long* counters)
 {
+    __shared__ uint32_t s_stack[kWalk == kWalkStack ? kLaneStack : 1][kTraceBlock];
+    __shared__ float s_entry[kWalk == kWalkStack ? kLaneStack : 1][kTraceBlock];
     int i = blockIdx.x * kTraceBlock + threadIdx.x;
     uint32_t nFetch = 0, nTest = 0;
     if (i < nRays) {
@@ -35,8 +47,11 @@ __global__ void __launch_bounds__(kTraceBlock)
         h.u = h.v = 0.0f;
         h.ng = mk(0, 0, 0);
         h.tri = 0xFFFFFFFFu;
-        if (kEscape)
+        if (kWalk == kWalkEscape)
             traverse_escape<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
+        else if (kWalk == kWalkStack)
+            traverse_stack<kAny, kCount, kTraceBlock>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest,
+                                                      &s_stack[0][threadIdx.x], &s_entry[0][threadIdx.x]);
         else
             traverse<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
         HitRec r;
@@ -60,8 +75,8 @@ __global__ void k_queue_reset(uint32_t* counter) { *counter = 0; }
 
 // Persistent work-queue trace over the meshes of a batched build.  Each wave
 // pulls `chunk` consecutive rays per atomic and traces chunk/64 of them per
-// lane; a ray names its mesh in the Ray pad word (u32).  Hits carry
-// mesh-local triangle indices.
+// lane with traverse_stack; a ray names its mesh in the Ray pad word (u32).
+// Hits carry mesh-local triangle indices.
 template <bool kAny, bool kCount>
 __global__ void __launch_bounds__(kTraceBlock)
     k_trace_batched(const Node* __restrict__ nodesAll, const Tri* __restrict__ trisAll,
@@ -69,6 +84,8 @@ __global__ void __launch_bounds__(kTraceBlock)
                     const float4* __restrict__ rays, HitRec* __restrict__ hits, uint32_t nRays, uint32_t chunk,
                     uint32_t* queue, unsigned long long* counters)
 {
+    __shared__ uint32_t s_stack[kLaneStack][kTraceBlock];
+    __shared__ float s_entry[kLaneStack][kTraceBlock];
     const int lane = threadIdx.x & 63;
     uint32_t nFetch = 0, nTest = 0;
     while (true) {
@@ -89,8 +106,9 @@ __global__ void __launch_bounds__(kTraceBlock)
             if (m < (uint32_t)nMeshes) {
                 uint32_t b0 = off[m];
                 f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
-                traverse<kAny, kCount>(nodesAll + (b0 - m), trisAll + b0, roots[m], ro, rd, inv_rd(rd), h, nFetch,
-                                       nTest);
+                traverse_stack<kAny, kCount, kTraceBlock>(nodesAll + (b0 - m), trisAll + b0, roots[m], ro, rd,
+                                                          inv_rd(rd), h, nFetch, nTest, &s_stack[0][threadIdx.x],
+                                                          &s_entry[0][threadIdx.x]);
             }
             HitRec r;
             r.t = h.t;
@@ -230,7 +248,10 @@ __global__ void __launch_bounds__(kTraceBlock)
     hit.ng = mk(0, 0, 0);
     hit.tri = 0xFFFFFFFFu;
     uint32_t a = 0, b = 0;
-    traverse<false, false>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), hit, a, b);
+    __shared__ uint32_t s_stack[kLaneStack][kTraceBlock];
+    __shared__ float s_entry[kLaneStack][kTraceBlock];
+    traverse_stack<false, false, kTraceBlock>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), hit, a, b,
+                                              &s_stack[0][threadIdx.x], &s_entry[0][threadIdx.x]);
     uint32_t px;
     if (hit.t != kFltMax) {
         f3 n = normalize(hit.ng);
@@ -256,19 +277,26 @@ __global__ void __launch_bounds__(kTraceBlock)
     const int pixel = xi + yi * w;
 
     // All of a lane's rays run in one "while-while" loop: first its primary ray
-    // (closest hit, the reference's stack-free walk and visit order), then its
-    // AO rays.  A lane whose ray ends takes its next one without waiting for
-    // the rest of the wave, so a wave is not held up by its slowest primary
-    // ray before any AO ray starts.  An AO ray only asks whether anything lies
-    // within `radius`, and that answer does not depend on the visit order, so
-    // AO rays walk with an explicit per-lane stack in LDS (near child first,
-    // far child pushed): every internal node is fetched once, where the
-    // stack-free walk fetches it again on the way back up.  A push past
-    // kAoStack entries restarts that AO ray with the stack-free walk.  The
-    // oracle's orc_render_ao (walk 2) does the same, counters included.
-    // Node fetch, slab tests and triangle test are shared by both walks; only
-    // the choice of the next node differs.
+    // (closest hit), then its AO rays.  A lane whose ray ends takes its next
+    // one without waiting for the rest of the wave, so a wave is not held up
+    // by its slowest primary ray before any AO ray starts.
+    //
+    // Every ray walks with an explicit per-lane stack in LDS: near child
+    // first, the far child pushed with its entry distance; a popped entry is
+    // skipped unless its entry distance is still <= tmax.  Every internal node
+    // is fetched once, where the stack-free walk fetches it again on the way
+    // back up, and the leaves are tested in the stack-free walk's order, so
+    // the primary hit (ties included) is the reference walk's: back at a node
+    // from its near child, intersect_stackfree re-tests the far box against
+    // the current hit distance, which is this entry-distance test.  An AO ray
+    // only asks whether anything lies within `radius` (its tmax never
+    // changes, so no entry is skipped).  A push past kAoStack entries
+    // restarts that ray with the stack-free walk from the root, keeping the
+    // primary ray's hit so far.  The oracle's orc_render_ao (walk 3) does the
+    // same, counters included.  Node fetch, slab tests and triangle test are
+    // shared by both walks; only the choice of the next node differs.
     __shared__ uint32_t s_stack[kAoStack][kTraceBlock];
+    __shared__ float s_entry[kAoStack][kTraceBlock];
     const int t = threadIdx.x;
     f3 o = mk(0, 0, 0), d = mk(0, 0, 1), iv = mk(0, 0, 1);  // the lane's current ray
     float tmax = kFltMax;                                    // primary: closest t so far; AO: radius
@@ -276,7 +304,7 @@ __global__ void __launch_bounds__(kTraceBlock)
     bool primary = inImage;       // the primary ray is in flight
     bool live = inImage;          // a ray is in flight
     bool waiting = false;         // the next AO sample is due
-    bool stackFree = true;        // the current ray walks stack-free
+    bool stackFree = false;       // the current ray walks stack-free (after a stack overflow)
     bool hasHit = false;
     uint32_t curr = kInvalidNode, prev = kInvalidNode, pend = 0, sp = 0;
     if (inImage) {
@@ -316,7 +344,7 @@ __global__ void __launch_bounds__(kTraceBlock)
             }
         }
         if (!live) continue;
-        bool done = false, blocked = false;
+        bool done = false, blocked = false, pop = false;
         if (curr & kLeafBit) {
             const Tri tri = tris[curr & ~kLeafBit];
             if (kCount) nTest++;
@@ -334,10 +362,8 @@ __global__ void __launch_bounds__(kTraceBlock)
                     uint32_t tmp = curr;
                     curr = prev;
                     prev = tmp;
-                } else if (sp == 0) {
-                    done = true;
                 } else {
-                    curr = s_stack[--sp][t];
+                    pop = true;
                 }
             }
         } else {
@@ -355,21 +381,32 @@ __global__ void __launch_bounds__(kTraceBlock)
                     curr = s.next;
                 }
             } else if (hitL && hitR) {
-                if (sp == kAoStack) {  // stack full: restart this AO ray stack-free
+                if (sp == kAoStack) {  // stack full: restart this ray stack-free
                     stackFree = true;
                     curr = root;
                     prev = kInvalidNode;
                     pend = 0;
                 } else {
-                    s_stack[sp++][t] = rFirst ? nd.c0 : nd.c1;
+                    s_stack[sp][t] = rFirst ? nd.c0 : nd.c1;
+                    s_entry[sp][t] = rFirst ? rl.x : rr.x;
+                    sp++;
                     curr = rFirst ? nd.c1 : nd.c0;
                 }
             } else if (hitL || hitR) {
                 curr = hitL ? nd.c0 : nd.c1;
-            } else if (sp == 0) {
-                done = true;
             } else {
-                curr = s_stack[--sp][t];
+                pop = true;
+            }
+        }
+        if (pop) {  // the next far child still entered within tmax, if any
+            done = true;
+            while (sp != 0) {
+                --sp;
+                if (s_entry[sp][t] <= tmax) {
+                    curr = s_stack[sp][t];
+                    done = false;
+                    break;
+                }
             }
         }
         if (done) {
@@ -425,6 +462,10 @@ __global__ void __launch_bounds__(kTraceBlock)
                 const Node* __restrict__ nodes, const Tri* __restrict__ tris, const Attrib* __restrict__ attribs,
                 int useSobol, int maxSPP, int maxDepth)
 {
+    __shared__ uint32_t s_stack[kLaneStack][kTraceBlock];
+    __shared__ float s_entry[kLaneStack][kTraceBlock];
+    uint32_t* const stk = &s_stack[0][threadIdx.x];
+    float* const ent = &s_entry[0][threadIdx.x];
     int xi, yi;
     tile_pixel(&xi, &yi);
     if (xi >= w || yi >= h) return;
@@ -453,7 +494,7 @@ __global__ void __launch_bounds__(kTraceBlock)
     hit.ng = mk(0, 0, 0);
     hit.tri = 0xFFFFFFFFu;
     uint32_t c0 = 0, c1 = 0;
-    traverse<false, false>(nodes, tris, root, ro, rd, inv_rd(rd), hit, c0, c1);
+    traverse_stack<false, false, kTraceBlock>(nodes, tris, root, ro, rd, inv_rd(rd), hit, c0, c1, stk, ent);
     if (hit.t != kFltMax) {
         for (int depth = 0; depth < maxDepth; depth++) {
             next2(&r0, &r1);
@@ -478,7 +519,7 @@ __global__ void __launch_bounds__(kTraceBlock)
             hit.u = hit.v = 0.0f;
             hit.ng = mk(0, 0, 0);
             hit.tri = 0xFFFFFFFFu;
-            traverse<false, false>(nodes, tris, root, nro, nrd, inv_rd(nrd), hit, c0, c1);
+            traverse_stack<false, false, kTraceBlock>(nodes, tris, root, nro, nrd, inv_rd(nrd), hit, c0, c1, stk, ent);
             if (hit.t == kFltMax) break;
             ro = nro;
             rd = nrd;
@@ -516,7 +557,7 @@ static dim3 image_grid(int w, int h) { return dim3((unsigned)((w + 15) / 16), (u
 
 using namespace mlbvh;
 
-template <bool kEscape, class TS>
+template <int kWalk, class TS>
 static int trace_impl(mlbvh_ctx* ctx, const void* d_nodes, TS tris, const void* d_root, const float* d_rays,
                       void* d_hits, int nRays, int rayQuery, unsigned long long* d_counters, void* stream,
                       const char* who)
@@ -543,14 +584,14 @@ static int trace_impl(mlbvh_ctx* ctx, const void* d_nodes, TS tris, const void* 
     bool any = rayQuery == MLBVH_RAY_QUERY_ANY;
     if (d_counters) {
         if (any)
-            k_trace<true, true, kEscape><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
+            k_trace<true, true, kWalk><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
         else
-            k_trace<false, true, kEscape><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
+            k_trace<false, true, kWalk><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
     } else {
         if (any)
-            k_trace<true, false, kEscape><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
+            k_trace<true, false, kWalk><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
         else
-            k_trace<false, false, kEscape><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
+            k_trace<false, false, kWalk><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? MLBVH_OK : hip_status(e, who);
@@ -564,8 +605,20 @@ extern "C" int mlbvh_trace(mlbvh_ctx* ctx, const void* d_nodes, const void* d_tr
         set_error("trace: null argument");
         return MLBVH_E_INVALID_ARG;
     }
-    return trace_impl<false>(ctx, d_nodes, (const Tri*)d_triangles, d_root, d_rays, d_hits, nRays, rayQuery,
-                             d_counters, stream, "trace");
+    return trace_impl<kWalkStack>(ctx, d_nodes, (const Tri*)d_triangles, d_root, d_rays, d_hits, nRays, rayQuery,
+                                  d_counters, stream, "trace");
+}
+
+extern "C" int mlbvh_trace_stackfree(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,
+                                     const void* d_root, const float* d_rays, void* d_hits, int nRays, int rayQuery,
+                                     unsigned long long* d_counters, void* stream)
+{
+    if (!d_triangles) {
+        set_error("trace_stackfree: null argument");
+        return MLBVH_E_INVALID_ARG;
+    }
+    return trace_impl<kWalkStackFree>(ctx, d_nodes, (const Tri*)d_triangles, d_root, d_rays, d_hits, nRays,
+                                      rayQuery, d_counters, stream, "trace_stackfree");
 }
 
 extern "C" int mlbvh_trace_escape_link(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,
@@ -576,8 +629,8 @@ extern "C" int mlbvh_trace_escape_link(mlbvh_ctx* ctx, const void* d_nodes, cons
         set_error("trace_escape_link: null argument");
         return MLBVH_E_INVALID_ARG;
     }
-    return trace_impl<true>(ctx, d_nodes, (const Tri*)d_triangles, d_root, d_rays, d_hits, nRays, rayQuery,
-                            d_counters, stream, "trace_escape_link");
+    return trace_impl<kWalkEscape>(ctx, d_nodes, (const Tri*)d_triangles, d_root, d_rays, d_hits, nRays, rayQuery,
+                                   d_counters, stream, "trace_escape_link");
 }
 
 extern "C" int mlbvh_trace_indexed(mlbvh_ctx* ctx, const void* d_nodes, const float* d_vertices, int nVertices,
@@ -589,8 +642,8 @@ extern "C" int mlbvh_trace_indexed(mlbvh_ctx* ctx, const void* d_nodes, const fl
         return MLBVH_E_INVALID_ARG;
     }
     IndexedTris src{d_vertices, d_indices, (uint32_t)nVertices - 1u};
-    return trace_impl<false>(ctx, d_nodes, src, d_root, d_rays, d_hits, nRays, rayQuery, d_counters, stream,
-                             "trace_indexed");
+    return trace_impl<kWalkStack>(ctx, d_nodes, src, d_root, d_rays, d_hits, nRays, rayQuery, d_counters, stream,
+                                  "trace_indexed");
 }
 
 extern "C" int mlbvh_trace_batched(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,

@@ -179,6 +179,76 @@ __device__ inline void traverse(const Node* __restrict__ nodes, TS tris, uint32_
     }
 }
 
+// Closest- or any-hit walk with an explicit per-lane stack of kLaneStack
+// entries in LDS (entry k of this lane at stackNode[k * kStride]): near child
+// first, the far child pushed with its entry distance; a popped entry is
+// skipped unless that distance is still <= hit.t.  It tests the leaves of the
+// stack-free walk in the same order and ends on the same Hit, ties included:
+// entry distances, which order the children, do not depend on hit.t, and back
+// at a node from its near child the stack-free walk takes the far child iff
+// its box is still entered, i.e. iff its entry distance is <= hit.t (its exit
+// bound held when it was pushed).  Each internal node is fetched once per
+// visit instead of again on the way back up.  A push past kLaneStack entries
+// finishes the ray with the stack-free walk from the root, keeping the hit so
+// far: every triangle at the final t is tested in both walks in the same
+// depth-first order, and the last of them wins in both.
+constexpr uint32_t kLaneStack = 16;
+template <bool kAny, bool kCount, int kStride, class TS>
+__device__ inline void traverse_stack(const Node* __restrict__ nodes, TS tris, uint32_t root, f3 ro, f3 rd, f3 inv,
+                                      Hit& hit, uint32_t& nFetch, uint32_t& nTest, uint32_t* stackNode,
+                                      float* stackEntry)
+{
+    uint32_t curr = root;
+    uint32_t sp = 0;
+    uint32_t guard = kMaxTraversalSteps;
+    while (guard-- != 0) {
+        bool pop = false;
+        if (curr & kLeafBit) {
+            const Tri tri = tris[curr & ~kLeafBit];
+            if (kCount) nTest++;
+            float t, u, v;
+            f3 ng;
+            if (isect_tri(&t, &u, &v, &ng, 0.0f, hit.t, ro, rd, tri.v[0], tri.v[1], tri.v[2])) {
+                hit.t = t;
+                hit.u = u;
+                hit.v = v;
+                hit.ng = ng;
+                hit.tri = curr & ~kLeafBit;
+                if (kAny) return;
+            }
+            pop = true;
+        } else {
+            const NodeView nd = load_node(nodes, curr);
+            if (kCount) nFetch++;
+            const float2 rl = slabs(ro, inv, nd.lo0, nd.hi0, hit.t);
+            const float2 rr = slabs(ro, inv, nd.lo1, nd.hi1, hit.t);
+            const bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y, rFirst = rr.x < rl.x;
+            if (hitL && hitR) {
+                if (sp == kLaneStack) {
+                    traverse<kAny, kCount>(nodes, tris, root, ro, rd, inv, hit, nFetch, nTest);
+                    return;
+                }
+                stackNode[sp * kStride] = rFirst ? nd.c0 : nd.c1;
+                stackEntry[sp * kStride] = rFirst ? rl.x : rr.x;
+                sp++;
+                curr = rFirst ? nd.c1 : nd.c0;
+            } else if (hitL || hitR) {
+                curr = hitL ? nd.c0 : nd.c1;
+            } else {
+                pop = true;
+            }
+        }
+        if (pop) {
+            while (true) {
+                if (sp == 0) return;
+                --sp;
+                if (stackEntry[sp * kStride] <= hit.t) break;
+            }
+            curr = stackNode[sp * kStride];
+        }
+    }
+}
+
 // Escape-link walk (intersect_escape_link, minimum_lbvh.h:1063-1137).  An
 // internal-node visit tests ONE child box: the left one when arriving from
 // the parent, else the right one.  A missed left box re-enters the same node

@@ -198,12 +198,15 @@ class RayGenerator:
 
 def trace(builder_or_nodes, triangles: torch.Tensor, rays: torch.Tensor, query: int = RAY_QUERY_CLOSEST,
           root: Optional[torch.Tensor] = None, counters: Optional[torch.Tensor] = None, stream=None,
-          device: int = 0, walk: str = "stackfree") -> torch.Tensor:
+          device: int = 0, walk: str = "stack") -> torch.Tensor:
     """Trace rays (n, 8) float32 [ox,oy,oz,tmax,dx,dy,dz,pad]; returns hits as an
-    (n, 8) int32 tensor holding the 32-byte Hit records.  ``walk`` is
-    "stackfree" (intersect_stackfree) or "escape_link" (intersect_escape_link)."""
-    if walk not in ("stackfree", "escape_link"):
-        raise ValueError(f"walk must be 'stackfree' or 'escape_link', not {walk!r}")
+    (n, 8) int32 tensor holding the 32-byte Hit records.  ``walk``: "stack"
+    (mlbvh_trace: intersect_stackfree's hits, per-lane LDS stack),
+    "stackfree" (intersect_stackfree's own walk) or "escape_link"
+    (intersect_escape_link)."""
+    fns = {"stack": "mlbvh_trace", "stackfree": "mlbvh_trace_stackfree", "escape_link": "mlbvh_trace_escape_link"}
+    if walk not in fns:
+        raise ValueError(f"walk must be one of {sorted(fns)}, not {walk!r}")
     if isinstance(builder_or_nodes, BVHGPUBuilder):
         nodes, root = builder_or_nodes.m_internals, builder_or_nodes.m_rootNode
     else:
@@ -211,7 +214,7 @@ def trace(builder_or_nodes, triangles: torch.Tensor, rays: torch.Tensor, query: 
     n = rays.shape[0]
     hits = torch.empty((n, 8), dtype=torch.int32, device=rays.device)
     ctx = context(device)
-    fn = ctx.lib.mlbvh_trace if walk == "stackfree" else ctx.lib.mlbvh_trace_escape_link
+    fn = getattr(ctx.lib, fns[walk])
     check(fn(ctx.handle, _ptr(nodes), _ptr(triangles), _ptr(root), _ptr(rays), _ptr(hits), n, query, _ptr(counters),
              _stream(stream)))
     return hits

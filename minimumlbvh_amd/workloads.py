@@ -226,10 +226,10 @@ def sphere_rays(n: int, seed: int = 0, tmax: float = FLT_MAX, device: str = "cud
 
 
 def run_walks(reps: int, out: dict, sc: "AOScene | None" = None):
-    """The alternatives beside the headline path, on the level-9 sphere:
-    stack-free vs escape-link traversal and the indexed-mesh trace on 4M
-    closest-hit rays, and the build from the index buffer and with 63-bit
-    keys."""
+    """The alternatives beside the headline path, on the level-9 sphere: the
+    stack walk, the stack-free walk, the escape-link walk and the indexed-mesh
+    trace on 4M closest-hit rays, and the build from the index buffer and
+    with 63-bit keys."""
     sc = sc or AOScene()
     sc.build()
     n = 1 << 22
@@ -238,7 +238,9 @@ def run_walks(reps: int, out: dict, sc: "AOScene | None" = None):
     ft = torch.from_numpy(sc.faces.astype(np.int32)).to(rays.device)
     walks = {}
     hits = {}
-    for name, fn in (("stackfree", lambda c=None: lbvh.trace(sc.builder, sc.tris, rays, counters=c)),
+    for name, fn in (("stack", lambda c=None: lbvh.trace(sc.builder, sc.tris, rays, counters=c)),
+                     ("stackfree", lambda c=None: lbvh.trace(sc.builder, sc.tris, rays, counters=c,
+                                                             walk="stackfree")),
                      ("escape_link", lambda c=None: lbvh.trace(sc.builder, sc.tris, rays, counters=c,
                                                                walk="escape_link")),
                      ("indexed", lambda c=None: lbvh.trace_indexed(sc.builder, vt, ft, rays, counters=c))):
@@ -251,8 +253,9 @@ def run_walks(reps: int, out: dict, sc: "AOScene | None" = None):
     torch.cuda.synchronize()
     out["rays"] = n
     out["trace"] = walks
-    out["escape_t_equal"] = bool(torch.equal(hits["stackfree"][:, 0], hits["escape_link"][:, 0]))
-    out["indexed_hits_equal"] = bool(torch.equal(hits["stackfree"], hits["indexed"]))
+    out["stackfree_hits_equal"] = bool(torch.equal(hits["stack"], hits["stackfree"]))
+    out["escape_t_equal"] = bool(torch.equal(hits["stack"][:, 0], hits["escape_link"][:, 0]))
+    out["indexed_hits_equal"] = bool(torch.equal(hits["stack"], hits["indexed"]))
     ref = sc.builder.m_internals.clone()
     b = lbvh.BVHGPUBuilder()
     b.build_indexed(vt, ft)

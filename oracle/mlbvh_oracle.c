@@ -720,6 +720,8 @@ void orc_trace(const orc_node* nodes, const orc_tri* tris, uint32_t root, const 
                 orc_intersect_stackfree_ex(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt, method == 3);
             else if (method == 1)
                 orc_intersect_stackfull(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), stack);
+            else if (method == 4)
+                orc_closest_hit_stack(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt, ORC_AO_STACK);
             else
                 orc_intersect_escape_link(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt);
             hits[i] = h;
@@ -990,6 +992,113 @@ int orc_any_hit_stack(const orc_node* nodes, const orc_tri* tris, uint32_t root,
     }
 }
 
+/* Closest (or any) hit with an explicit stack of `cap` (<= 64) entries, each
+   a far child and its entry distance.  It tests the leaves of
+   intersect_stackfree (minimum_lbvh.h:981-1061) in the same order and so ends
+   on the same Hit, ties included:
+   - descending, both walks order the children by entry distance (rmin),
+     which does not depend on hit.t;
+   - back at a node from its near child, the stack-free walk re-runs both
+     slab tests with the current hit.t and takes the far child iff it is still
+     entered.  Its box's own exit bound held when it was pushed, so that is
+     rmin_far <= hit.t, the test a popped entry takes here.  (A near child
+     culled on the way back implies a culled far child: rmin_far >= rmin_near.)
+   A push past cap restarts the walk stack-free from the root with the hit
+   found so far.  That walk also ends on the same Hit: every triangle at the
+   final t is tested in both walks, in the same depth-first order, and the
+   last of them wins.  query ANY stops at the first hit.
+   cnt: [0] internal-node fetches, [1] triangle tests. */
+void orc_closest_hit_stack(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root, orc_v3 ro,
+                           orc_v3 rd, orc_v3 inv, int query, uint64_t* cnt, int cap)
+{
+    uint32_t stack[64];
+    float entry[64];
+    int sp = 0;
+    uint32_t node = root;
+    if (cap > 64) cap = 64;
+    for (;;) {
+        int pop = 0;
+        if (node & ORC_LEAF_BIT) {
+            float t, u, v;
+            orc_v3 ng;
+            const orc_tri* tri = &tris[node & ~ORC_LEAF_BIT];
+            cnt[1]++;
+            if (orc_intersect_tri(&t, &u, &v, &ng, 0.0f, hit->t, ro, rd, tri->vs[0], tri->vs[1], tri->vs[2])) {
+                hit->t = t;
+                hit->u = u;
+                hit->v = v;
+                hit->triangleIndex = node & ~ORC_LEAF_BIT;
+                hit->ngx = ng.x;
+                hit->ngy = ng.y;
+                hit->ngz = ng.z;
+                if (query == ORC_QUERY_ANY) return;
+            }
+            pop = 1;
+        } else {
+            const orc_node* nd = &nodes[node];
+            cnt[0]++;
+            float lmin, lmax, rmin, rmax;
+            orc_slabs(ro, inv, nd->aabbs[0].lower, nd->aabbs[0].upper, hit->t, &lmin, &lmax);
+            orc_slabs(ro, inv, nd->aabbs[1].lower, nd->aabbs[1].upper, hit->t, &rmin, &rmax);
+            int hitL = lmin <= lmax;
+            int hitR = rmin <= rmax;
+            if (hitL && hitR) {
+                uint32_t nearN = nd->children[0], farN = nd->children[1];
+                float farMin = rmin;
+                if (rmin < lmin) {
+                    nearN = nd->children[1];
+                    farN = nd->children[0];
+                    farMin = lmin;
+                }
+                if (sp == cap) {
+                    orc_intersect_stackfree_ex(hit, nodes, tris, root, ro, rd, inv, query, cnt, 1);
+                    return;
+                }
+                stack[sp] = farN;
+                entry[sp] = farMin;
+                sp++;
+                node = nearN;
+            } else if (hitL || hitR) {
+                node = nd->children[hitL ? 0 : 1];
+            } else {
+                pop = 1;
+            }
+        }
+        if (pop) {
+            for (;;) {
+                if (sp == 0) return;
+                sp--;
+                if (entry[sp] <= hit->t) break;
+            }
+            node = stack[sp];
+        }
+    }
+}
+
+/* orc_closest_hit_stack over a batch of rays (8 floats each, tmax in slot 3). */
+void orc_closest_hit_stack_batch(const orc_node* nodes, const orc_tri* tris, uint32_t root, const float* rays,
+                                 int nRays, int query, int cap, orc_hit* hits, uint64_t* counters, int nthreads)
+{
+    int nt = pick_threads(nthreads);
+    uint64_t c0 = 0, c1 = 0;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt) reduction(+ : c0, c1)
+    for (int i = 0; i < nRays; i++) {
+        const float* r = rays + 8 * (size_t)i;
+        orc_v3 ro = V(r[0], r[1], r[2]), rd = V(r[4], r[5], r[6]);
+        uint64_t cnt[2] = {0, 0};
+        orc_hit h;
+        hit_reset(&h, r[3]);
+        orc_closest_hit_stack(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt, cap);
+        hits[i] = h;
+        c0 += cnt[0];
+        c1 += cnt[1];
+    }
+    if (counters) {
+        counters[0] += c0;
+        counters[1] += c1;
+    }
+}
+
 /* orc_any_hit_stack over a batch of rays (8 floats each, tmax in slot 3). */
 void orc_any_hit_stack_batch(const orc_node* nodes, const orc_tri* tris, uint32_t root, const float* rays, int nRays,
                              int cap, uint8_t* occluded, uint64_t* counters, int nthreads)
@@ -1013,7 +1122,8 @@ void orc_any_hit_stack_batch(const orc_node* nodes, const orc_tri* tris, uint32_
 
 /* walk: 0 = the reference stack-free walk for every ray; 1 = stack-free with
    the early exit; 2 = stack-free with early exit for the primary ray and
-   orc_any_hit_stack (stack of ORC_AO_STACK entries) for the AO rays. */
+   orc_any_hit_stack (stack of ORC_AO_STACK entries) for the AO rays; 3 = as
+   2 with orc_closest_hit_stack for the primary ray. */
 void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_raygen* g, uint32_t root,
                    const orc_node* nodes, const orc_tri* tris, int useSobol, int nSample, float radius, uint32_t seed,
                    int row0, int row1, uint64_t* counters, int nthreads, int walk) /* main_gpu.cu:111-177 */
@@ -1032,8 +1142,12 @@ void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_rayge
             orc_raygen_shoot(g, &ro, &rd, (float)xi / (float)w, (float)yi / (float)h);
             orc_hit hit;
             hit_reset(&hit, FLT_MAX_H);
-            orc_intersect_stackfree_ex(&hit, nodes, tris, root, ro, rd, orc_inv_rd(rd), ORC_QUERY_CLOSEST, cnt,
-                                       earlyExit);
+            if (walk == 3)
+                orc_closest_hit_stack(&hit, nodes, tris, root, ro, rd, orc_inv_rd(rd), ORC_QUERY_CLOSEST, cnt,
+                                      ORC_AO_STACK);
+            else
+                orc_intersect_stackfree_ex(&hit, nodes, tris, root, ro, rd, orc_inv_rd(rd), ORC_QUERY_CLOSEST, cnt,
+                                           earlyExit);
             cRays++;
             if (hit.t == FLT_MAX_H) {
                 pixels[pixel] = orc_pack_rgba(0, 0, 0, 1);
@@ -1061,7 +1175,7 @@ void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_rayge
                 orc_v3 dl = hemisphere_cos(r0, r1);
                 orc_v3 dir = vadd(vadd(vmuls(xaxis, dl.x), vmuls(zaxis, dl.z)), vmuls(n, dl.y));
                 int occluded;
-                if (walk == 2) {
+                if (walk >= 2) {
                     occluded = orc_any_hit_stack(nodes, tris, root, ao_origin, dir, orc_inv_rd(dir), radius, cnt,
                                                  ORC_AO_STACK);
                 } else {

@@ -78,6 +78,7 @@ def lib():
                                c_int, P, c_int, c_int], None),
             "orc_render_pt": ([P, c_int, c_int, P, c_uint32, P, P, P, c_int, c_int, c_int, c_int], None),
             "orc_any_hit_stack_batch": ([P, P, c_uint32, P, c_int, c_int, P, P, c_int], None),
+            "orc_closest_hit_stack_batch": ([P, P, c_uint32, P, c_int, c_int, c_int, P, P, c_int], None),
             "orc_pack": ([P, P, c_int], None),
             "orc_primary_eval64": ([P, P, P, c_int, c_int, P, P, P], None),
             "orc_primary_grad64": ([P, P, P, c_int, c_int, P, P, P, P], None),
@@ -222,18 +223,30 @@ def render_normal(g, w, h, root, nodes, tris, nthreads=0):
 
 
 def render_ao(g, w, h, root, nodes, tris, use_sobol=1, n_sample=16, radius=1.0, seed=12345, row0=0, row1=None,
-              nthreads=0, early_exit=False, ao_stack=False):
+              nthreads=0, early_exit=False, ao_stack=False, primary_stack=False):
     """walk: the reference stack-free walk; early_exit: with the pending-count
-    exit; ao_stack: early-exit primary ray + stack any-hit AO rays (the GPU
-    kernel's walks, so its counters match)."""
+    exit; ao_stack: early-exit primary ray + stack any-hit AO rays;
+    primary_stack: stack closest-hit primary ray + stack any-hit AO rays (the
+    GPU kernel's walks, so its counters match)."""
     px = np.zeros(w * h, np.uint32)
     ao = np.full(w * h, np.nan, np.float32)
     cnt = np.zeros(3, np.uint64)
-    walk = 2 if ao_stack else (1 if early_exit else 0)
+    walk = 3 if primary_stack else (2 if ao_stack else (1 if early_exit else 0))
     lib().orc_render_ao(_p(px), _p(ao), w, h, ctypes.byref(g), root, _p(np.ascontiguousarray(nodes, np.uint32)),
                         _p(_f32(tris)), use_sobol, n_sample, radius, seed & 0xFFFFFFFF, row0,
                         h if row1 is None else row1, _p(cnt), nthreads, walk)
     return px, ao, cnt
+
+
+def closest_hit_stack(nodes, tris, root, rays, query=0, cap=16, nthreads=0):
+    """Hit records and (node fetches, triangle tests) of the stack walk with
+    pop re-test, stack of `cap` entries (overflow restarts stack-free)."""
+    rays = _f32(rays)
+    hits = np.zeros(rays.shape[0], HIT_DTYPE)
+    cnt = np.zeros(2, np.uint64)
+    lib().orc_closest_hit_stack_batch(_p(np.ascontiguousarray(nodes, np.uint32)), _p(_f32(tris)), root, _p(rays),
+                                      rays.shape[0], query, cap, _p(hits), _p(cnt), nthreads)
+    return hits, cnt
 
 
 def any_hit_stack(nodes, tris, root, rays, cap=16, nthreads=0):

@@ -85,7 +85,7 @@ def test_indexed_trace_matches_soup_trace(gpu, orc, query):
     torch.cuda.synchronize()
     nodes, root, _, _ = orc.build(tris)
     oh, _ = orc.trace(nodes, tris, root, rays, query=query)
-    _, oc = orc.trace(nodes, tris, root, rays, query=query, method=3)
+    _, oc = orc.trace(nodes, tris, root, rays, query=query, method=4)
     got = gh.cpu().numpy().view(np.uint32).reshape(-1, 8)
     assert np.array_equal(got, oh.view(np.uint32).reshape(-1, 8))
     assert np.array_equal(cnt.cpu().numpy().view(np.uint64), oc)

@@ -1,10 +1,11 @@
-"""GPU stack-free traversal and render kernels vs the oracle.
+"""GPU traversal and render kernels vs the oracle.
 
-Bar: Hit records (t, uv, ng, triangleIndex) bit-identical to the oracle;
-node-fetch / triangle-test counters identical to the oracle's walks of the
-same kind (early-exit stack-free for closest/any-hit rays, the stack any-hit
-walk for AO rays).  Normal images bit-identical.  AO: primary hit/miss identical on every
-pixel; the occlusion fraction may differ only where an AO direction, which
+Bar: Hit records (t, uv, ng, triangleIndex) bit-identical to the oracle's
+reference stack-free walk; node-fetch / triangle-test counters identical to
+the oracle's walk of the same kind (the stack walk with pop re-test, the
+early-exit stack-free walk, the escape-link walk, and for the AO kernel the
+stack walks of its primary and AO rays).  Normal images bit-identical.  AO:
+primary hit/miss identical on every pixel; the occlusion fraction may differ only where an AO direction, which
 goes through the device cosf/sinf instead of the host libm ones, grazes an
 edge -- at most 0.5% of hit pixels and by at most 1/N each.  PT: mean radiance
 within 2% and 98% of pixels within 5% (paths fork where a bounce direction
@@ -38,27 +39,32 @@ def upload_bvh(nodes, root):
             torch.from_numpy(np.array([root], np.uint32).view(np.int32)).cuda())
 
 
-def gpu_trace(gpu, nodes_t, root_t, tris_t, rays, query):
+def gpu_trace(gpu, nodes_t, root_t, tris_t, rays, query, walk="stack"):
     r = torch.from_numpy(rays).cuda()
     cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
-    hits = gpu.trace(nodes_t, tris_t, r, query=query, root=root_t, counters=cnt)
+    hits = gpu.trace(nodes_t, tris_t, r, query=query, root=root_t, counters=cnt, walk=walk)
     torch.cuda.synchronize()
     return hits.cpu().numpy().view(np.uint32).reshape(-1, 8), cnt.cpu().numpy().view(np.uint64)
 
 
+@pytest.mark.parametrize("walk", ["stack", "stackfree"])
 @pytest.mark.parametrize("query", [0, 1])
 @pytest.mark.parametrize("n", [2, 1000, 65536])
-def test_trace_bit_exact_on_oracle_bvh(gpu, orc, n, query):
-    print(f"trace n={n} query={query}", flush=True)
+def test_trace_bit_exact_on_oracle_bvh(gpu, orc, n, query, walk):
+    """Hit records bit-identical to the reference walk (oracle method 0),
+    with equal-t ties from duplicated triangles; visit counters equal to the
+    oracle's walk of the same kind: the stack walk with pop re-test (method
+    4) or the stack-free walk with the pending-count early exit (method 3)."""
+    print(f"trace n={n} query={query} walk={walk}", flush=True)
     tris = scenes.random_triangles(n, 7 + n, sigma=0.02)
+    if n > 4:
+        tris[-(n // 5):] = tris[: n // 5]  # exact duplicates: equal-t ties
     nodes, root, _, _ = orc.build(tris)
     rays = random_rays(50_000, n, tmax=None if query == 0 else 0.25)
     oh, _ = orc.trace(nodes, tris, root, rays, query=query)
-    # the kernels stop the walk once nothing is owed (oracle trace method 3):
-    # same hits as the reference walk, fewer fetches, which must match exactly
-    _, oc = orc.trace(nodes, tris, root, rays, query=query, method=3)
+    _, oc = orc.trace(nodes, tris, root, rays, query=query, method=4 if walk == "stack" else 3)
     nt, rt = upload_bvh(nodes, root)
-    gh, gc = gpu_trace(gpu, nt, rt, torch.from_numpy(tris).cuda(), rays, query)
+    gh, gc = gpu_trace(gpu, nt, rt, torch.from_numpy(tris).cuda(), rays, query, walk)
     assert np.array_equal(gh, oh.view(np.uint32).reshape(-1, 8))
     assert np.array_equal(gc, oc)
 
@@ -74,7 +80,7 @@ def test_trace_on_gpu_built_bvh(gpu, orc):
     torch.cuda.synchronize()
     nodes, root, _, _ = orc.build(tris)
     oh, _ = orc.trace(nodes, tris, root, rays)
-    _, oc = orc.trace(nodes, tris, root, rays, method=3)
+    _, oc = orc.trace(nodes, tris, root, rays, method=4)
     assert np.array_equal(gh.cpu().numpy().view(np.uint32).reshape(-1, 8), oh.view(np.uint32).reshape(-1, 8))
     assert np.array_equal(cnt.cpu().numpy().view(np.uint64), oc)
 
@@ -170,8 +176,9 @@ def test_render_ao_matches_oracle(gpu, orc, w, h, nsample, radius, seed):
     assert dpx.max() <= 1
     gc = cnt.cpu().numpy().view(np.uint64)
     assert gc[0] == rcnt[0]  # rays traced: 1 + N per hit pixel
-    # the kernel's walks: early-exit stack-free primary ray, stack any-hit AO rays
-    _, eao, ecnt = orc.render_ao(og, w, h, root, nodes, tris, 1, nsample, radius, seed, ao_stack=True)
+    # the kernel's walks: stack closest-hit primary ray (pop re-test), stack
+    # any-hit AO rays
+    _, eao, ecnt = orc.render_ao(og, w, h, root, nodes, tris, 1, nsample, radius, seed, primary_stack=True)
     assert np.array_equal(eao, rao)
     if (diff == 0).all():
         assert np.array_equal(gc, ecnt)

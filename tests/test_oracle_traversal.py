@@ -186,3 +186,58 @@ def test_escape_link_bounds_and_occlusion(orc, query, tmax):
     ref, _ = orc.trace(nodes, tris, root, rays, query=0, method=0)
     assert np.array_equal(h["tri"] != 0xFFFFFFFF, ref["tri"] != 0xFFFFFFFF)
     assert (h["tri"] != 0xFFFFFFFF).sum() > 50
+
+
+def tie_scene(n, seed):
+    """Random triangles plus exact copies and coplanar neighbours, so that
+    many rays meet several triangles at the same t."""
+    tris = scenes.random_triangles(n, seed, sigma=0.03)
+    k = n // 4
+    tris[n - k:] = tris[:k]  # exact duplicates (equal t, different index)
+    return tris
+
+
+@pytest.mark.parametrize("cap", [16, 4, 1])
+@pytest.mark.parametrize("query,tmax", [(0, None), (1, 0.3)])
+def test_closest_hit_stack_equals_stackfree(orc, cap, query, tmax):
+    """The stack walk with pop re-test returns the reference walk's Hit records
+    bit for bit, equal-t ties included; caps 1 and 4 force the stack-free
+    restart."""
+    for n, seed in [(2, 1), (600, 2), (30000, 3)]:
+        tris = tie_scene(n, seed) if n > 4 else scenes.random_triangles(n, seed)
+        nodes, root, _, _ = orc.build(tris)
+        rays = random_rays(20000, seed + 40, tmax=tmax)
+        ref, cref = orc.trace(nodes, tris, root, rays, query=query, method=0)
+        got, cnt = orc.closest_hit_stack(nodes, tris, root, rays, query=query, cap=cap)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (n, cap)
+    if cap == 16 and query == 0:
+        assert cnt[0] < cref[0] // 2  # each internal node fetched once per visit
+    # the duplicates do tie: some hit records name the second copy
+    dup = (ref["tri"] >= 30000 - 7500) & (ref["tri"] != 0xFFFFFFFF)
+    assert dup.sum() > 0 or query == 1
+
+
+def test_closest_hit_stack_trace_method(orc):
+    """orc_trace method 4 is the stack walk with 16 entries."""
+    tris = scenes.random_triangles(5000, 11)
+    nodes, root, _, _ = orc.build(tris)
+    rays = random_rays(5000, 12)
+    a, ca = orc.trace(nodes, tris, root, rays, method=4)
+    b, cb = orc.closest_hit_stack(nodes, tris, root, rays, cap=16)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32)) and np.array_equal(ca, cb)
+
+
+def test_ao_primary_stack_walk(orc):
+    """AO with the primary ray on the stack walk: pixels and occlusion equal
+    to the reference walks, fewer node fetches."""
+    tris = scenes.random_triangles(30000, 9)
+    nodes, root, _, _ = orc.build(tris)
+    eye, ctr, up, fov = scenes.CAMERA_SMALL
+    import math
+    g = orc.raygen_lookat(eye, ctr, up, math.radians(fov), 96, 80)
+    px0, ao0, c0 = orc.render_ao(g, 96, 80, root, nodes, tris, 1, 8, 0.5, 3)
+    px3, ao3, c3 = orc.render_ao(g, 96, 80, root, nodes, tris, 1, 8, 0.5, 3, primary_stack=True)
+    _, _, c2 = orc.render_ao(g, 96, 80, root, nodes, tris, 1, 8, 0.5, 3, ao_stack=True)
+    assert np.array_equal(px0, px3) and np.array_equal(np.nan_to_num(ao0), np.nan_to_num(ao3))
+    assert c3[0] == c0[0] and c3[2] == c2[2]
+    assert c3[1] < c2[1]
