@@ -15,12 +15,14 @@ rays traced per second over the K timed steps.  The same run also times:
   * on the same mesh, 4M closest-hit rays through the stack-free, the
     escape-link and the indexed-mesh traversal, and the build from the
     index buffer and with 63-bit keys;
+  * the path-tracing scene (64 displaced level-7 spheres, 20,971,524
+    triangles): GPU build and 1080p frames of 8 spp, max depth 4;
   * 200 Adam steps of the differentiable primary render (level-7 sphere,
     512x512) against its forward-only time;
   * the CPU oracle (full build + the whole AO frame) on this host's cores.
 Workload set-up lives in minimumlbvh_amd/workloads.py.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-diff]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-pt] [--no-diff]
 """
 import argparse
 import json
@@ -80,6 +82,7 @@ def main():
     ap.add_argument("--sort-log2", type=int, default=25, help="standalone sort size = 2^k u64 keys")
     ap.add_argument("--no-batched", action="store_true", help="skip the 4096-mesh batched leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render leg")
+    ap.add_argument("--no-pt", action="store_true", help="skip the 21M-triangle path-tracing leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this path has no multi-GPU "
@@ -157,6 +160,11 @@ def main():
     if args.level == 9:
         log("[bench] escape-link / indexed traversal, indexed and 63-bit builds")
         workloads.run_walks(max(args.steps, 3), walks, sc)
+    ptres = {}
+    if not args.no_pt:
+        log("[bench] path tracing: 64 spheres, 21M triangles, 1080p 8 spp")
+        workloads.run_pt(3, ptres)
+        torch.cuda.empty_cache()
     diffres = {}
     if not args.no_diff:
         log("[bench] differentiable render: 200 Adam steps")
@@ -234,6 +242,7 @@ def main():
         },
         "batched": batched or None,
         "walks": walks or None,
+        "pt": ptres or None,
         "diff": diffres or None,
         "tuning": autotune.get_tuning(),
     }

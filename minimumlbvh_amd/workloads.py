@@ -7,6 +7,7 @@ profiler can wrap exactly one of them:
     python -m minimumlbvh_amd.workloads batched  [--reps K]   # 4096 meshes, one build + trace
     python -m minimumlbvh_amd.workloads diff     [--reps K]   # level-7 sphere, 512^2, fwd+bwd steps
     python -m minimumlbvh_amd.workloads walks    [--reps K]   # escape-link / indexed trace, indexed / 63-bit build
+    python -m minimumlbvh_amd.workloads pt       [--reps K]   # 21M-tri sphere field: build + 1080p 8 spp PT
     python -m minimumlbvh_amd.workloads copy     [--reps K]   # 1 GiB device copy (counter calibration)
 
 Scenes:
@@ -267,6 +268,38 @@ def run_walks(reps: int, out: dict, sc: "AOScene | None" = None):
     return sc
 
 
+def run_pt(reps: int, out: dict, spp: int = 8, max_depth: int = 4):
+    """The path-tracing scene: 64 displaced level-7 spheres + ground + light
+    (20,971,524 triangles) built on the GPU, then `spp` samples per pixel at
+    1920x1080, max depth 4, Sobol streams; time per build and per frame of
+    `spp` samples."""
+    tris_np, attribs_np = scenes.pt_scene()
+    dev = "cuda:0"
+    tris = torch.from_numpy(tris_np).to(dev)
+    attribs = torch.from_numpy(attribs_np).to(dev)
+    n = tris_np.shape[0]
+    del tris_np, attribs_np
+    sorter, builder = lbvh.OnesweepSort(), lbvh.BVHGPUBuilder()
+    build = lambda: builder.build(tris, n, 0, sorter)  # noqa: E731
+    build()
+    build_ms = events_ms(build, reps)
+    eye, ctr, up, fov = scenes.CAMERA_PT
+    w, h = AO_W, AO_H
+    cam = lbvh.RayGenerator().lookat(eye, ctr, up, math.radians(fov), w, h)
+    acc = torch.zeros((w * h, 4), dtype=torch.float32, device=dev)
+
+    def frame():
+        acc.zero_()
+        for _ in range(spp):
+            lbvh.render_pt(acc, w, h, cam, builder.m_rootNode, builder.m_internals, tris, attribs, 1, 1024, max_depth)
+
+    frame_ms = events_ms(frame, reps)
+    out.update(triangles=n, width=w, height=h, spp=spp, max_depth=max_depth, build_ms=round(build_ms, 4),
+               build_mtris_per_s=round(n / build_ms / 1e3, 1), frame_ms=round(frame_ms, 3),
+               mpaths_per_s=round(w * h * spp / frame_ms / 1e3, 1),
+               mean_radiance=float(acc[:, :3].mean()))
+
+
 def run_diff(steps: int, out: dict):
     """C4: Adam steps (fwd + bwd + update) against forward-only time."""
     sc = DiffScene()
@@ -290,7 +323,7 @@ def run_diff(steps: int, out: dict):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("workload", choices=["ao", "build", "sort", "batched", "diff", "walks", "copy"])
+    ap.add_argument("workload", choices=["ao", "build", "sort", "batched", "diff", "walks", "pt", "copy"])
     ap.add_argument("--reps", type=int, default=3)
     args = ap.parse_args(argv)
     res = {"workload": args.workload}
@@ -318,6 +351,8 @@ def main(argv=None):
         run_diff(max(args.reps, 1), res)
     elif args.workload == "walks":
         run_walks(args.reps, res)
+    elif args.workload == "pt":
+        run_pt(args.reps, res)
     else:
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
         b = torch.empty_like(a)

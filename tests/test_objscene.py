@@ -1,0 +1,137 @@
+"""OBJ + MTL scene input (minimumlbvh_amd.objscene): the Triangle /
+TriangleAttrib arrays main_gpu.cpp:209-282 builds from tinyobjloader's
+output, checked on a hand-written file with known answers, on the generated
+Cornell box, and on the reference's own Cornell box asset when it is present
+(this container only)."""
+import os
+
+import numpy as np
+import pytest
+
+from minimumlbvh_amd import objscene
+
+REF_ASSET = "/root/reference/bin/assets/cornelbox.obj"
+
+SMALL_OBJ = """# known-answer scene
+mtllib small.mtl
+v 0 0 0
+v 1 0 0
+v 0 1 0
+v 0 0 1
+vt 0.5 0.5
+vn 0 0 1
+vn 0 1 0
+f 1 2 3
+usemtl lamp
+f 1//1 2//1 3//2
+f -4/1/1 -3/1/1 -2/1/2
+usemtl nothing
+f 2 3 4
+v 10 0 0
+v 11 0 0
+v 11 3 0
+v 10 3 0
+usemtl wall
+f 5 6 7 8
+v 20 0 0
+v 23 0 0
+v 23 1 0
+v 20 1 0
+f 9 10 11 12
+v 30 0 0
+v 31 0 0
+v 32 1 0
+v 31 2 0
+v 30 1 0
+f 13 14 15 16 17
+"""
+
+SMALL_MTL = """newmtl wall
+Kd 0.25 0.5 0.75
+newmtl lamp
+Kd 0.8 0.8 0.8
+Ke 30 24.5 17.5
+"""
+
+
+def eq32(got, want):
+    np.testing.assert_array_equal(got, np.asarray(want, np.float32))
+
+
+def test_known_answer_scene(tmp_path):
+    (tmp_path / "small.obj").write_text(SMALL_OBJ)
+    (tmp_path / "small.mtl").write_text(SMALL_MTL)
+    sc = objscene.load_obj(str(tmp_path / "small.obj"))
+    assert [m.name for m in sc.materials] == ["wall", "lamp"]
+    assert sc.faces == 7
+    # 4 triangles, 2 + 2 from the quads, 3 from the pentagon
+    assert sc.triangles.shape == (11, 3, 3) and sc.attribs.shape == (11, 15)
+    t, a = sc.triangles, sc.attribs
+    grey = [0.5, 0.5, 0.5, 0, 0, 0]
+    lamp = [0.8, 0.8, 0.8, 30, 24.5, 17.5]
+    wall = [0.25, 0.5, 0.75, 0, 0, 0]
+    assert sc.material_ids.tolist() == [-1, 1, 1, -1, 0, 0, 0, 0, 0, 0, 0]
+    eq32(t[0], [[0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    eq32(a[0], [0] * 9 + grey)
+    eq32(a[1], [0, 0, 1, 0, 0, 1, 0, 1, 0] + lamp)
+    # negative indices count back from the vertices read so far
+    eq32(t[2], [[0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    eq32(a[2][:9], [0, 0, 1, 0, 0, 1, 0, 1, 0])
+    eq32(a[3], [0] * 9 + grey)  # unknown usemtl name
+    # 1 x 3 quad: |v2-v0|^2 = 10 == |v3-v1|^2 = 10, not smaller -> (0,1,3),(1,2,3)
+    eq32(t[4], [[10, 0, 0], [11, 0, 0], [10, 3, 0]])
+    eq32(t[5], [[11, 0, 0], [11, 3, 0], [10, 3, 0]])
+    eq32(a[4], [0] * 9 + wall)
+    # 3 x 1 quad: same diagonals again -> (0,1,3),(1,2,3)
+    eq32(t[6], [[20, 0, 0], [23, 0, 0], [20, 1, 0]])
+    # pentagon: fan from the first vertex
+    eq32(t[8], [[30, 0, 0], [31, 0, 0], [32, 1, 0]])
+    eq32(t[9], [[30, 0, 0], [32, 1, 0], [31, 2, 0]])
+    eq32(t[10], [[30, 0, 0], [31, 2, 0], [30, 1, 0]])
+
+
+def test_quad_split_takes_the_shorter_diagonal(tmp_path):
+    obj = "v 0 0 0\nv 4 0 0\nv 5 1 0\nv 1 1 0\nf 1 2 3 4\n"
+    (tmp_path / "q.obj").write_text(obj)
+    sc = objscene.load_obj(str(tmp_path / "q.obj"))
+    # |v2-v0|^2 = 26 > |v3-v1|^2 = 10 -> split along 1-3: (0,1,3),(1,2,3)
+    np.testing.assert_array_equal(sc.triangles[0], [[0, 0, 0], [4, 0, 0], [1, 1, 0]])
+    np.testing.assert_array_equal(sc.triangles[1], [[4, 0, 0], [5, 1, 0], [1, 1, 0]])
+    obj2 = "v 0 0 0\nv 1 0 0\nv 2 1 0\nv -2 1 0\nf 1 2 3 4\n"
+    (tmp_path / "r.obj").write_text(obj2)
+    sc2 = objscene.load_obj(str(tmp_path / "r.obj"))
+    # |v2-v0|^2 = 5 < |v3-v1|^2 = 10 -> (0,1,2),(0,2,3)
+    np.testing.assert_array_equal(sc2.triangles[0], [[0, 0, 0], [1, 0, 0], [2, 1, 0]])
+    np.testing.assert_array_equal(sc2.triangles[1], [[0, 0, 0], [2, 1, 0], [-2, 1, 0]])
+
+
+def test_generated_cornell_box(tmp_path, orc):
+    sc = objscene.load_obj(objscene.cornell_box_obj(str(tmp_path)))
+    assert sc.faces == 16 and sc.triangles.shape[0] == 32
+    emissive = sc.attribs[:, 12:15].max(axis=1) > 0
+    assert emissive.sum() == 2
+    # shading normals are unit and face the room's inside for the walls
+    n = sc.attribs[:, 0:3]
+    np.testing.assert_allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-6)
+    nodes, root, _, d = orc.build(sc.triangles)
+    assert orc.validate(root, nodes, d)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_ASSET), reason="reference asset not present")
+def test_reference_cornell_box_asset(orc):
+    """The reference's bin/assets/cornelbox.obj: 8,858 quads (17,716
+    triangles), 9 materials in .mtl order, the ceiling light's Ke on its two
+    triangles, 8,842 quads of the head scan on defaultMat."""
+    sc = objscene.load_obj(REF_ASSET)
+    assert sc.faces == 8858 and sc.triangles.shape[0] == 17716
+    assert [m.name for m in sc.materials] == ["backWall", "ceiling", "defaultMat", "floor", "leftWall", "light",
+                                              "rightWall", "shortBox", "tallBox"]
+    light = sc.materials.index(next(m for m in sc.materials if m.name == "light"))
+    lt = sc.material_ids == light
+    assert lt.sum() == 2
+    np.testing.assert_array_equal(sc.attribs[lt, 12:15], np.float32([[30.0, 24.665525, 17.499191]] * 2))
+    head = sc.material_ids == sc.materials.index(next(m for m in sc.materials if m.name == "defaultMat"))
+    assert head.sum() == 2 * 8842
+    assert np.isfinite(sc.triangles).all() and np.isfinite(sc.attribs).all()
+    nodes, root, _, d = orc.build(sc.triangles)
+    assert orc.validate(root, nodes, d)
