@@ -20,14 +20,40 @@
 namespace mlbvh {
 
 constexpr int kBuildBlock = 256;
+// The scene box is reduced into kBoxCopies copies of its 6 order-mapped
+// words (block b uses copy b % kBoxCopies), so that the blocks' atomics
+// spread over many L2 lines; the Morton kernels fold the copies.
+#ifndef MLBVH_BOX_COPIES
+#define MLBVH_BOX_COPIES 32
+#endif
+constexpr int kBoxCopies = MLBVH_BOX_COPIES;
+constexpr int kBoxWords = 6 * kBoxCopies;
+static_assert(kBoxWords <= 256, "k_build_init clears the box copies with one block");
+
+// The scene box from the kBoxCopies partial boxes (all threads of the block
+// call it; the result is the same in every thread).
+__device__ inline void fold_scene_box(const uint32_t* sceneBox, f3* lo, f3* hi)
+{
+    __shared__ uint32_t s_box[6];
+    if (threadIdx.x < 6) {
+        uint32_t r = sceneBox[threadIdx.x];
+        for (int c = 1; c < kBoxCopies; c++) {
+            uint32_t v = sceneBox[6 * c + threadIdx.x];
+            r = threadIdx.x < 3 ? (v < r ? v : r) : (v > r ? v : r);
+        }
+        s_box[threadIdx.x] = r;
+    }
+    __syncthreads();
+    *lo = mk(ord2f(s_box[0]), ord2f(s_box[1]), ord2f(s_box[2]));
+    *hi = mk(ord2f(s_box[3]), ord2f(s_box[4]), ord2f(s_box[5]));
+}
 
 // Scene-box slots plus the sort state of the 4-pass record sort.
 __global__ void k_build_init(uint32_t* sceneBox, uint32_t* ctrl, uint32_t* hist, uint64_t* status,
                              size_t statusWords)
 {
     int t = threadIdx.x;
-    if (t < 3) sceneBox[t] = 0xFFFFFFFFu;  // atomicMin slots (order-mapped)
-    else if (t < 6) sceneBox[t] = 0u;      // atomicMax slots
+    if (t < kBoxWords) sceneBox[t] = t % 6 < 3 ? 0xFFFFFFFFu : 0u;  // atomicMin / atomicMax slots (order-mapped)
     sort_state_init(ctrl, 4, hist, 4, status, statusWords);
 }
 
@@ -64,8 +90,9 @@ __global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(TS tris, int n, uint
         int c = threadIdx.x;
         float r = s_v[0][c];
         for (int w = 1; w < kBuildBlock / 64; w++) r = c < 3 ? fminf(r, s_v[w][c]) : fmaxf(r, s_v[w][c]);
-        if (c < 3) atomicMin(&sceneBox[c], f2ord(r));
-        else atomicMax(&sceneBox[c], f2ord(r));
+        uint32_t* box = sceneBox + 6 * (blockIdx.x % kBoxCopies);
+        if (c < 3) atomicMin(&box[c], f2ord(r));
+        else atomicMax(&box[c], f2ord(r));
     }
 }
 
@@ -82,9 +109,8 @@ __global__ void __launch_bounds__(kBuildBlock)
 {
     __shared__ uint32_t s_h[4][256];
     for (int p = 0; p < 4; p++) s_h[p][threadIdx.x] = 0;
-    __syncthreads();
-    const f3 lo = mk(ord2f(sceneBox[0]), ord2f(sceneBox[1]), ord2f(sceneBox[2]));
-    const f3 hi = mk(ord2f(sceneBox[3]), ord2f(sceneBox[4]), ord2f(sceneBox[5]));
+    f3 lo, hi;
+    fold_scene_box(sceneBox, &lo, &hi);  // its barrier also orders the clears above
     for (int i = blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += gridDim.x * kBuildBlock) {
         Tri tri = tris[i];
         f3 center = ((tri.v[0] + tri.v[1]) + tri.v[2]) / 3.0f;
@@ -117,8 +143,8 @@ __global__ void __launch_bounds__(kBuildBlock)
     k_morton64(TS tris, int n, const uint32_t* __restrict__ sceneBox, uint64_t* __restrict__ keys,
                uint32_t* __restrict__ idx, uint32_t* __restrict__ flags, uint32_t* __restrict__ ctrl)
 {
-    const f3 lo = mk(ord2f(sceneBox[0]), ord2f(sceneBox[1]), ord2f(sceneBox[2]));
-    const f3 hi = mk(ord2f(sceneBox[3]), ord2f(sceneBox[4]), ord2f(sceneBox[5]));
+    f3 lo, hi;
+    fold_scene_box(sceneBox, &lo, &hi);
     for (int i = blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += gridDim.x * kBuildBlock) {
         Tri tri = tris[i];
         f3 center = ((tri.v[0] + tri.v[1]) + tri.v[2]) / 3.0f;
@@ -171,9 +197,11 @@ __global__ void __launch_bounds__(kBuildBlock)
 //
 // Phase 1 (k_climb_local): a block of kLocalLeaves threads climbs its leaves
 // through such block-local nodes, handing the sibling bounds over in LDS.  A
-// thread whose next parent is not block-local appends (node, lo, hi) to the
-// frontier list and stops.  Node fields are written with plain stores; only
-// the next kernel reads them.
+// thread whose next parent is not block-local keeps (node, lo, hi) for the
+// frontier list and stops.  The block-local nodes are assembled in LDS
+// (children, boxes, parent, context) and written at the end as whole 64-byte
+// nodes with coalesced 16-byte stores; a block-local node whose parent is not
+// block-local gets its parent word from phase 2.
 //
 // Phase 2 (k_climb_upper): the frontier entries (about 1% of the nodes for
 // surface meshes) climb on through the device-scope arrival words
@@ -186,9 +214,9 @@ constexpr int kLocalLeaves = MLBVH_CLIMB_LEAVES;
 
 template <class TS>
 __global__ void __launch_bounds__(kLocalLeaves)
-    k_climb_local(TS tris, uint32_t n, const uint64_t* __restrict__ rec,
-                  const uint8_t* __restrict__ deltas, Node* __restrict__ nodes, uint32_t* __restrict__ frontier,
-                  uint32_t* __restrict__ ctrl, uint32_t* __restrict__ root)
+    k_climb_local(TS tris, uint32_t n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
+                  Node* __restrict__ nodes, uint32_t* __restrict__ frontier, uint32_t* __restrict__ ctrl,
+                  uint32_t* __restrict__ root)
 {
     constexpr int kWaves = kLocalLeaves / 64;
     // Local delta j is d[b0 - 1 + j], j in [0, m] (m = leaves in the block).
@@ -196,10 +224,13 @@ __global__ void __launch_bounds__(kLocalLeaves)
     __shared__ uint8_t s_pm[kLocalLeaves];      // s_pm[j] = max s_d[0 .. j]
     __shared__ uint8_t s_sm[kLocalLeaves + 1];  // s_sm[j] = max s_d[j .. kLocalLeaves]
     __shared__ uint32_t s_wmax[2][kWaves];
-    __shared__ uint32_t s_flag[kLocalLeaves];
-    __shared__ Box s_box[kLocalLeaves][2];
-    __shared__ uint32_t s_out[kLocalLeaves * 3];
-    __shared__ uint32_t s_nOut, s_base;
+    __shared__ uint32_t s_flag[kLocalLeaves];   // arrival word of local node b0 + slot
+    __shared__ Box s_box[kLocalLeaves][2];      // the node's child boxes
+    __shared__ uint32_t s_child[kLocalLeaves][2];
+    __shared__ uint32_t s_parent[kLocalLeaves];
+    __shared__ uint32_t s_context[kLocalLeaves];
+    __shared__ uint32_t s_wcount[kWaves];
+    __shared__ uint32_t s_base;
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t b0 = blockIdx.x * kLocalLeaves;
@@ -218,7 +249,7 @@ __global__ void __launch_bounds__(kLocalLeaves)
     s_d[t] = (uint8_t)e0;
     if (t == kLocalLeaves - 1) s_d[kLocalLeaves] = (uint8_t)e1;
     s_flag[t] = 0xFFFFFFFFu;
-    if (t == 0) s_nOut = 0;
+    s_parent[t] = kInvalidNode;
     uint32_t pm = e0, sm = e1;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -237,12 +268,11 @@ __global__ void __launch_bounds__(kLocalLeaves)
     __syncthreads();
 
     const uint32_t pos = b0 + (uint32_t)t;
-    bool rootHere = false;
-    uint32_t node = 0;
+    bool rootHere = false, emit = false;
+    uint32_t node = 0, lo = pos, hi = pos;
     if ((uint32_t)t < m) {
         const uint32_t triIndex = (uint32_t)(rec[pos] >> 32);
         Box box = tri_box(tris[triIndex]);
-        uint32_t lo = pos, hi = pos;
         node = triIndex | kLeafBit;
         while (true) {
             if (hi - lo >= nInternals) {
@@ -260,38 +290,70 @@ __global__ void __launch_bounds__(kLocalLeaves)
                 local = (b0 == 0 || (uint32_t)s_pm[jp - 1] >= dp) && (b1 == nInternals || (uint32_t)s_sm[jp + 1] > dp);
             }
             if (!local) {
-                uint32_t k = atomicAdd(&s_nOut, 1u);
-                s_out[3 * k] = node;
-                s_out[3 * k + 1] = lo;
-                s_out[3 * k + 2] = hi;
+                emit = true;
                 break;
             }
-            Node* p = nodes + parent;
-            p->child[goLeft] = node;
-            p->box[goLeft] = box;
-            if (!(node & kLeafBit)) nodes[node].parent = parent;
             const uint32_t slot = parent - b0;
+            if (!(node & kLeafBit)) s_parent[node - b0] = parent;
+            s_child[slot][goLeft] = node;
             s_box[slot][goLeft] = box;
             uint32_t idx = __hip_atomic_exchange(&s_flag[slot], goLeft ? hi : lo, __ATOMIC_ACQ_REL,
                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
             if (idx == 0xFFFFFFFFu) break;
             lo = lo < idx ? lo : idx;
             hi = hi < idx ? idx : hi;
-            p->context = hi < nInternals ? hi : 0xFFFFFFFFu;
+            s_context[slot] = hi < nInternals ? hi : 0xFFFFFFFFu;
             node = parent;
             Box other = s_box[slot][goLeft ^ 1];
             box.lo = hmin3(box.lo, other.lo);
             box.hi = hmax3(box.hi, other.hi);
         }
     }
-    if (rootHere) {
-        *root = node;
-        nodes[node & ~kLeafBit].parent = kInvalidNode;
+    if (rootHere) *root = node;
+
+    // frontier entries: a block-wide compaction, one global atomic per block
+    const uint64_t emitMask = __ballot(emit);
+    if (lane == 0) s_wcount[wave] = (uint32_t)__popcll(emitMask);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < kWaves; w++) {
+            uint32_t c = s_wcount[w];
+            s_wcount[w] = total;
+            total += c;
+        }
+        s_base = total ? atomicAdd(&ctrl[kCtrlFrontier], total) : 0u;
     }
     __syncthreads();
-    if (t == 0 && s_nOut) s_base = atomicAdd(&ctrl[kCtrlFrontier], s_nOut);
-    __syncthreads();
-    for (uint32_t i = t; i < 3 * s_nOut; i += kLocalLeaves) frontier[3 * (size_t)s_base + i] = s_out[i];
+    if (emit) {
+        const uint32_t k = s_base + s_wcount[wave] + (uint32_t)__popcll(emitMask & ((1ull << lane) - 1ull));
+        frontier[3 * (size_t)k] = node;
+        frontier[3 * (size_t)k + 1] = lo;
+        frontier[3 * (size_t)k + 2] = hi;
+    }
+
+    // whole block-local nodes (both arrivals seen: the arrival word holds the
+    // second one's bound), four 16-byte stores each
+    const uint32_t nSlots = m - 1;  // internal nodes b0 .. b1-1
+    for (uint32_t q = (uint32_t)t; q < 4 * nSlots; q += kLocalLeaves) {
+        const uint32_t slot = q >> 2, part = q & 3;
+        if (s_flag[slot] == 0xFFFFFFFFu) continue;
+        const Box& l = s_box[slot][0];
+        const Box& r = s_box[slot][1];
+        uint4 v;
+        if (part == 0) {
+            v = make_uint4(s_child[slot][0], s_child[slot][1], __float_as_uint(l.lo.x), __float_as_uint(l.lo.y));
+        } else if (part == 1) {
+            v = make_uint4(__float_as_uint(l.lo.z), __float_as_uint(l.hi.x), __float_as_uint(l.hi.y),
+                           __float_as_uint(l.hi.z));
+        } else if (part == 2) {
+            v = make_uint4(__float_as_uint(r.lo.x), __float_as_uint(r.lo.y), __float_as_uint(r.lo.z),
+                           __float_as_uint(r.hi.x));
+        } else {
+            v = make_uint4(__float_as_uint(r.hi.y), __float_as_uint(r.hi.z), s_parent[slot], s_context[slot]);
+        }
+        reinterpret_cast<uint4*>(nodes + b0 + slot)[part] = v;
+    }
 }
 
 template <class TS>

@@ -79,7 +79,7 @@ extern "C" int mlbvh_ctx_create(int device, mlbvh_ctx** out)
         if (e != hipSuccess) rc = hip_status(e, "hipMemset(ctrl)");
     }
     cap = 0;
-    if (!rc) rc = ensure(&ctx->sceneBox, &cap, (size_t)8, "hipMalloc(sceneBox)");
+    if (!rc) rc = ensure(&ctx->sceneBox, &cap, (size_t)256, "hipMalloc(sceneBox)");
     if (rc) {
         mlbvh_ctx_destroy(ctx);
         return rc;

@@ -35,7 +35,7 @@ struct mlbvh_ctx {
     size_t flagsCap = 0;
     uint32_t* frontier = nullptr;  // (node, lo, hi) where the block-local climb stops, [n][3]
     size_t frontierCap = 0;
-    uint32_t* sceneBox = nullptr;  // 6 order-mapped uint32 (lo xyz, hi xyz)
+    uint32_t* sceneBox = nullptr;  // copies of 6 order-mapped uint32 (lo xyz, hi xyz), 256 words
 
     // --- batched build / trace ------------------------------------------------
     uint32_t* meshBox = nullptr;  // [meshes][6] order-mapped bounds
