@@ -721,7 +721,7 @@ void orc_trace(const orc_node* nodes, const orc_tri* tris, uint32_t root, const 
             else if (method == 1)
                 orc_intersect_stackfull(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), stack);
             else if (method == 4)
-                orc_closest_hit_stack(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt, ORC_AO_STACK);
+                orc_closest_hit_stack(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt, ORC_LANE_STACK);
             else
                 orc_intersect_escape_link(&h, nodes, tris, root, ro, rd, orc_inv_rd(rd), query, cnt);
             hits[i] = h;

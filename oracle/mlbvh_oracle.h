@@ -102,7 +102,7 @@ void orc_intersect_stackfull(orc_hit* hit, const orc_node* nodes, const orc_tri*
 void orc_intersect_escape_link(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root,
                                orc_v3 ro, orc_v3 rd, orc_v3 inv, int query, uint64_t* counters);
 /* rays: 8 floats each (ox,oy,oz,tmax,dx,dy,dz,pad); method 0 stackfree, 1 stackfull, 2 escape link,
-   3 stackfree with the pending-count early exit, 4 orc_closest_hit_stack (ORC_AO_STACK entries) */
+   3 stackfree with the pending-count early exit, 4 orc_closest_hit_stack (ORC_LANE_STACK entries) */
 void orc_trace(const orc_node* nodes, const orc_tri* tris, uint32_t root, const float* rays, int nRays,
                int query, int method, orc_hit* hits, uint64_t* counters, int nthreads);
 void orc_brute_force(const orc_tri* tris, int n, const float* rays, int nRays, orc_hit* hits);
@@ -121,7 +121,8 @@ void orc_render_normal(uint32_t* pixels, int w, int h, const orc_raygen* g, uint
    rows [row0,row1) only (for bounded CPU baselines); counters: rays traced, node fetches, tri tests */
 /* walk: 0 reference stack-free walk, 1 stack-free with early exit, 2 early-exit primary + stack any-hit AO rays,
    3 stack closest-hit primary + stack any-hit AO rays */
-#define ORC_AO_STACK 16
+#define ORC_AO_STACK 12   /* the AO kernel's per-lane stack */
+#define ORC_LANE_STACK 16 /* the trace and render kernels' per-lane stack */
 int orc_any_hit_stack(const orc_node* nodes, const orc_tri* tris, uint32_t root, orc_v3 ro, orc_v3 rd, orc_v3 inv,
                       float tmax, uint64_t* cnt, int cap);
 void orc_closest_hit_stack(orc_hit* hit, const orc_node* nodes, const orc_tri* tris, uint32_t root, orc_v3 ro,
