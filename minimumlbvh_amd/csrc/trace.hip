@@ -24,7 +24,13 @@ constexpr int kAoRefill = MLBVH_AO_REFILL;  // lanes waiting before a wave start
 // per block) let 6 blocks share a CU, against 5 with 16; the AO frame of the
 // level-9 sphere took 5.27 ms against 5.56 ms, the rarer stack-free restarts
 // costing 0.1 % more node fetches.
-constexpr uint32_t kAoStack = 12;
+#ifndef MLBVH_AO_STACK
+#define MLBVH_AO_STACK 12
+#endif
+#ifndef MLBVH_AO_WAVES
+#define MLBVH_AO_WAVES 6
+#endif
+constexpr uint32_t kAoStack = MLBVH_AO_STACK;
 
 // Walks of k_trace.  kWalkStack: traverse_stack, the hits of
 // intersect_stackfree with each internal node fetched once per visit (the
@@ -268,11 +274,13 @@ __global__ void __launch_bounds__(kTraceBlock)
 
 // 6 waves per SIMD (at most 80 VGPRs), matching the 6 blocks per CU that its
 // LDS allows.
-template <bool kCount>
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6)))
+// kSobol: AO directions from the Sobol stream (the reference's useSobol),
+// else from the per-pixel PCG stream.
+template <bool kCount, bool kSobol>
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(MLBVH_AO_WAVES)))
     k_render_ao(uint32_t* pixels, float* aoOut, int w, int h, mlbvh_raygen cam, const uint32_t* __restrict__ rootPtr,
-                const Node* __restrict__ nodes, const Tri* __restrict__ tris, int useSobol, int nSample, float radius,
-                uint32_t seed, unsigned long long* counters)
+                const Node* __restrict__ nodes, const Tri* __restrict__ tris, int nSample, float radius, uint32_t seed,
+                unsigned long long* counters)
 {
     int xi, yi;
     tile_pixel(&xi, &yi);
@@ -303,9 +311,12 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
     __shared__ uint32_t s_stack[kAoStack][kTraceBlock];
     __shared__ float s_entry[kAoStack][kTraceBlock];
     const int t = threadIdx.x;
-    f3 o = mk(0, 0, 0), d = mk(0, 0, 1), iv = mk(0, 0, 1);  // the lane's current ray
-    float tmax = kFltMax;                                    // primary: closest t so far; AO: radius
-    f3 hitNg = mk(0, 0, 0);
+    // The lane's current ray; after the primary ray, o is the AO origin.
+    f3 o = mk(0, 0, 0), d = mk(0, 0, 1), iv = mk(0, 0, 1);
+    float tmax = kFltMax;  // primary: closest t so far; AO: radius
+    // primary: the closest hit's geometric normal; then the shading frame's
+    // normal (xaxis, zaxis complete the frame)
+    f3 n = mk(0, 0, 0), xaxis = mk(1, 0, 0), zaxis = mk(0, 1, 0);
     bool primary = inImage;       // the primary ray is in flight
     bool live = inImage;          // a ray is in flight
     bool waiting = false;         // the next AO sample is due
@@ -317,8 +328,7 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
         iv = inv_rd(d);
         curr = root;
     }
-    f3 n = mk(0, 0, 1), xaxis = mk(1, 0, 0), zaxis = mk(0, 1, 0), org = mk(0, 0, 0);
-    Pcg rng(hash_pcg((uint32_t)pixel), 0);
+    Pcg rng(kSobol ? 0u : hash_pcg((uint32_t)pixel), 0);
     int sample = 0, occluded = 0;
     uint32_t guard = kMaxTraversalSteps;
     while (guard-- != 0) {
@@ -328,14 +338,13 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
         if (waitMask != 0 && (liveMask == 0 || kAoRefill <= __popcll(waitMask))) {
             if (waiting) {
                 float r0, r1;
-                if (useSobol) {
+                if (kSobol) {
                     sobol2d(&r0, &r1, (uint32_t)sample, (uint32_t)xi, (uint32_t)yi, seed);
                 } else {
                     r0 = rng.uniformf();
                     r1 = rng.uniformf();
                 }
                 f3 dl = hemisphere_cos(r0, r1);
-                o = org;
                 d = ((xaxis * dl.x) + (zaxis * dl.z)) + (n * dl.y);
                 iv = inv_rd(d);
                 tmax = radius;
@@ -361,7 +370,7 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
             } else {
                 if (hitTri) {  // primary: a closer hit (or an equal-t one visited later)
                     tmax = tt;
-                    hitNg = ng;
+                    n = ng;
                 }
                 if (stackFree) {
                     uint32_t tmp = curr;
@@ -421,10 +430,10 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
                 primary = false;
                 hasHit = tmax != kFltMax;
                 if (hasHit) {
-                    n = normalize(hitNg);
+                    n = normalize(n);
                     if (0.0f < dot(n, d)) n = -n;
                     onb(n, &xaxis, &zaxis);
-                    org = (o + d * tmax) + n * 0.001f;
+                    o = (o + d * tmax) + n * 0.001f;
                     waiting = 0 < nSample;
                 }
             } else {
@@ -747,14 +756,24 @@ extern "C" int mlbvh_render_ao(mlbvh_ctx* ctx, uint32_t* d_pixels, float* d_ao, 
     }
     dim3 g = image_grid(width, height);
     hipStream_t s = (hipStream_t)stream;
-    if (d_counters)
-        k_render_ao<true><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, (const uint32_t*)d_root,
-                                                   (const Node*)d_nodes, (const Tri*)d_triangles, useSobol, nSample,
-                                                   radius, seed, d_counters);
-    else
-        k_render_ao<false><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, (const uint32_t*)d_root,
-                                                    (const Node*)d_nodes, (const Tri*)d_triangles, useSobol, nSample,
-                                                    radius, seed, nullptr);
+    const uint32_t* root = (const uint32_t*)d_root;
+    const Node* nodes = (const Node*)d_nodes;
+    const Tri* tris = (const Tri*)d_triangles;
+    if (d_counters) {
+        if (useSobol)
+            k_render_ao<true, true><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, root, nodes, tris,
+                                                              nSample, radius, seed, d_counters);
+        else
+            k_render_ao<true, false><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, root, nodes,
+                                                               tris, nSample, radius, seed, d_counters);
+    } else {
+        if (useSobol)
+            k_render_ao<false, true><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, root, nodes,
+                                                               tris, nSample, radius, seed, nullptr);
+        else
+            k_render_ao<false, false><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, root, nodes,
+                                                                tris, nSample, radius, seed, nullptr);
+    }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? MLBVH_OK : hip_status(e, "render_ao launch");
 }
