@@ -35,6 +35,7 @@ __global__ void k_batch_init(uint32_t* meshBox, int nMeshes, uint32_t* ctrl, uin
                              uint64_t* status, size_t statusWords)
 {
     for (int i = threadIdx.x; i < nMeshes * 6; i += blockDim.x) meshBox[i] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
+    if (threadIdx.x == 0) ctrl[kCtrlFrontier] = 0;
     sort_state_init(ctrl, nPass, hist, nPass, status, statusWords);
 }
 
@@ -143,21 +144,198 @@ __global__ void __launch_bounds__(kBatchBlock)
     }
 }
 
-// Mesh-local agglomerative climb (build_lbvh, minimum_lbvh.h:284-361) for the
-// sorted position i of every mesh.
-__global__ void __launch_bounds__(kBatchBlock)
-    k_batch_climb(const Tri* __restrict__ tris, const uint32_t* __restrict__ off, uint32_t total, int localBits,
-                  const uint64_t* __restrict__ keys, const uint8_t* __restrict__ deltasAll, Node* nodesAll,
-                  uint32_t* flagsAll, uint32_t* roots)
+// Mesh-local agglomerative climb (build_lbvh, minimum_lbvh.h:284-361) in the
+// two phases of the single-mesh build (k_climb_local / k_climb_upper in
+// build.hip, where the block-local predicate is derived).  A block takes
+// kBatchLeaves consecutive sorted positions, which may belong to several
+// meshes.  In the block's delta array a split between two meshes (or past
+// either end of the batch) holds 255, above every Morton delta (<= 32), so no
+// node that a block finishes reaches across a mesh.  Node, leaf and range
+// values are mesh-local; mesh m's split q is global split off[m] + q and its
+// node is stored at nodesAll + (off[m] - m) + q.
+constexpr int kBatchLeaves = 1024;
+constexpr uint32_t kMeshEdge = 255u;
+
+__global__ void __launch_bounds__(kBatchLeaves)
+    k_batch_climb_local(const Tri* __restrict__ tris, const uint32_t* __restrict__ off, uint32_t total,
+                        int localBits, const uint64_t* __restrict__ keys, const uint8_t* __restrict__ deltasAll,
+                        Node* __restrict__ nodesAll, uint32_t* __restrict__ frontier, uint32_t* __restrict__ ctrl,
+                        uint32_t* __restrict__ roots)
 {
-    uint32_t i = blockIdx.x * kBatchBlock + threadIdx.x;
-    if (i >= total) return;
-    const uint64_t key = keys[i];
-    const uint32_t m = (uint32_t)(key >> (32 + localBits));
-    const uint32_t triIndex = (uint32_t)key & ((1u << localBits) - 1u);
-    const uint32_t base = off[m];
-    climb_leaf(tris + base, off[m + 1] - base, i - base, triIndex, deltasAll + base, nodesAll + (base - m),
-               flagsAll + (base - m), roots + m);
+    constexpr int kL = kBatchLeaves, kWaves = kL / 64;
+    // s_d[j] = delta of global split b0 - 1 + j (j in [0, m]); kMeshEdge across meshes or ends
+    __shared__ uint8_t s_d[kL + 1];
+    __shared__ uint8_t s_pm[kL];
+    __shared__ uint8_t s_sm[kL + 1];
+    __shared__ uint32_t s_wmax[2][kWaves];
+    __shared__ uint32_t s_mesh[kL];  // mesh of position b0 + t
+    __shared__ uint32_t s_edge[2];   // mesh of positions b0 - 1 and b1 + 1 (~0: none)
+    __shared__ uint32_t s_flag[kL];
+    __shared__ Box s_box[kL][2];
+    __shared__ uint32_t s_child[kL][2];
+    __shared__ uint32_t s_parent[kL];
+    __shared__ uint32_t s_context[kL];
+    __shared__ uint32_t s_wcount[kWaves];
+    __shared__ uint32_t s_base;
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t b0 = blockIdx.x * kL;
+    const uint32_t m = total - b0 < (uint32_t)kL ? total - b0 : (uint32_t)kL;
+    const uint32_t b1 = b0 + m - 1;
+    const int meshShift = 32 + localBits;
+
+    uint64_t key = 0;
+    uint32_t mesh = 0;
+    if ((uint32_t)t < m) {
+        key = keys[b0 + t];
+        mesh = (uint32_t)(key >> meshShift);
+        s_mesh[t] = mesh;
+    }
+    if (t == 0) s_edge[0] = b0 > 0 ? (uint32_t)(keys[b0 - 1] >> meshShift) : 0xFFFFFFFFu;
+    if (t == 1) s_edge[1] = b1 + 1 < total ? (uint32_t)(keys[b1 + 1] >> meshShift) : 0xFFFFFFFFu;
+    s_flag[t] = 0xFFFFFFFFu;
+    s_parent[t] = kInvalidNode;
+    __syncthreads();
+    auto delta_at = [&](uint32_t j) -> uint32_t {
+        if (j > m) return 0u;  // past a partial block: never decides anything
+        const uint32_t meshA = j == 0 ? s_edge[0] : s_mesh[j - 1];
+        const uint32_t meshB = j == m ? s_edge[1] : s_mesh[j];
+        if (meshA == 0xFFFFFFFFu || meshB == 0xFFFFFFFFu || meshA != meshB) return kMeshEdge;
+        return (uint32_t)deltasAll[b0 - 1 + j];
+    };
+    uint32_t e0 = delta_at((uint32_t)t), e1 = delta_at((uint32_t)t + 1);
+    s_d[t] = (uint8_t)e0;
+    if (t == kL - 1) s_d[kL] = (uint8_t)e1;
+    uint32_t pm = e0, sm = e1;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t up = __shfl_up(pm, o, 64);
+        uint32_t dn = __shfl_down(sm, o, 64);
+        if (lane >= o) pm = pm > up ? pm : up;
+        if (lane + o < 64) sm = sm > dn ? sm : dn;
+    }
+    if (lane == 63) s_wmax[0][wave] = pm;
+    if (lane == 0) s_wmax[1][wave] = sm;
+    __syncthreads();
+    for (int w = 0; w < wave; w++) pm = pm > s_wmax[0][w] ? pm : s_wmax[0][w];
+    for (int w = wave + 1; w < kWaves; w++) sm = sm > s_wmax[1][w] ? sm : s_wmax[1][w];
+    s_pm[t] = (uint8_t)pm;
+    s_sm[t + 1] = (uint8_t)sm;
+    __syncthreads();
+
+    bool rootHere = false, emit = false;
+    uint32_t node = 0, lo = 0, hi = 0;
+    if ((uint32_t)t < m) {
+        const uint32_t base = off[mesh];
+        const uint32_t nInternals = off[mesh + 1] - base - 1;
+        const uint32_t triIndex = (uint32_t)key & ((1u << localBits) - 1u);
+        Box box = tri_box(tris[base + triIndex]);
+        lo = hi = b0 + (uint32_t)t - base;
+        node = triIndex | kLeafBit;
+        const uint32_t gOff = base - b0 + 1;  // s_d index of mesh-local split q is q + gOff
+        while (true) {
+            if (hi - lo >= nInternals) {
+                rootHere = true;
+                break;
+            }
+            uint32_t dL = 0 < lo ? (uint32_t)s_d[lo - 1 + gOff] : 0xFFFFFFFFu;
+            uint32_t dR = hi < nInternals ? (uint32_t)s_d[hi + gOff] : 0xFFFFFFFFu;
+            int goLeft = dL < dR ? 1 : 0;
+            uint32_t parent = goLeft ? lo - 1 : hi;
+            bool local = goLeft ? base + lo > b0 : base + hi < b1;
+            if (local) {
+                uint32_t jp = parent + gOff;
+                uint32_t dp = s_d[jp];
+                local = (uint32_t)s_pm[jp - 1] >= dp && (uint32_t)s_sm[jp + 1] > dp;
+            }
+            if (!local) {
+                emit = true;
+                break;
+            }
+            const uint32_t slot = parent + gOff - 1;
+            if (!(node & kLeafBit)) s_parent[node + gOff - 1] = parent;
+            s_child[slot][goLeft] = node;
+            s_box[slot][goLeft] = box;
+            uint32_t idx = __hip_atomic_exchange(&s_flag[slot], goLeft ? hi : lo, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (idx == 0xFFFFFFFFu) break;
+            lo = lo < idx ? lo : idx;
+            hi = hi < idx ? idx : hi;
+            s_context[slot] = hi < nInternals ? hi : 0xFFFFFFFFu;
+            node = parent;
+            Box other = s_box[slot][goLeft ^ 1];
+            box.lo = hmin3(box.lo, other.lo);
+            box.hi = hmax3(box.hi, other.hi);
+        }
+    }
+    if (rootHere) roots[mesh] = node;
+
+    const uint64_t emitMask = __ballot(emit);
+    if (lane == 0) s_wcount[wave] = (uint32_t)__popcll(emitMask);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t sum = 0;
+        for (int w = 0; w < kWaves; w++) {
+            uint32_t c = s_wcount[w];
+            s_wcount[w] = sum;
+            sum += c;
+        }
+        s_base = sum ? atomicAdd(&ctrl[kCtrlFrontier], sum) : 0u;
+    }
+    __syncthreads();
+    if (emit) {
+        const uint32_t k = s_base + s_wcount[wave] + (uint32_t)__popcll(emitMask & ((1ull << lane) - 1ull));
+        uint4* f = reinterpret_cast<uint4*>(frontier) + k;
+        *f = make_uint4(node, lo, hi, mesh);
+    }
+
+    // whole block-local nodes: slot s is global split b0 + s of mesh s_mesh[s]
+    const uint32_t nSlots = m - 1;
+    for (uint32_t q = (uint32_t)t; q < 4 * nSlots; q += kL) {
+        const uint32_t slot = q >> 2, part = q & 3;
+        if (s_flag[slot] == 0xFFFFFFFFu) continue;
+        const Box& l = s_box[slot][0];
+        const Box& r = s_box[slot][1];
+        uint4 v;
+        if (part == 0) {
+            v = make_uint4(s_child[slot][0], s_child[slot][1], __float_as_uint(l.lo.x), __float_as_uint(l.lo.y));
+        } else if (part == 1) {
+            v = make_uint4(__float_as_uint(l.lo.z), __float_as_uint(l.hi.x), __float_as_uint(l.hi.y),
+                           __float_as_uint(l.hi.z));
+        } else if (part == 2) {
+            v = make_uint4(__float_as_uint(r.lo.x), __float_as_uint(r.lo.y), __float_as_uint(r.lo.z),
+                           __float_as_uint(r.hi.x));
+        } else {
+            v = make_uint4(__float_as_uint(r.hi.y), __float_as_uint(r.hi.z), s_parent[slot], s_context[slot]);
+        }
+        reinterpret_cast<uint4*>(nodesAll + (b0 + slot - s_mesh[slot]))[part] = v;
+    }
+}
+
+// Phase 2: the frontier entries (node, lo, hi, mesh) climb on through the
+// device-scope arrival words of their mesh.
+__global__ void __launch_bounds__(kBatchBlock)
+    k_batch_climb_upper(const Tri* __restrict__ tris, const uint32_t* __restrict__ off,
+                        const uint8_t* __restrict__ deltasAll, Node* nodesAll, uint32_t* flagsAll,
+                        const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ ctrl, uint32_t* roots)
+{
+    const uint32_t count = ctrl[kCtrlFrontier];
+    for (uint32_t i = blockIdx.x * kBatchBlock + threadIdx.x; i < count; i += gridDim.x * kBatchBlock) {
+        const uint4 e = reinterpret_cast<const uint4*>(frontier)[i];
+        const uint32_t node = e.x, lo = e.y, hi = e.z, mesh = e.w;
+        const uint32_t base = off[mesh];
+        Node* nodes = nodesAll + (base - mesh);
+        Box box;
+        if (node & kLeafBit) {
+            box = tri_box(tris[base + (node & ~kLeafBit)]);
+        } else {
+            const Node& nd = nodes[node];
+            box.lo = hmin3(nd.box[0].lo, nd.box[1].lo);
+            box.hi = hmax3(nd.box[0].hi, nd.box[1].hi);
+        }
+        climb_from(off[mesh + 1] - base, lo, hi, node, box, deltasAll + base, nodes, flagsAll + (base - mesh),
+                   roots + mesh);
+    }
 }
 
 static int bits_for(uint32_t v)  // smallest b with (1 << b) >= v
@@ -202,6 +380,7 @@ extern "C" int mlbvh_build_batched(mlbvh_ctx* ctx, const void* d_triangles, cons
     if ((rc = ensure(&ctx->deltas, &ctx->deltasCap, total, "hipMalloc(deltas)"))) return rc;
     if ((rc = ensure(&ctx->flags, &ctx->flagsCap, total, "hipMalloc(flags)"))) return rc;
     if ((rc = ensure(&ctx->meshBox, &ctx->meshBoxCap, (size_t)nMeshes * 6, "hipMalloc(meshBox)"))) return rc;
+    if ((rc = ensure(&ctx->frontier, &ctx->frontierCap, total * 4, "hipMalloc(climb frontier)"))) return rc;
     if ((rc = ensure_status(ctx, sort_tiles(ctx, totalTriangles)))) return rc;
     unsigned g = (unsigned)((total + kBatchBlock - 1) / kBatchBlock);
     unsigned gKeys = g < (unsigned)ctx->cuCount * 4 ? g : (unsigned)ctx->cuCount * 4;
@@ -218,8 +397,16 @@ extern "C" int mlbvh_build_batched(mlbvh_ctx* ctx, const void* d_triangles, cons
     rc = sort_records_u64(ctx, ctx->recA, ctx->recB, totalTriangles, localBits, localBits + keyBits, true, s);
     if (rc) return rc;
     k_batch_deltas<<<g, kBatchBlock, 0, s>>>(ctx->recA, totalTriangles, localBits, ctx->deltas);
-    k_batch_climb<<<g, kBatchBlock, 0, s>>>(tris, d_triOffsets, totalTriangles, localBits, ctx->recA, ctx->deltas,
-                                            (Node*)d_nodes, ctx->flags, d_roots);
+    unsigned gLocal = (unsigned)((total + kBatchLeaves - 1) / kBatchLeaves);
+    k_batch_climb_local<<<gLocal, kBatchLeaves, 0, s>>>(tris, d_triOffsets, totalTriangles, localBits, ctx->recA,
+                                                        ctx->deltas, (Node*)d_nodes, ctx->frontier, ctx->ctrl,
+                                                        d_roots);
+    // one lane per expected entry (a few percent of the positions), at most 8
+    // blocks per CU; lanes stride over a larger list
+    size_t want = (total / 32 + kBatchBlock - 1) / kBatchBlock + 1;
+    unsigned cap = (unsigned)ctx->cuCount * 8;
+    k_batch_climb_upper<<<want < cap ? (unsigned)want : cap, kBatchBlock, 0, s>>>(
+        tris, d_triOffsets, ctx->deltas, (Node*)d_nodes, ctx->flags, ctx->frontier, ctx->ctrl, d_roots);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? MLBVH_OK : hip_status(e, "build_batched launch");
 }
