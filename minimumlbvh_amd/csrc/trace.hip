@@ -22,15 +22,15 @@ constexpr int kTraceBlock = 256;
 constexpr int kAoRefill = MLBVH_AO_REFILL;  // lanes waiting before a wave starts new AO rays
 // Per-lane stack entries of the AO kernel's walks: 12 entries (24 KB of LDS
 // per block) let 6 blocks share a CU, against 5 with 16; the AO frame of the
-// level-9 sphere took 5.27 ms against 5.56 ms, the rarer stack-free restarts
-// costing 0.1 % more node fetches.
-#ifndef MLBVH_AO_STACK
-#define MLBVH_AO_STACK 12
-#endif
+// level-9 sphere took 5.27 ms against 5.56 ms, the more frequent stack-free
+// restarts costing 0.1 % more node fetches.  The oracle's AO walks use the
+// same depth (ORC_AO_STACK), so the kernel's counters can be compared.
+constexpr uint32_t kAoStack = 12;
+// Waves per SIMD the AO kernel is compiled for (the register budget): 6
+// matches the 6 blocks per CU its LDS allows.
 #ifndef MLBVH_AO_WAVES
 #define MLBVH_AO_WAVES 6
 #endif
-constexpr uint32_t kAoStack = MLBVH_AO_STACK;
 
 // Walks of k_trace.  kWalkStack: traverse_stack, the hits of
 // intersect_stackfree with each internal node fetched once per visit (the
