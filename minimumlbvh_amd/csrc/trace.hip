@@ -464,6 +464,210 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
     }
 }
 
+// AO with a per-wave pool of AO rays (Sobol sampling).  The lanes of a wave
+// still trace their own pixels' primary rays, but an AO ray is no longer
+// bound to its pixel's lane: once a primary ray hits, its pixel's nSample AO
+// rays join the wave's pool (the hit distance and normal go to LDS), and any
+// lane whose ray has ended takes the next ray of the pool.  Lanes of a tile
+// with short walks then take over the work of lanes with long ones, and a
+// lane whose pixel missed still traces AO rays.  Each ray is traced exactly
+// as in k_render_ao (same walk, same Sobol sample, same counters); a pixel's
+// occluded samples are counted with LDS atomics and turned into its AO value
+// once the wave's pool is empty.
+#ifndef MLBVH_AO_POOL_STACK
+#define MLBVH_AO_POOL_STACK 12
+#endif
+constexpr uint32_t kAoPoolStack = MLBVH_AO_POOL_STACK;
+
+template <bool kCount>
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(MLBVH_AO_WAVES)))
+    k_render_ao_pool(uint32_t* pixels, float* aoOut, int w, int h, mlbvh_raygen cam,
+                     const uint32_t* __restrict__ rootPtr, const Node* __restrict__ nodes,
+                     const Tri* __restrict__ tris, int nSample, float radius, uint32_t seed,
+                     unsigned long long* counters)
+{
+    constexpr int kWaves = kTraceBlock / 64;
+    __shared__ uint32_t s_stack[kAoPoolStack][kTraceBlock];
+    __shared__ float s_entry[kAoPoolStack][kTraceBlock];
+    __shared__ float4 s_hit[kWaves][64];   // (t, shading normal) of each pixel's primary hit
+    __shared__ uint32_t s_occ[kWaves][64];  // occluded samples per pixel
+    __shared__ uint8_t s_ready[kWaves][64]; // pixels whose AO rays are in the pool, in arrival order
+    int xi, yi;
+    tile_pixel(&xi, &yi);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int tileX = xi - (lane & 7), tileY = yi - (lane >> 3);
+    const bool inImage = xi < w && yi < h;
+    const uint32_t root = *rootPtr;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t nRays = 0, nFetch = 0, nTest = 0;
+    s_occ[wave][lane] = 0;
+
+    f3 o = mk(0, 0, 0), d = mk(0, 0, 1), iv = mk(0, 0, 1);
+    f3 n = mk(0, 0, 0);  // primary: the closest hit's geometric normal
+    float tmax = kFltMax;
+    bool primary = inImage, live = inImage, stackFree = false, hasHit = false, justHit = false;
+    uint32_t curr = kInvalidNode, prev = kInvalidNode, pend = 0, sp = 0, rayPix = (uint32_t)lane;
+    if (inImage) {
+        shoot(cam, &o, &d, (float)xi / (float)w, (float)yi / (float)h);
+        iv = inv_rd(d);
+        curr = root;
+    }
+    // wave-uniform pool state: pixels in the pool, AO rays handed out
+    uint32_t readyCount = 0, nextTicket = 0;
+    uint32_t guard = kMaxTraversalSteps;
+    while (guard-- != 0) {
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t readyMask = __ballot(justHit);
+        if (justHit) {
+            s_ready[wave][readyCount + (uint32_t)__popcll(readyMask & below)] = (uint8_t)lane;
+            justHit = false;
+        }
+        readyCount += (uint32_t)__popcll(readyMask);
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t liveMask = __ballot(live);
+        const uint32_t avail = (uint32_t)nSample * readyCount - nextTicket;
+        if (liveMask == 0 && avail == 0) break;
+        const uint64_t needMask = ~liveMask;
+        const uint32_t nNeed = (uint32_t)__popcll(needMask);
+        if (avail != 0 && (liveMask == 0 || (uint32_t)kAoRefill <= nNeed)) {
+            const uint32_t take = nNeed < avail ? nNeed : avail;
+            if (!live) {
+                const uint32_t rank = (uint32_t)__popcll(needMask & below);
+                if (rank < take) {
+                    const uint32_t ticket = nextTicket + rank;
+                    rayPix = s_ready[wave][ticket / (uint32_t)nSample];
+                    const uint32_t sample = ticket % (uint32_t)nSample;
+                    const float4 hv = s_hit[wave][rayPix];
+                    const int px = tileX + (int)(rayPix & 7), py = tileY + (int)(rayPix >> 3);
+                    f3 po, pd;
+                    shoot(cam, &po, &pd, (float)px / (float)w, (float)py / (float)h);
+                    const f3 nn = mk(hv.y, hv.z, hv.w);
+                    f3 xaxis, zaxis;
+                    onb(nn, &xaxis, &zaxis);
+                    o = (po + pd * hv.x) + nn * 0.001f;
+                    float r0, r1;
+                    sobol2d(&r0, &r1, sample, (uint32_t)px, (uint32_t)py, seed);
+                    const f3 dl = hemisphere_cos(r0, r1);
+                    d = ((xaxis * dl.x) + (zaxis * dl.z)) + (nn * dl.y);
+                    iv = inv_rd(d);
+                    tmax = radius;
+                    curr = root;
+                    prev = kInvalidNode;
+                    pend = 0;
+                    sp = 0;
+                    stackFree = false;
+                    live = true;
+                }
+            }
+            nextTicket += take;
+        }
+        if (!live) continue;
+        bool done = false, blocked = false, pop = false;
+        if (curr & kLeafBit) {
+            const Tri tri = tris[curr & ~kLeafBit];
+            if (kCount) nTest++;
+            float tt, u, v;
+            f3 ng;
+            const bool hitTri = isect_tri(&tt, &u, &v, &ng, 0.0f, tmax, o, d, tri.v[0], tri.v[1], tri.v[2]);
+            if (hitTri && !primary) {
+                done = blocked = true;
+            } else {
+                if (hitTri) {  // primary: a closer hit (or an equal-t one visited later)
+                    tmax = tt;
+                    n = ng;
+                }
+                if (stackFree) {
+                    uint32_t tmp = curr;
+                    curr = prev;
+                    prev = tmp;
+                } else {
+                    pop = true;
+                }
+            }
+        } else {
+            const NodeView nd = load_node(nodes, curr);
+            if (kCount) nFetch++;
+            const float2 rl = slabs(o, iv, nd.lo0, nd.hi0, tmax);
+            const float2 rr = slabs(o, iv, nd.lo1, nd.hi1, tmax);
+            const bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y, rFirst = rr.x < rl.x;
+            if (stackFree) {
+                const StepOut st = stackfree_decide(nd, prev, hitL, hitR, rFirst, pend);
+                if (st.done || st.next == kInvalidNode) {
+                    done = true;
+                } else {
+                    prev = curr;
+                    curr = st.next;
+                }
+            } else if (hitL && hitR) {
+                if (sp == kAoPoolStack) {  // stack full: restart this ray stack-free
+                    stackFree = true;
+                    curr = root;
+                    prev = kInvalidNode;
+                    pend = 0;
+                } else {
+                    s_stack[sp][t] = rFirst ? nd.c0 : nd.c1;
+                    s_entry[sp][t] = rFirst ? rl.x : rr.x;
+                    sp++;
+                    curr = rFirst ? nd.c1 : nd.c0;
+                }
+            } else if (hitL || hitR) {
+                curr = hitL ? nd.c0 : nd.c1;
+            } else {
+                pop = true;
+            }
+        }
+        if (pop) {
+            done = true;
+            while (sp != 0) {
+                --sp;
+                if (s_entry[sp][t] <= tmax) {
+                    curr = s_stack[sp][t];
+                    done = false;
+                    break;
+                }
+            }
+        }
+        if (done) {
+            nRays++;
+            live = false;
+            if (primary) {
+                primary = false;
+                hasHit = tmax != kFltMax;
+                if (hasHit) {
+                    f3 nn = normalize(n);
+                    if (0.0f < dot(nn, d)) nn = -nn;
+                    s_hit[wave][lane] = make_float4(tmax, nn.x, nn.y, nn.z);
+                    justHit = 0 < nSample;
+                }
+            } else if (blocked) {
+                atomicAdd(&s_occ[wave][rayPix], 1u);
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (inImage) {
+        const int pixel = xi + yi * w;
+        if (!hasHit) {
+            pixels[pixel] = pack_rgba(0, 0, 0);
+            if (aoOut) aoOut[pixel] = -1.0f;
+        } else {
+            const uint32_t occluded = s_occ[wave][lane];
+            // ao += 1/N once per occluded sample, in float, as the reference does
+            const float step = 1.0f / (float)nSample;
+            float ao = 0.0f;
+            for (uint32_t k = 0; k < occluded; k++) ao += step;
+            float brightness = powf(1.0f - ao, 1.0f / 2.2f);
+            pixels[pixel] = pack_rgba(brightness, brightness, brightness);
+            if (aoOut) aoOut[pixel] = ao;
+        }
+    }
+    if (kCount) {
+        wave_add_u64(&counters[0], nRays);
+        wave_add_u64(&counters[1], nFetch);
+        wave_add_u64(&counters[2], nTest);
+    }
+}
+
 struct Attrib {
     f3 sn[3];
     f3 reflectance;
@@ -759,7 +963,17 @@ extern "C" int mlbvh_render_ao(mlbvh_ctx* ctx, uint32_t* d_pixels, float* d_ao, 
     const uint32_t* root = (const uint32_t*)d_root;
     const Node* nodes = (const Node*)d_nodes;
     const Tri* tris = (const Tri*)d_triangles;
-    if (d_counters) {
+#ifndef MLBVH_AO_POOL
+#define MLBVH_AO_POOL 1
+#endif
+    if (MLBVH_AO_POOL && useSobol && nSample > 0 && nSample <= 255) {
+        if (d_counters)
+            k_render_ao_pool<true><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, root, nodes, tris,
+                                                             nSample, radius, seed, d_counters);
+        else
+            k_render_ao_pool<false><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, root, nodes,
+                                                              tris, nSample, radius, seed, nullptr);
+    } else if (d_counters) {
         if (useSobol)
             k_render_ao<true, true><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, root, nodes, tris,
                                                               nSample, radius, seed, d_counters);

@@ -145,18 +145,22 @@ def test_render_normal_bit_exact(gpu, orc):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("w,h,nsample,radius,seed", [(256, 256, 4, 0.1, 0), (200, 150, 16, 1.0, 12345),
-                                                     (67, 45, 3, 0.05, 7)])
-def test_render_ao_matches_oracle(gpu, orc, w, h, nsample, radius, seed):
+@pytest.mark.parametrize("w,h,nsample,radius,seed,sobol", [(256, 256, 4, 0.1, 0, 1), (200, 150, 16, 1.0, 12345, 1),
+                                                           (67, 45, 3, 0.05, 7, 1), (96, 64, 8, 0.5, 3, 0),
+                                                           (40, 30, 300, 0.3, 5, 1)])
+def test_render_ao_matches_oracle(gpu, orc, w, h, nsample, radius, seed, sobol):
+    """Sobol sampling with up to 255 samples runs the per-wave ray-pool kernel;
+    PCG sampling (sobol=0) and more samples run the per-lane kernel."""
     tris, nodes, root, cam = small_scene(orc)
     g = gpu.RayGenerator().lookat(*cam, w, h)
     og = orc.raygen_lookat(*cam, w, h)
-    rpx, rao, rcnt = orc.render_ao(og, w, h, root, nodes, tris, 1, nsample, radius, seed)
+    rpx, rao, rcnt = orc.render_ao(og, w, h, root, nodes, tris, sobol, nsample, radius, seed)
     nt, rt = upload_bvh(nodes, root)
     px = torch.zeros(w * h, dtype=torch.int32, device="cuda")
     ao = torch.zeros(w * h, dtype=torch.float32, device="cuda")
     cnt = torch.zeros(3, dtype=torch.int64, device="cuda")
-    gpu.render_ao(px, w, h, g, rt, nt, torch.from_numpy(tris).cuda(), 1, nsample, radius, seed, ao=ao, counters=cnt)
+    gpu.render_ao(px, w, h, g, rt, nt, torch.from_numpy(tris).cuda(), sobol, nsample, radius, seed, ao=ao,
+                  counters=cnt)
     torch.cuda.synchronize()
     gao = ao.cpu().numpy()
     gpx = px.cpu().numpy().view(np.uint32)
@@ -178,7 +182,7 @@ def test_render_ao_matches_oracle(gpu, orc, w, h, nsample, radius, seed):
     assert gc[0] == rcnt[0]  # rays traced: 1 + N per hit pixel
     # the kernel's walks: stack closest-hit primary ray (pop re-test), stack
     # any-hit AO rays
-    _, eao, ecnt = orc.render_ao(og, w, h, root, nodes, tris, 1, nsample, radius, seed, primary_stack=True)
+    _, eao, ecnt = orc.render_ao(og, w, h, root, nodes, tris, sobol, nsample, radius, seed, primary_stack=True)
     assert np.array_equal(eao, rao)
     if (diff == 0).all():
         assert np.array_equal(gc, ecnt)
