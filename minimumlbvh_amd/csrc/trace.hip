@@ -274,6 +274,85 @@ __global__ void __launch_bounds__(kTraceBlock)
 
 // 6 waves per SIMD (at most 80 VGPRs), matching the 6 blocks per CU that its
 // LDS allows.
+// One step of a ray of the AO kernels: the LDS stack walk with entry-distance
+// re-test (this lane's stack entry k at stk[k * kTraceBlock]), stack-free
+// from the root after a push past kDepth entries.  A primary ray (closest
+// hit) narrows tmax and keeps the hit's geometric normal in n; an AO ray ends
+// at its first hit (*blocked).  Returns true when the ray has ended.
+template <bool kCount, uint32_t kDepth>
+__device__ inline bool ao_walk_step(const Node* __restrict__ nodes, const Tri* __restrict__ tris, uint32_t root,
+                                    bool primary, f3 o, f3 d, f3 iv, float& tmax, f3& n, uint32_t& curr,
+                                    uint32_t& prev, uint32_t& pend, uint32_t& sp, bool& stackFree, uint32_t* stk,
+                                    float* ent, bool& blocked, uint32_t& nFetch, uint32_t& nTest)
+{
+    bool done = false, pop = false;
+    if (curr & kLeafBit) {
+        const Tri tri = tris[curr & ~kLeafBit];
+        if (kCount) nTest++;
+        float tt, u, v;
+        f3 ng;
+        const bool hitTri = isect_tri(&tt, &u, &v, &ng, 0.0f, tmax, o, d, tri.v[0], tri.v[1], tri.v[2]);
+        if (hitTri && !primary) {
+            done = blocked = true;
+        } else {
+            if (hitTri) {  // primary: a closer hit (or an equal-t one visited later)
+                tmax = tt;
+                n = ng;
+            }
+            if (stackFree) {
+                uint32_t tmp = curr;
+                curr = prev;
+                prev = tmp;
+            } else {
+                pop = true;
+            }
+        }
+    } else {
+        const NodeView nd = load_node(nodes, curr);
+        if (kCount) nFetch++;
+        const float2 rl = slabs(o, iv, nd.lo0, nd.hi0, tmax);
+        const float2 rr = slabs(o, iv, nd.lo1, nd.hi1, tmax);
+        const bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y, rFirst = rr.x < rl.x;
+        if (stackFree) {
+            const StepOut st = stackfree_decide(nd, prev, hitL, hitR, rFirst, pend);
+            if (st.done || st.next == kInvalidNode) {
+                done = true;
+            } else {
+                prev = curr;
+                curr = st.next;
+            }
+        } else if (hitL && hitR) {
+            if (sp == kDepth) {  // stack full: restart this ray stack-free
+                stackFree = true;
+                curr = root;
+                prev = kInvalidNode;
+                pend = 0;
+            } else {
+                stk[sp * kTraceBlock] = rFirst ? nd.c0 : nd.c1;
+                ent[sp * kTraceBlock] = rFirst ? rl.x : rr.x;
+                sp++;
+                curr = rFirst ? nd.c1 : nd.c0;
+            }
+        } else if (hitL || hitR) {
+            curr = hitL ? nd.c0 : nd.c1;
+        } else {
+            pop = true;
+        }
+    }
+    if (pop) {  // the next far child still entered within tmax, if any
+        done = true;
+        while (sp != 0) {
+            --sp;
+            if (ent[sp * kTraceBlock] <= tmax) {
+                curr = stk[sp * kTraceBlock];
+                done = false;
+                break;
+            }
+        }
+    }
+    return done;
+}
+
 // kSobol: AO directions from the Sobol stream (the reference's useSobol),
 // else from the per-pixel PCG stream.
 template <bool kCount, bool kSobol>
@@ -358,71 +437,10 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
             }
         }
         if (!live) continue;
-        bool done = false, blocked = false, pop = false;
-        if (curr & kLeafBit) {
-            const Tri tri = tris[curr & ~kLeafBit];
-            if (kCount) nTest++;
-            float tt, u, v;
-            f3 ng;
-            const bool hitTri = isect_tri(&tt, &u, &v, &ng, 0.0f, tmax, o, d, tri.v[0], tri.v[1], tri.v[2]);
-            if (hitTri && !primary) {
-                done = blocked = true;
-            } else {
-                if (hitTri) {  // primary: a closer hit (or an equal-t one visited later)
-                    tmax = tt;
-                    n = ng;
-                }
-                if (stackFree) {
-                    uint32_t tmp = curr;
-                    curr = prev;
-                    prev = tmp;
-                } else {
-                    pop = true;
-                }
-            }
-        } else {
-            const NodeView nd = load_node(nodes, curr);
-            if (kCount) nFetch++;
-            const float2 rl = slabs(o, iv, nd.lo0, nd.hi0, tmax);
-            const float2 rr = slabs(o, iv, nd.lo1, nd.hi1, tmax);
-            const bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y, rFirst = rr.x < rl.x;
-            if (stackFree) {
-                const StepOut s = stackfree_decide(nd, prev, hitL, hitR, rFirst, pend);
-                if (s.done || s.next == kInvalidNode) {
-                    done = true;
-                } else {
-                    prev = curr;
-                    curr = s.next;
-                }
-            } else if (hitL && hitR) {
-                if (sp == kAoStack) {  // stack full: restart this ray stack-free
-                    stackFree = true;
-                    curr = root;
-                    prev = kInvalidNode;
-                    pend = 0;
-                } else {
-                    s_stack[sp][t] = rFirst ? nd.c0 : nd.c1;
-                    s_entry[sp][t] = rFirst ? rl.x : rr.x;
-                    sp++;
-                    curr = rFirst ? nd.c1 : nd.c0;
-                }
-            } else if (hitL || hitR) {
-                curr = hitL ? nd.c0 : nd.c1;
-            } else {
-                pop = true;
-            }
-        }
-        if (pop) {  // the next far child still entered within tmax, if any
-            done = true;
-            while (sp != 0) {
-                --sp;
-                if (s_entry[sp][t] <= tmax) {
-                    curr = s_stack[sp][t];
-                    done = false;
-                    break;
-                }
-            }
-        }
+        bool blocked = false;
+        const bool done = ao_walk_step<kCount, kAoStack>(nodes, tris, root, primary, o, d, iv, tmax, n, curr, prev, pend,
+                                                         sp, stackFree, &s_stack[0][t], &s_entry[0][t], blocked,
+                                                         nFetch, nTest);
         if (done) {
             nRays++;
             live = false;
@@ -478,6 +496,11 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
 #define MLBVH_AO_POOL_STACK 12
 #endif
 constexpr uint32_t kAoPoolStack = MLBVH_AO_POOL_STACK;
+// Sobol AO with up to 255 samples runs on the pool kernel (0: the per-lane kernel).
+#ifndef MLBVH_AO_POOL
+#define MLBVH_AO_POOL 1
+#endif
+constexpr bool kAoPool = MLBVH_AO_POOL != 0;
 
 template <bool kCount>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(MLBVH_AO_WAVES)))
@@ -562,71 +585,10 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
             nextTicket += take;
         }
         if (!live) continue;
-        bool done = false, blocked = false, pop = false;
-        if (curr & kLeafBit) {
-            const Tri tri = tris[curr & ~kLeafBit];
-            if (kCount) nTest++;
-            float tt, u, v;
-            f3 ng;
-            const bool hitTri = isect_tri(&tt, &u, &v, &ng, 0.0f, tmax, o, d, tri.v[0], tri.v[1], tri.v[2]);
-            if (hitTri && !primary) {
-                done = blocked = true;
-            } else {
-                if (hitTri) {  // primary: a closer hit (or an equal-t one visited later)
-                    tmax = tt;
-                    n = ng;
-                }
-                if (stackFree) {
-                    uint32_t tmp = curr;
-                    curr = prev;
-                    prev = tmp;
-                } else {
-                    pop = true;
-                }
-            }
-        } else {
-            const NodeView nd = load_node(nodes, curr);
-            if (kCount) nFetch++;
-            const float2 rl = slabs(o, iv, nd.lo0, nd.hi0, tmax);
-            const float2 rr = slabs(o, iv, nd.lo1, nd.hi1, tmax);
-            const bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y, rFirst = rr.x < rl.x;
-            if (stackFree) {
-                const StepOut st = stackfree_decide(nd, prev, hitL, hitR, rFirst, pend);
-                if (st.done || st.next == kInvalidNode) {
-                    done = true;
-                } else {
-                    prev = curr;
-                    curr = st.next;
-                }
-            } else if (hitL && hitR) {
-                if (sp == kAoPoolStack) {  // stack full: restart this ray stack-free
-                    stackFree = true;
-                    curr = root;
-                    prev = kInvalidNode;
-                    pend = 0;
-                } else {
-                    s_stack[sp][t] = rFirst ? nd.c0 : nd.c1;
-                    s_entry[sp][t] = rFirst ? rl.x : rr.x;
-                    sp++;
-                    curr = rFirst ? nd.c1 : nd.c0;
-                }
-            } else if (hitL || hitR) {
-                curr = hitL ? nd.c0 : nd.c1;
-            } else {
-                pop = true;
-            }
-        }
-        if (pop) {
-            done = true;
-            while (sp != 0) {
-                --sp;
-                if (s_entry[sp][t] <= tmax) {
-                    curr = s_stack[sp][t];
-                    done = false;
-                    break;
-                }
-            }
-        }
+        bool blocked = false;
+        const bool done = ao_walk_step<kCount, kAoPoolStack>(nodes, tris, root, primary, o, d, iv, tmax, n, curr, prev,
+                                                             pend, sp, stackFree, &s_stack[0][t], &s_entry[0][t],
+                                                             blocked, nFetch, nTest);
         if (done) {
             nRays++;
             live = false;
@@ -963,10 +925,7 @@ extern "C" int mlbvh_render_ao(mlbvh_ctx* ctx, uint32_t* d_pixels, float* d_ao, 
     const uint32_t* root = (const uint32_t*)d_root;
     const Node* nodes = (const Node*)d_nodes;
     const Tri* tris = (const Tri*)d_triangles;
-#ifndef MLBVH_AO_POOL
-#define MLBVH_AO_POOL 1
-#endif
-    if (MLBVH_AO_POOL && useSobol && nSample > 0 && nSample <= 255) {
+    if (kAoPool && useSobol && nSample > 0 && nSample <= 255) {
         if (d_counters)
             k_render_ao_pool<true><<<g, kTraceBlock, 0, s>>>(d_pixels, d_ao, width, height, *camera, root, nodes, tris,
                                                              nSample, radius, seed, d_counters);
