@@ -297,6 +297,14 @@ __device__ inline uint32_t prefix24(K k, int shift)
     return shift < (int)(8 * sizeof(K)) ? (uint32_t)(k >> shift) & 0xFFFFFFu : 0u;
 }
 
+// The order of a key among keys sharing its 24-bit prefix: its sort bits
+// below the prefix (at most 40) above its window position (12 bits).
+template <class K>
+__device__ inline uint64_t order_word(K k, K lowMask, int pos)
+{
+    return ((uint64_t)(k & lowMask) << 12) | (uint64_t)(uint32_t)pos;
+}
+
 // sortMask selects the key bits [startBit, startBit + 8 * passes) that order
 // keys.  A wave covers 64 consecutive positions per step, so the segment
 // bounds of most keys come from two ballots (segment starts, segment ends);
@@ -307,6 +315,9 @@ __global__ void __launch_bounds__(kFinishBlock)
                      uint32_t* __restrict__ vdst, uint32_t n, int prefixShift, K sortMask, uint32_t* ctrl)
 {
     constexpr int kWin = kFinishChunk + 2 * kFinishHalo;
+    static_assert(kWin <= 4096, "window positions must fit the 12 low bits of order_word");
+    // sort bits below the prefix (the prefix lies inside the key, below its top)
+    const K lowMask = sortMask & (K)(((K)1 << prefixShift) - 1);
     __shared__ K s_k[kWin];
     __shared__ uint32_t s_pf[kWin];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -356,15 +367,18 @@ __global__ void __launch_bounds__(kFinishBlock)
         const int aFast = below ? w0 + 63 - __clzll(below) : w0;
         const int eFast = above ? w0 + __builtin_ctzll(above) + 1 : w0 + 64;
         const K key = s_k[w];
-        const K mine = key & sortMask;
+        // Inside a segment the prefix is common, so keys order by their sort
+        // bits below the prefix, ties by window position: one comparison of
+        // (low bits << 12 | position), window positions being < 4096.
+        const uint64_t mine = order_word(key, lowMask, w);
         // short segment inside this wave's positions: a fixed count of
         // comparisons from its first key (a uniform loop, no per-lane trip count)
         int rank = 0;
 #pragma unroll
         for (int j = 0; j <= kFinishNear; j++) {
             const int idx = aFast + j;
-            const K other = s_k[idx < kWin ? idx : kWin - 1] & sortMask;
-            rank += (idx < eFast && (other < mine || (other == mine && idx < w))) ? 1 : 0;
+            const uint64_t other = order_word(s_k[idx < kWin ? idx : kWin - 1], lowMask, idx);
+            rank += (idx < eFast && other < mine) ? 1 : 0;
         }
         if (!in) continue;
         int a = aFast, e = eFast;
@@ -380,10 +394,7 @@ __global__ void __launch_bounds__(kFinishBlock)
                 continue;
             }
             rank = 0;
-            for (int j = a; j < e; j++) {
-                const K other = s_k[j] & sortMask;
-                rank += (other < mine || (other == mine && j < w)) ? 1 : 0;
-            }
+            for (int j = a; j < e; j++) rank += order_word(s_k[j], lowMask, j) < mine ? 1 : 0;
         }
         dst[lo + a + rank] = key;
         if (kVals) vdst[lo + a + rank] = val[r];
