@@ -138,9 +138,9 @@ __device__ inline void shoot(const mlbvh_raygen& g, f3* ro, f3* rd, float u, flo
 }
 
 // ---------------------------------------------------------------------------
-// Agglomerative climb for one sorted leaf (build_lbvh, minimum_lbvh.h:284-361)
-// on mesh-local indices: leaf position `pos` of an n-triangle mesh, its
-// triangle `triIndex`, deltas of the mesh, its nodes and arrival words.
+// Agglomerative climb (build_lbvh, minimum_lbvh.h:284-361) through device
+// memory, on mesh-local indices: the deltas, nodes and arrival words of one
+// n-triangle mesh.
 //
 // Cross-XCD hand-off without L2 flushes: the only data one thread reads from
 // another is the sibling subtree's AABB.  It is written with device-scope
@@ -249,13 +249,6 @@ __device__ inline bool climb_from(uint32_t n, uint32_t lo, uint32_t hi, uint32_t
     *root = node;
     nodes[node & ~kLeafBit].parent = kInvalidNode;
     return true;
-}
-
-// The whole climb of one sorted leaf through the device-scope protocol.
-__device__ inline bool climb_leaf(const Tri* __restrict__ tris, uint32_t n, uint32_t pos, uint32_t triIndex,
-                                  const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags, uint32_t* root)
-{
-    return climb_from(n, pos, pos, triIndex | kLeafBit, tri_box(tris[triIndex]), deltas, nodes, flags, root);
 }
 
 // Error plumbing shared by the entry points.
