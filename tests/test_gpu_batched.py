@@ -123,3 +123,26 @@ def test_batched_build_trace_graph_replays(gpu):
         assert np.array_equal(out.cpu().numpy().view(np.uint32).reshape(-1, 8), hits), rep
         assert np.array_equal(bb.nodes.cpu().numpy().view(np.uint32), nodes), rep
         assert np.array_equal(bb.roots.cpu().numpy().view(np.uint32), roots), rep
+
+
+def test_batched_full_configuration(gpu):
+    """The batching configuration at full size (4096 meshes, about 1.3e8
+    triangles and 4.8e7 rays, generated on the device): one segmented build +
+    persistent trace gives the same nodes, roots and hits, bit for bit, as the
+    plain per-mesh build + trace loop (each of which the other tests check
+    against the oracle)."""
+    from minimumlbvh_amd import workloads
+
+    sc = workloads.BatchedScene()
+    print(f"batched full size: {sc.n_meshes} meshes, {sc.n_tris} triangles, {sc.n_rays} rays", flush=True)
+    sc.build_trace()
+    torch.cuda.synchronize()
+    nodes, roots, hits = sc.bvh.nodes.clone(), sc.bvh.roots.clone(), sc.hits.clone()
+    sc.bvh.nodes.fill_(-1)
+    sc.bvh.roots.fill_(-1)
+    loop_hits = sc.per_mesh_loop()
+    torch.cuda.synchronize()
+    assert torch.equal(sc.bvh.roots, roots)
+    assert torch.equal(sc.bvh.nodes, nodes)
+    assert torch.equal(loop_hits, hits)
+    assert (hits[:, 7] != -1).float().mean().item() > 0.05
