@@ -40,6 +40,15 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert lib.mlbvh_build(None, None, 10, 0, None, None, None) == _capi.MLBVH_E_INVALID_ARG
     assert lib.mlbvh_trace(None, None, None, None, None, None, 1, 0, None, None) == _capi.MLBVH_E_INVALID_ARG
     assert b"null" in lib.mlbvh_last_error()
+    E = _capi.MLBVH_E_INVALID_ARG
+    assert lib.mlbvh_trace_stackfree(None, None, None, None, None, None, 1, 0, None, None) == E
+    assert lib.mlbvh_trace_escape_link(None, None, None, None, None, None, 1, 0, None, None) == E
+    # indexed entry points check their vertex / index buffers before any device work
+    assert lib.mlbvh_build_indexed(None, None, 3, None, 10, 0, None, None, None, None, None) == E
+    assert b"build_indexed" in lib.mlbvh_last_error()
+    assert lib.mlbvh_refit_indexed(None, None, 0, None, 10, None, None, None, None, None) == E
+    assert lib.mlbvh_trace_indexed(None, None, None, 0, None, None, None, None, 1, 0, None, None) == E
+    assert b"trace_indexed" in lib.mlbvh_last_error()
     out = ctypes.c_void_p()
     assert lib.mlbvh_ctx_create(0, None) == _capi.MLBVH_E_INVALID_ARG
     rc = lib.mlbvh_ctx_create(-1, ctypes.byref(out))
