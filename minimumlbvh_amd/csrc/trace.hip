@@ -32,11 +32,26 @@ constexpr uint32_t kAoStack = 12;
 #define MLBVH_AO_WAVES 6
 #endif
 
-// Walks of k_trace.  kWalkStack: traverse_stack, the hits of
-// intersect_stackfree with each internal node fetched once per visit (the
-// default); kWalkStackFree: the parent-pointer walk itself (traverse);
-// kWalkEscape: the escape-link walk (traverse_escape).
+// Walks of the plain trace.  kWalkStack: the stack walk (traverse_stack /
+// stack_walk_step), the hits of intersect_stackfree with each internal node
+// fetched once per visit (the default; k_trace_ww); kWalkStackFree: the
+// parent-pointer walk itself (traverse); kWalkEscape: the escape-link walk
+// (traverse_escape).  The last two run one ray per lane in k_trace.
 enum : int { kWalkStack = 0, kWalkStackFree = 1, kWalkEscape = 2 };
+
+__device__ inline void write_hit(HitRec* hits, uint32_t i, const Hit& h)
+{
+    HitRec r;
+    r.t = h.t;
+    r.pad = 0;
+    r.u = h.u;
+    r.v = h.v;
+    r.ngx = h.ng.x;
+    r.ngy = h.ng.y;
+    r.ngz = h.ng.z;
+    r.tri = h.tri;
+    hits[i] = r;
+}
 
 // One ray per lane.  TS is the triangle source (`const Tri*` or IndexedTris).
 template <bool kAny, bool kCount, int kWalk, class TS>
@@ -44,8 +59,7 @@ __global__ void __launch_bounds__(kTraceBlock)
     k_trace(const Node* __restrict__ nodes, TS tris, const uint32_t* __restrict__ rootPtr,
             const float4* __restrict__ rays, HitRec* __restrict__ hits, int nRays, unsigned long long* counters)
 {
-    __shared__ uint32_t s_stack[kWalk == kWalkStack ? kLaneStack : 1][kTraceBlock];
-    __shared__ float s_entry[kWalk == kWalkStack ? kLaneStack : 1][kTraceBlock];
+    static_assert(kWalk == kWalkStackFree || kWalk == kWalkEscape, "the stack walk runs in k_trace_ww");
     int i = blockIdx.x * kTraceBlock + threadIdx.x;
     uint32_t nFetch = 0, nTest = 0;
     if (i < nRays) {
@@ -58,21 +72,9 @@ __global__ void __launch_bounds__(kTraceBlock)
         h.tri = 0xFFFFFFFFu;
         if (kWalk == kWalkEscape)
             traverse_escape<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
-        else if (kWalk == kWalkStack)
-            traverse_stack<kAny, kCount, kTraceBlock>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest,
-                                                      &s_stack[0][threadIdx.x], &s_entry[0][threadIdx.x]);
         else
             traverse<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
-        HitRec r;
-        r.t = h.t;
-        r.pad = 0;
-        r.u = h.u;
-        r.v = h.v;
-        r.ngx = h.ng.x;
-        r.ngy = h.ng.y;
-        r.ngz = h.ng.z;
-        r.tri = h.tri;
-        hits[i] = r;
+        write_hit(hits, (uint32_t)i, h);
     }
     if (kCount) {
         wave_add_u64(&counters[0], nFetch);
@@ -82,59 +84,148 @@ __global__ void __launch_bounds__(kTraceBlock)
 
 __global__ void k_queue_reset(uint32_t* counter) { *counter = 0; }
 
-// Persistent work-queue trace over the meshes of a batched build.  Each wave
-// pulls `chunk` consecutive rays per atomic and traces chunk/64 of them per
-// lane with traverse_stack; a ray names its mesh in the Ray pad word (u32).
-// Hits carry mesh-local triangle indices.
-template <bool kAny, bool kCount>
+// "While-while" trace of the stack walk.  Each wave takes `chunk`
+// consecutive rays at a time and hands them to its lanes one by one: a lane
+// whose walk has ended takes the next ray of the wave's chunk once
+// kTraceRefill lanes are idle (or the whole wave is), so lanes with short
+// walks do not wait for the wave's longest one.  Each ray is walked by
+// stack_walk_step exactly as traverse_stack walks it (same hits, same
+// counters).  The chunks come from the work counter `queue` (one atomic per
+// chunk), or, with queue null, wave w of the grid takes chunks w, w + W, ...
+// kBatched: a ray names its mesh in the Ray pad word (u32) and is traced in
+// that mesh's BVH (nodes at nodesAll + off[m] - m, triangles from off[m],
+// root roots[m]); hits carry mesh-local triangle indices, and a ray naming no
+// mesh gets a miss record.  Otherwise off is unused and roots[0] is the root.
+// Against one ray per lane (traverse_stack to the end), 4M closest-hit rays
+// on the level-9 sphere took 3.10 ms instead of 3.68 ms and the 4096-mesh
+// batched trace 28.5 ms instead of 34.7 ms.  Refill at 8 idle lanes: 4 and 16
+// were within 1 %, 2 and 32 slower by 4 % (batched).
+#ifndef MLBVH_TRACE_REFILL
+#define MLBVH_TRACE_REFILL 8
+#endif
+constexpr uint32_t kTraceRefill = MLBVH_TRACE_REFILL;
+// Most rays per wave of the plain trace, whose grid gives each wave one chunk
+// (the hardware dispatcher balances the waves): 512 took 3.10 ms on the 4M
+// rays, 256 3.27, 1024 3.12, 2048 4.15.  Fewer rays fill too few waves at
+// 512 per wave; the chunk then shrinks (to 64, one ray per lane).
+#ifndef MLBVH_PLAIN_CHUNK
+#define MLBVH_PLAIN_CHUNK 512
+#endif
+constexpr uint32_t kPlainChunk = MLBVH_PLAIN_CHUNK;
+
+template <bool kAny, bool kCount, bool kBatched, class TS>
 __global__ void __launch_bounds__(kTraceBlock)
-    k_trace_batched(const Node* __restrict__ nodesAll, const Tri* __restrict__ trisAll,
-                    const uint32_t* __restrict__ off, const uint32_t* __restrict__ roots, int nMeshes,
-                    const float4* __restrict__ rays, HitRec* __restrict__ hits, uint32_t nRays, uint32_t chunk,
-                    uint32_t* queue, unsigned long long* counters)
+    k_trace_ww(const Node* __restrict__ nodesAll, TS trisAll, const uint32_t* __restrict__ off,
+               const uint32_t* __restrict__ roots, int nMeshes, const float4* __restrict__ rays,
+               HitRec* __restrict__ hits, uint32_t nRays, uint32_t chunk, uint32_t* queue,
+               unsigned long long* counters)
 {
     __shared__ uint32_t s_stack[kLaneStack][kTraceBlock];
     __shared__ float s_entry[kLaneStack][kTraceBlock];
-    const int lane = threadIdx.x & 63;
+    const int t = threadIdx.x, lane = t & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
     uint32_t nFetch = 0, nTest = 0;
+    uint32_t cBase = 0, cEnd = 0;  // the wave's chunk of rays not yet handed out
+    uint32_t nextChunk = blockIdx.x * (kTraceBlock / 64) + (t >> 6);
+    bool queueDone = false, live = false;
+    uint32_t ray = 0, nodeOff = 0, triOff = 0, root = kBatched ? 0u : roots[0];
+    f3 ro = mk(0, 0, 0), rd = mk(0, 0, 1), inv = mk(0, 0, 1);
+    Hit h;
+    h.t = 0.0f;
+    h.u = h.v = 0.0f;
+    h.ng = mk(0, 0, 0);
+    h.tri = 0xFFFFFFFFu;
+    StackWalk w = stack_walk_start(root);
+    uint32_t steps = 0;  // of the lane's ray; the cap only stops a malformed node array
     while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(queue, chunk);
-        base = __shfl(base, 0, 64);
-        if (base >= nRays) break;
-        for (uint32_t k = 0; k < chunk; k += 64) {
-            uint32_t i = base + k + lane;
-            if (i >= nRays) break;
-            float4 a = rays[2 * i], b = rays[2 * i + 1];
-            uint32_t m = __float_as_uint(b.w);
-            Hit h;
-            h.t = a.w;
-            h.u = h.v = 0.0f;
-            h.ng = mk(0, 0, 0);
-            h.tri = 0xFFFFFFFFu;
-            if (m < (uint32_t)nMeshes) {
-                uint32_t b0 = off[m];
-                f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
-                traverse_stack<kAny, kCount, kTraceBlock>(nodesAll + (b0 - m), trisAll + b0, roots[m], ro, rd,
-                                                          inv_rd(rd), h, nFetch, nTest, &s_stack[0][threadIdx.x],
-                                                          &s_entry[0][threadIdx.x]);
+        const uint64_t liveMask = __ballot(live);
+        if (liveMask == 0 && queueDone && cBase >= cEnd) break;
+        const uint32_t nNeed = 64u - (uint32_t)__popcll(liveMask);
+        if (liveMask == 0 || kTraceRefill <= nNeed) {
+            if (cBase >= cEnd && !queueDone) {
+                uint32_t base = 0;
+                if (queue) {
+                    if (lane == 0) base = atomicAdd(queue, chunk);
+                    base = __shfl(base, 0, 64);
+                } else {
+                    // no work counter: wave w takes chunks w, w + W, w + 2W, ...
+                    const uint64_t c = (uint64_t)nextChunk * chunk;
+                    base = c < nRays ? (uint32_t)c : nRays;
+                    nextChunk += gridDim.x * (kTraceBlock / 64);
+                }
+                if (base >= nRays) {
+                    queueDone = true;
+                } else {
+                    cBase = base;
+                    cEnd = nRays - base < chunk ? nRays : base + chunk;
+                }
             }
-            HitRec r;
-            r.t = h.t;
-            r.pad = 0;
-            r.u = h.u;
-            r.v = h.v;
-            r.ngx = h.ng.x;
-            r.ngy = h.ng.y;
-            r.ngz = h.ng.z;
-            r.tri = h.tri;
-            hits[i] = r;
+            const uint32_t avail = cEnd - cBase;
+            const uint32_t take = nNeed < avail ? nNeed : avail;
+            if (!live) {
+                const uint32_t rank = (uint32_t)__popcll(~liveMask & below);
+                if (rank < take) {
+                    ray = cBase + rank;
+                    const float4 a = rays[2 * ray], b = rays[2 * ray + 1];
+                    h.t = a.w;
+                    h.u = h.v = 0.0f;
+                    h.ng = mk(0, 0, 0);
+                    h.tri = 0xFFFFFFFFu;
+                    bool valid = true;
+                    if (kBatched) {
+                        const uint32_t m = __float_as_uint(b.w);
+                        valid = m < (uint32_t)nMeshes;
+                        if (valid) {
+                            triOff = off[m];
+                            nodeOff = triOff - m;
+                            root = roots[m];
+                        }
+                    }
+                    if (valid) {
+                        ro = mk(a.x, a.y, a.z);
+                        rd = mk(b.x, b.y, b.z);
+                        inv = inv_rd(rd);
+                        w = stack_walk_start(root);
+                        steps = 0;
+                        live = true;
+                    } else {
+                        write_hit(hits, ray, h);
+                    }
+                }
+            }
+            cBase += take;
+        }
+        if (!live) continue;
+        bool done;
+        if constexpr (kBatched)
+            done = stack_walk_step<kAny, kCount, kTraceBlock, kLaneStack>(nodesAll + nodeOff, trisAll + triOff, root,
+                                                                          ro, rd, inv, h, w, &s_stack[0][t],
+                                                                          &s_entry[0][t], nFetch, nTest);
+        else
+            done = stack_walk_step<kAny, kCount, kTraceBlock, kLaneStack>(nodesAll, trisAll, root, ro, rd, inv, h, w,
+                                                                          &s_stack[0][t], &s_entry[0][t], nFetch,
+                                                                          nTest);
+        if (done || ++steps == kMaxTraversalSteps) {
+            write_hit(hits, ray, h);
+            live = false;
         }
     }
     if (kCount) {
         wave_add_u64(&counters[0], nFetch);
         wave_add_u64(&counters[1], nTest);
     }
+}
+
+// Grid of the persistent batched trace: CUs x blocks per CU, fewer when the
+// rays fill fewer waves.
+static unsigned persistent_grid(const mlbvh_ctx* ctx, uint32_t nRays, uint32_t* chunkOut)
+{
+    const uint32_t chunk = (uint32_t)(ctx->traceChunk < 64 ? 64 : ctx->traceChunk / 64 * 64);
+    unsigned grid = (unsigned)(ctx->cuCount * (ctx->traceBlocksPerCU < 1 ? 1 : ctx->traceBlocksPerCU));
+    const uint64_t waves = ((uint64_t)nRays + chunk - 1) / chunk;
+    if ((uint64_t)grid * (kTraceBlock / 64) > waves) grid = (unsigned)((waves + 3) / 4);
+    *chunkOut = chunk;
+    return grid;
 }
 
 // ---------------------------------------------------------------------------
@@ -762,23 +853,34 @@ static int trace_impl(mlbvh_ctx* ctx, const void* d_nodes, TS tris, const void* 
     }
     if (nRays == 0) return MLBVH_OK;
     hipStream_t s = (hipStream_t)stream;
-    unsigned g = (unsigned)((nRays + kTraceBlock - 1) / kTraceBlock);
     const Node* nodes = (const Node*)d_nodes;
     const uint32_t* root = (const uint32_t*)d_root;
     const float4* rays = (const float4*)d_rays;
     HitRec* hits = (HitRec*)d_hits;
-    bool any = rayQuery == MLBVH_RAY_QUERY_ANY;
-    if (d_counters) {
-        if (any)
-            k_trace<true, true, kWalk><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
-        else
-            k_trace<false, true, kWalk><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, d_counters);
-    } else {
-        if (any)
-            k_trace<true, false, kWalk><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
-        else
-            k_trace<false, false, kWalk><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, nullptr);
+    const uint32_t n = (uint32_t)nRays;
+    // stack walk: one chunk per wave, kPlainChunk rays or fewer so that the
+    // grid still has 16 waves per CU; else one ray per lane
+    uint32_t chunk = n / (uint32_t)(ctx->cuCount * 16) / 64 * 64;
+    chunk = chunk < 64 ? 64 : chunk > kPlainChunk ? kPlainChunk : chunk;
+    const uint32_t perBlock = kWalk == kWalkStack ? chunk * (kTraceBlock / 64) : kTraceBlock;
+    const unsigned g = (unsigned)(((uint64_t)n + perBlock - 1) / perBlock);
+#define LAUNCH(A, C, CNT)                                                                                        \
+    {                                                                                                            \
+        if constexpr (kWalk == kWalkStack)                                                                       \
+            k_trace_ww<A, C, false><<<g, kTraceBlock, 0, s>>>(nodes, tris, nullptr, root, 1, rays, hits, n,      \
+                                                              chunk, nullptr, CNT);                              \
+        else                                                                                                     \
+            k_trace<A, C, kWalk><<<g, kTraceBlock, 0, s>>>(nodes, tris, root, rays, hits, nRays, CNT);           \
     }
+    const bool any = rayQuery == MLBVH_RAY_QUERY_ANY;
+    if (d_counters) {
+        if (any) LAUNCH(true, true, d_counters)
+        else LAUNCH(false, true, d_counters)
+    } else {
+        if (any) LAUNCH(true, false, nullptr)
+        else LAUNCH(false, false, nullptr)
+    }
+#undef LAUNCH
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? MLBVH_OK : hip_status(e, who);
 }
@@ -847,10 +949,8 @@ extern "C" int mlbvh_trace_batched(mlbvh_ctx* ctx, const void* d_nodes, const vo
     if (nRays == 0) return MLBVH_OK;
     hipStream_t s = (hipStream_t)stream;
     uint32_t* queue = ctx->ctrl + kCtrlWorkQueue;
-    uint32_t chunk = (uint32_t)(ctx->traceChunk < 64 ? 64 : ctx->traceChunk / 64 * 64);
-    unsigned grid = (unsigned)(ctx->cuCount * (ctx->traceBlocksPerCU < 1 ? 1 : ctx->traceBlocksPerCU));
-    uint64_t waves = ((uint64_t)nRays + chunk - 1) / chunk;
-    if ((uint64_t)grid * (kTraceBlock / 64) > waves) grid = (unsigned)((waves + 3) / 4);
+    uint32_t chunk;
+    const unsigned grid = persistent_grid(ctx, (uint32_t)nRays, &chunk);
     k_queue_reset<<<1, 1, 0, s>>>(queue);
     const Node* nodes = (const Node*)d_nodes;
     const Tri* tris = (const Tri*)d_triangles;
@@ -859,8 +959,8 @@ extern "C" int mlbvh_trace_batched(mlbvh_ctx* ctx, const void* d_nodes, const vo
     uint32_t n = (uint32_t)nRays;
     bool any = rayQuery == MLBVH_RAY_QUERY_ANY;
 #define LAUNCH(A, C)                                                                                             \
-    k_trace_batched<A, C><<<grid, kTraceBlock, 0, s>>>(nodes, tris, d_triOffsets, d_roots, nMeshes, rays, hits, n, \
-                                                       chunk, queue, d_counters)
+    k_trace_ww<A, C, true><<<grid, kTraceBlock, 0, s>>>(nodes, tris, d_triOffsets, d_roots, nMeshes, rays, hits, n, \
+                                                        chunk, queue, d_counters)
     if (d_counters) {
         if (any) LAUNCH(true, true);
         else LAUNCH(false, true);

@@ -249,6 +249,89 @@ __device__ inline void traverse_stack(const Node* __restrict__ nodes, TS tris, u
     }
 }
 
+// traverse_stack as a single step, for kernels that keep a ray in flight per
+// lane and give a lane its next ray as soon as its walk ends.  The state of a
+// walk; a push past kDepth entries switches it to the stack-free walk from
+// the root (prev, pend), as traverse_stack does.
+struct StackWalk {
+    uint32_t curr, prev, pend, sp;
+    bool stackFree;
+};
+__device__ inline StackWalk stack_walk_start(uint32_t root)
+{
+    return StackWalk{root, kInvalidNode, 0u, 0u, false};
+}
+// One node or leaf of the walk; returns true when the ray's walk has ended.
+template <bool kAny, bool kCount, int kStride, uint32_t kDepth, class TS>
+__device__ inline bool stack_walk_step(const Node* __restrict__ nodes, TS tris, uint32_t root, f3 ro, f3 rd, f3 inv,
+                                       Hit& hit, StackWalk& w, uint32_t* stackNode, float* stackEntry,
+                                       uint32_t& nFetch, uint32_t& nTest)
+{
+    bool pop = false;
+    if (w.curr & kLeafBit) {
+        const Tri tri = tris[w.curr & ~kLeafBit];
+        if (kCount) nTest++;
+        float t, u, v;
+        f3 ng;
+        if (isect_tri(&t, &u, &v, &ng, 0.0f, hit.t, ro, rd, tri.v[0], tri.v[1], tri.v[2])) {
+            hit.t = t;
+            hit.u = u;
+            hit.v = v;
+            hit.ng = ng;
+            hit.tri = w.curr & ~kLeafBit;
+            if (kAny) return true;
+        }
+        if (w.stackFree) {
+            const uint32_t tmp = w.curr;
+            w.curr = w.prev;
+            w.prev = tmp;
+            return w.curr == kInvalidNode;
+        }
+        pop = true;
+    } else {
+        const NodeView nd = load_node(nodes, w.curr);
+        if (kCount) nFetch++;
+        const float2 rl = slabs(ro, inv, nd.lo0, nd.hi0, hit.t);
+        const float2 rr = slabs(ro, inv, nd.lo1, nd.hi1, hit.t);
+        const bool hitL = rl.x <= rl.y, hitR = rr.x <= rr.y, rFirst = rr.x < rl.x;
+        if (w.stackFree) {
+            const StepOut st = stackfree_decide(nd, w.prev, hitL, hitR, rFirst, w.pend);
+            if (st.done || st.next == kInvalidNode) return true;
+            w.prev = w.curr;
+            w.curr = st.next;
+            return false;
+        }
+        if (hitL && hitR) {
+            if (w.sp == kDepth) {
+                w.stackFree = true;
+                w.curr = root;
+                w.prev = kInvalidNode;
+                w.pend = 0;
+                return false;
+            }
+            stackNode[w.sp * kStride] = rFirst ? nd.c0 : nd.c1;
+            stackEntry[w.sp * kStride] = rFirst ? rl.x : rr.x;
+            w.sp++;
+            w.curr = rFirst ? nd.c1 : nd.c0;
+        } else if (hitL || hitR) {
+            w.curr = hitL ? nd.c0 : nd.c1;
+        } else {
+            pop = true;
+        }
+    }
+    if (pop) {
+        while (w.sp != 0) {
+            --w.sp;
+            if (stackEntry[w.sp * kStride] <= hit.t) {
+                w.curr = stackNode[w.sp * kStride];
+                return false;
+            }
+        }
+        return true;
+    }
+    return false;
+}
+
 // Escape-link walk (intersect_escape_link, minimum_lbvh.h:1063-1137).  An
 // internal-node visit tests ONE child box: the left one when arriving from
 // the parent, else the right one.  A missed left box re-enters the same node
