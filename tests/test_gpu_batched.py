@@ -79,6 +79,29 @@ def test_batched_any_hit_and_tuning(gpu, orc):
         assert np.array_equal(ref[roff[m]:roff[m + 1]], oh.view(np.uint32).reshape(-1, 8))
 
 
+def test_batched_ray_naming_no_mesh_misses(gpu):
+    """A ray whose mesh id is not a mesh of the batch gets a miss record
+    (t = its tmax, uv and ng zero, triangle 0xFFFFFFFF); the other rays'
+    hits are unchanged."""
+    counts = np.array([300, 5000, 64])
+    rcounts = np.array([700, 900, 300])
+    tris, toff, rays, roff = scenes.batch_scene(counts, rcounts, seed=13)
+    _, _, _, _, _, ref = _run_batched(gpu, tris, toff, rays)
+    bad = rays.copy()
+    idx = np.arange(0, bad.shape[0], 7)
+    mesh = bad[:, 7].view(np.uint32)
+    mesh[idx] = np.where(idx % 2, 3, 0xFFFFFFFF).astype(np.uint32)
+    bad[idx[::3], 3] = 0.75
+    _, _, _, _, _, hits = _run_batched(gpu, tris, toff, bad)
+    keep = np.ones(bad.shape[0], bool)
+    keep[idx] = False
+    assert np.array_equal(hits[keep], ref[keep])
+    miss = np.zeros((idx.size, 8), np.uint32)
+    miss[:, 0] = bad[idx, 3].view(np.uint32)
+    miss[:, 7] = 0xFFFFFFFF
+    assert np.array_equal(hits[idx], miss)
+
+
 def test_batched_rejects_bad_offsets(gpu):
     with pytest.raises(ValueError):
         gpu.BatchedBVH([0, 5, 6])  # a one-triangle mesh

@@ -86,6 +86,40 @@ def test_trace_on_gpu_built_bvh(gpu, orc):
 
 
 @pytest.mark.parametrize("query", [0, 1])
+def test_trace_ray_counts_and_lane_refill(gpu, orc, query):
+    """The stack-walk trace gives each wave a chunk of rays (64 to 512 by ray
+    count) and refills idle lanes from it: ragged counts around the wave and
+    chunk sizes, up to a count that takes 512-ray chunks, give hits
+    bit-identical to the one-ray-per-lane stack-free walk; at the smaller
+    counts the counters equal the oracle's stack walk, and a permutation of
+    the rays (other lanes, other refill order) permutes the hits and keeps
+    the counters."""
+    tris = scenes.random_triangles(60_000, 21, sigma=0.02)
+    tris[-5000:] = tris[:5000]  # equal-t ties
+    t = torch.from_numpy(tris).cuda()
+    b = gpu.BVHGPUBuilder()
+    b.build(t, tris.shape[0], 0, gpu.OnesweepSort())
+    nodes, root, _, _ = orc.build(tris)
+    tmax = None if query == 0 else 0.3
+    for n in (1, 63, 64, 65, 4097, 300_001, 1_000_003, 2_300_001):
+        print(f"trace chunking n={n} query={query}", flush=True)
+        rays = random_rays(n, 100 + n, tmax=tmax)
+        r = torch.from_numpy(rays).cuda()
+        cnt = torch.zeros(2, dtype=torch.int64, device="cuda")
+        gh = gpu.trace(b, t, r, query=query, counters=cnt)
+        sf = gpu.trace(b, t, r, query=query, walk="stackfree")
+        assert torch.equal(gh, sf), n
+        perm = torch.randperm(n, generator=torch.Generator().manual_seed(n)).cuda()
+        cnt2 = torch.zeros(2, dtype=torch.int64, device="cuda")
+        ph = gpu.trace(b, t, r[perm].contiguous(), query=query, counters=cnt2)
+        assert torch.equal(ph, gh[perm]), n
+        assert torch.equal(cnt2, cnt), n
+        if n <= 300_001:
+            _, oc = orc.trace(nodes, tris, root, rays, query=query, method=4)
+            assert np.array_equal(cnt.cpu().numpy().view(np.uint64), oc), n
+
+
+@pytest.mark.parametrize("query", [0, 1])
 @pytest.mark.parametrize("n", [2, 1000, 65536])
 def test_trace_escape_link_bit_exact(gpu, orc, n, query):
     """mlbvh_trace_escape_link against the oracle's escape-link walk: Hit
