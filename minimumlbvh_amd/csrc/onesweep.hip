@@ -285,7 +285,12 @@ __global__ void __launch_bounds__(kBlock)
 // stable order, written out of place.
 // ---------------------------------------------------------------------------
 constexpr int kFinishBlock = 256;
-constexpr int kFinishItems = 8;                        // positions per thread
+#ifndef MLBVH_FINISH_ITEMS
+#define MLBVH_FINISH_ITEMS 6
+#endif
+// keys per thread: at 2^25 u64 the finish took 132 us with 6, 137 with 4,
+// 146 with 8 (fewer blocks per CU), 152 with 3 and 171 with 2 (more halo)
+constexpr int kFinishItems = MLBVH_FINISH_ITEMS;
 constexpr int kFinishChunk = kFinishBlock * kFinishItems;
 constexpr int kFinishHalo = 64;                        // longest segment handled
 constexpr int kFinishNear = 8;                         // segments up to 9 keys take the fixed-count path
@@ -306,98 +311,139 @@ __device__ inline uint64_t order_word(K k, K lowMask, int pos)
 }
 
 // sortMask selects the key bits [startBit, startBit + 8 * passes) that order
-// keys.  A wave covers 64 consecutive positions per step, so the segment
-// bounds of most keys come from two ballots (segment starts, segment ends);
-// a segment that crosses the wave's 64 positions is walked in LDS.
+// keys.  Thread t loads window positions t + 256u; those in the middle chunk
+// are its keys to place.  The window is cut into rows of 64 consecutive
+// positions, one wave's lanes each, so the segment starts and ends within a
+// row are two ballots; each row posts its last start and first end, which
+// give the bounds [a, e) of a segment that crosses rows.  Every window
+// position gets the word (top << 52 | order_word) in LDS, where top is the
+// segment's start a when the segment holds at most kFinishNear + 1 keys and
+// kFinishFar otherwise.  A key of a short segment is ranked by counting,
+// among the kFinishNear + 1 words from a, those below its own: keys of later
+// segments have a later start or kFinishFar, so only the segment's own keys
+// that order before it are counted, one 64-bit comparison each.  A key of a
+// longer segment compares the words of [a, e).
+constexpr uint64_t kFinishFar = 0xFFFull;
 template <class K, bool kVals>
 __global__ void __launch_bounds__(kFinishBlock)
     k_segment_finish(const K* __restrict__ src, K* __restrict__ dst, const uint32_t* __restrict__ vsrc,
                      uint32_t* __restrict__ vdst, uint32_t n, int prefixShift, K sortMask, uint32_t* ctrl)
 {
     constexpr int kWin = kFinishChunk + 2 * kFinishHalo;
-    static_assert(kWin <= 4096, "window positions must fit the 12 low bits of order_word");
+    constexpr int kRows = kWin / 64;
+    constexpr int kWaves = kFinishBlock / 64;
+    static_assert(kWin < (int)kFinishFar, "window positions must fit the 12 low bits of order_word, below kFinishFar");
+    static_assert(kFinishHalo > kFinishNear, "the fixed-count reads stay inside the window");
+    static_assert(kFinishHalo % 64 == 0 && kFinishBlock % 64 == 0, "rows are all keys or all halo");
+    constexpr int kNone = 0x7FFFFFFF;
     // sort bits below the prefix (the prefix lies inside the key, below its top)
     const K lowMask = sortMask & (K)(((K)1 << prefixShift) - 1);
-    __shared__ K s_k[kWin];
     __shared__ uint32_t s_pf[kWin];
+    __shared__ uint64_t s_ow[kWin];
+    __shared__ int s_rowLast[kRows];   // the row's last segment start, or -1
+    __shared__ int s_rowFirst[kRows];  // one past the row's first segment end, or kNone
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t lo = (int64_t)blockIdx.x * kFinishChunk - kFinishHalo;  // window start (may be < 0)
     // all loads in flight before the first LDS store
     constexpr int kLoads = (kWin + kFinishBlock - 1) / kFinishBlock;
     K ld[kLoads];
+    uint32_t val[kVals ? kLoads : 1];
 #pragma unroll
     for (int u = 0; u < kLoads; u++) {
         const int i = t + u * kFinishBlock;
         const int64_t pos = lo + i;
-        ld[u] = (i < kWin && pos >= 0 && pos < (int64_t)n) ? src[pos] : (K)0;
-    }
-    uint32_t val[kVals ? kFinishItems : 1];
-    if (kVals) {
-#pragma unroll
-        for (int r = 0; r < kFinishItems; r++) {
-            const int64_t pos = lo + kFinishHalo + r * kFinishBlock + t;
-            val[r] = pos < (int64_t)n ? vsrc[pos] : 0u;
-        }
+        const bool load = i < kWin && pos >= 0 && pos < (int64_t)n;
+        ld[u] = load ? src[pos] : (K)0;
+        if (kVals) val[u] = load && i >= kFinishHalo && i < kFinishHalo + kFinishChunk ? vsrc[pos] : 0u;
     }
 #pragma unroll
     for (int u = 0; u < kLoads; u++) {
         const int i = t + u * kFinishBlock;
-        if (i < kWin) {
-            s_k[i] = ld[u];
-            s_pf[i] = prefix24(ld[u], prefixShift);
-        }
+        if (i < kWin) s_pf[i] = prefix24(ld[u], prefixShift);
     }
     __syncthreads();
     const int wLo = (int)((lo < 0 ? 0 : lo) - lo);                                   // first valid window index
     const int wHi = (int)((lo + kWin < (int64_t)n ? lo + kWin : (int64_t)n) - lo);   // one past the last
     const uint64_t atOrBelow = lane == 63 ? ~0ull : ((2ull << lane) - 1);
     const uint64_t atOrAbove = ~((1ull << lane) - 1);
-    bool overflow = false;
+    // segment starts and ends of each row (wave-uniform masks)
+    uint64_t startMask[kLoads], endMask[kLoads];
 #pragma unroll
-    for (int r = 0; r < kFinishItems; r++) {
-        const int w0 = kFinishHalo + r * kFinishBlock + wave * 64;  // window index of lane 0
-        const int w = w0 + lane;
-        const bool in = w < wHi;
+    for (int u = 0; u < kLoads; u++) {
+        const int r = u * kWaves + wave, w0 = 64 * r, w = w0 + lane;
+        startMask[u] = endMask[u] = 0;
+        if (r >= kRows) continue;
+        const bool valid = w >= wLo && w < wHi;
         const uint32_t pf = s_pf[w];
-        const bool starts = in && (w == wLo || s_pf[w - 1] != pf);
-        const bool ends = in && (w + 1 == wHi || s_pf[w + 1] != pf);
-        const uint64_t below = __ballot(starts) & atOrBelow;
-        const uint64_t above = __ballot(ends) & atOrAbove;
-        // bounds when the segment starts and ends inside this wave's 64 positions
-        const int aFast = below ? w0 + 63 - __clzll(below) : w0;
-        const int eFast = above ? w0 + __builtin_ctzll(above) + 1 : w0 + 64;
-        const K key = s_k[w];
+        startMask[u] = __ballot(valid && (w == wLo || s_pf[w - 1] != pf));
+        endMask[u] = __ballot(valid && (w + 1 == wHi || s_pf[w + 1] != pf));
+        if (lane == 0) {
+            s_rowLast[r] = startMask[u] ? w0 + 63 - __clzll(startMask[u]) : -1;
+            s_rowFirst[r] = endMask[u] ? w0 + __builtin_ctzll(endMask[u]) + 1 : kNone;
+        }
+    }
+    __syncthreads();
+    uint64_t mine[kLoads];
+    int segA[kLoads], segE[kLoads];
+#pragma unroll
+    for (int u = 0; u < kLoads; u++) {
+        const int r = u * kWaves + wave, w0 = 64 * r, w = w0 + lane;
+        mine[u] = 0;
+        segA[u] = segE[u] = 0;
+        if (r >= kRows || w < wLo || w >= wHi) continue;
+        const uint64_t below = startMask[u] & atOrBelow, above = endMask[u] & atOrAbove;
+        int a, e;
+        if (below) {
+            a = w0 + 63 - __clzll(below);
+        } else {  // the segment began in an earlier row
+            int k = r - 1;
+            while (k >= 0 && s_rowLast[k] < 0) k--;
+            a = k >= 0 ? s_rowLast[k] : wLo;
+        }
+        if (above) {
+            e = w0 + __builtin_ctzll(above) + 1;
+        } else {  // the segment ends in a later row
+            int k = r + 1;
+            while (k < kRows && s_rowFirst[k] == kNone) k++;
+            e = k < kRows ? s_rowFirst[k] : wHi;
+        }
+        const bool isShort = e - a <= kFinishNear + 1;
         // Inside a segment the prefix is common, so keys order by their sort
         // bits below the prefix, ties by window position: one comparison of
         // (low bits << 12 | position), window positions being < 4096.
-        const uint64_t mine = order_word(key, lowMask, w);
-        // short segment inside this wave's positions: a fixed count of
-        // comparisons from its first key (a uniform loop, no per-lane trip count)
-        int rank = 0;
+        mine[u] = ((uint64_t)(isShort ? a : (int)kFinishFar) << 52) | order_word(ld[u], lowMask, w);
+        s_ow[w] = mine[u];
+        segA[u] = a;
+        segE[u] = e;
+    }
+    // positions outside the array: words no key counts
 #pragma unroll
-        for (int j = 0; j <= kFinishNear; j++) {
-            const int idx = aFast + j;
-            const uint64_t other = order_word(s_k[idx < kWin ? idx : kWin - 1], lowMask, idx);
-            rank += (idx < eFast && other < mine) ? 1 : 0;
+    for (int u = 0; u < kLoads; u++) {
+        const int i = t + u * kFinishBlock;
+        if (i < kWin && (i < wLo || i >= wHi)) s_ow[i] = ~0ull;
+    }
+    __syncthreads();
+    bool overflow = false;
+#pragma unroll
+    for (int u = 0; u < kLoads; u++) {
+        const int w0 = u * kFinishBlock + (t - lane);
+        const int w = w0 + lane;
+        if (w0 < kFinishHalo || w0 >= kFinishHalo + kFinishChunk || w >= wHi) continue;
+        const int a = segA[u], e = segE[u];
+        // a segment reaching the window edge inside the array is longer than the halo
+        if ((a == wLo && lo + wLo > 0) || (e == wHi && lo + wHi < (int64_t)n)) {
+            overflow = true;
+            continue;
         }
-        if (!in) continue;
-        int a = aFast, e = eFast;
-        if (!below || !above || e - a > 1 + kFinishNear) {
-            // longer segment, or one crossing this wave's positions: walk and count
-            if (!below)
-                while (a > wLo && s_pf[a - 1] == pf) a--;
-            if (!above)
-                while (e < wHi && s_pf[e] == pf) e++;
-            // a segment reaching the window edge inside the array is longer than the halo
-            if ((a == wLo && lo + wLo > 0) || (e == wHi && lo + wHi < (int64_t)n)) {
-                overflow = true;
-                continue;
-            }
-            rank = 0;
-            for (int j = a; j < e; j++) rank += order_word(s_k[j], lowMask, j) < mine ? 1 : 0;
+        int rank = 0;
+        if (e - a <= kFinishNear + 1) {
+#pragma unroll
+            for (int j = 0; j <= kFinishNear; j++) rank += s_ow[a + j] < mine[u] ? 1 : 0;
+        } else {
+            for (int j = a; j < e; j++) rank += s_ow[j] < mine[u] ? 1 : 0;
         }
-        dst[lo + a + rank] = key;
-        if (kVals) vdst[lo + a + rank] = val[r];
+        dst[lo + a + rank] = ld[u];
+        if (kVals) vdst[lo + a + rank] = val[u];
     }
     if (overflow) __hip_atomic_store(&ctrl[kCtrlSortFallback], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

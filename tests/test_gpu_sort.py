@@ -133,15 +133,44 @@ def _prefix_keys(r, n, dtype, prefix_of, shift):
 
 
 @pytest.mark.parametrize("case", ["tiny_segments", "one_segment", "half_one_segment", "seg_2049",
-                                  "seg_3000", "seg_4097", "all_equal"])
+                                  "seg_3000", "seg_4097", "all_equal", "seg24_9", "seg24_12", "seg24_40",
+                                  "seg24_130", "seg24_mixed", "seg24_rows"])
 def test_hybrid_segment_shapes(gpu, orc, case):
-    """Large sorts finish each 16-bit-prefix segment in LDS; segments longer
-    than a block's look-ahead switch the whole sort to the full pass sequence.
-    Every shape must give the oracle's stable result bit for bit."""
+    """Large sorts finish each run of equal 24-bit prefix (a segment) in LDS:
+    segments of up to 9 keys by a fixed count of comparisons, longer ones
+    (also ones spanning rows of 64 positions with no segment start) by a walk;
+    segments longer than a block's look-ahead switch the whole sort to the
+    full pass sequence.  The seg_* cases group keys by their top 16 bits
+    (bits 40-47 random, so 24-bit segments of about length/256 keys); seg24_L
+    makes 24-bit segments of exactly L keys, seg24_mixed of 1 to 200, and
+    seg24_rows segments of 200, 200 and 112 keys that tile every 512 sorted
+    positions (no segment crosses a 512-aligned boundary, so the finish needs
+    no fallback, and whole rows of 64 positions hold no segment start), with
+    only 8 distinct low values per segment (many ties).  Every shape must give
+    the oracle's stable result bit for bit."""
     n = (1 << 20) + 17
     print(f"hybrid sort case {case} n={n}", flush=True)
-    r = np.random.default_rng(hash(case) % 2**32)
-    if case == "tiny_segments":
+    r = np.random.default_rng(sum(map(ord, case)))
+    if case == "seg24_rows":
+        lengths = np.resize(np.array([200, 200, 112]), 3 * (n // 512))
+        group = np.concatenate([np.repeat(np.arange(lengths.size), lengths),
+                                np.full(n - int(lengths.sum()), lengths.size)])
+        keys = (group.astype(np.uint64) << np.uint64(40)) | r.integers(0, 8, size=n, dtype=np.uint64)
+        keys = keys[r.permutation(n)]
+        pf = None
+    elif case.startswith("seg24_"):
+        if case == "seg24_mixed":
+            lengths = r.integers(1, 201, size=n // 50)
+            group = np.repeat(np.arange(lengths.size), lengths)[:n]
+            group = np.concatenate([group, np.arange(n - group.size) + lengths.size])
+        else:
+            group = np.arange(n) // int(case.split("_")[1])
+        pf24 = r.permutation(int(group.max()) + 1)[group].astype(np.uint64)
+        keys = (pf24 << np.uint64(40)) | (r.integers(0, 1 << 40, size=n, dtype=np.uint64))
+        keys = keys[r.permutation(n)]
+        keys[::7] = keys[3]  # ties keep input order
+        pf = None
+    elif case == "tiny_segments":
         pf = r.integers(0, 1 << 16, size=n)
     elif case == "one_segment":
         pf = np.full(n, 0x1234)
@@ -152,7 +181,9 @@ def test_hybrid_segment_shapes(gpu, orc, case):
     else:
         length = int(case.split("_")[1])
         pf = r.permutation(np.arange(n) // length)
-    if pf is None:
+    if case.startswith("seg24_"):
+        pass
+    elif pf is None:
         keys = np.full(n, 0x0123456789ABCDEF, np.uint64)
     else:
         keys = _prefix_keys(r, n, np.uint64, pf, 48)
