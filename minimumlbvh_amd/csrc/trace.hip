@@ -34,9 +34,10 @@ constexpr uint32_t kAoStack = 12;
 
 // Walks of the plain trace.  kWalkStack: the stack walk (traverse_stack /
 // stack_walk_step), the hits of intersect_stackfree with each internal node
-// fetched once per visit (the default; k_trace_ww); kWalkStackFree: the
-// parent-pointer walk itself (traverse); kWalkEscape: the escape-link walk
-// (traverse_escape).  The last two run one ray per lane in k_trace.
+// fetched once per visit (the default; k_trace_ww when each wave gets enough
+// rays, else k_trace); kWalkStackFree: the parent-pointer walk itself
+// (traverse); kWalkEscape: the escape-link walk (traverse_escape).  k_trace
+// runs one ray per lane.
 enum : int { kWalkStack = 0, kWalkStackFree = 1, kWalkEscape = 2 };
 
 __device__ inline void write_hit(HitRec* hits, uint32_t i, const Hit& h)
@@ -59,7 +60,8 @@ __global__ void __launch_bounds__(kTraceBlock)
     k_trace(const Node* __restrict__ nodes, TS tris, const uint32_t* __restrict__ rootPtr,
             const float4* __restrict__ rays, HitRec* __restrict__ hits, int nRays, unsigned long long* counters)
 {
-    static_assert(kWalk == kWalkStackFree || kWalk == kWalkEscape, "the stack walk runs in k_trace_ww");
+    __shared__ uint32_t s_stack[kWalk == kWalkStack ? kLaneStack : 1][kTraceBlock];
+    __shared__ float s_entry[kWalk == kWalkStack ? kLaneStack : 1][kTraceBlock];
     int i = blockIdx.x * kTraceBlock + threadIdx.x;
     uint32_t nFetch = 0, nTest = 0;
     if (i < nRays) {
@@ -72,6 +74,9 @@ __global__ void __launch_bounds__(kTraceBlock)
         h.tri = 0xFFFFFFFFu;
         if (kWalk == kWalkEscape)
             traverse_escape<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
+        else if (kWalk == kWalkStack)
+            traverse_stack<kAny, kCount, kTraceBlock>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest,
+                                                      &s_stack[0][threadIdx.x], &s_entry[0][threadIdx.x]);
         else
             traverse<kAny, kCount>(nodes, tris, *rootPtr, ro, rd, inv_rd(rd), h, nFetch, nTest);
         write_hit(hits, (uint32_t)i, h);
@@ -104,14 +109,17 @@ __global__ void k_queue_reset(uint32_t* counter) { *counter = 0; }
 #define MLBVH_TRACE_REFILL 8
 #endif
 constexpr uint32_t kTraceRefill = MLBVH_TRACE_REFILL;
-// Most rays per wave of the plain trace, whose grid gives each wave one chunk
-// (the hardware dispatcher balances the waves): 512 took 3.10 ms on the 4M
-// rays, 256 3.27, 1024 3.12, 2048 4.15.  Fewer rays fill too few waves at
-// 512 per wave; the chunk then shrinks (to 64, one ray per lane).
+// Rays per wave of the plain trace, whose grid gives each wave one chunk
+// (the hardware dispatcher balances the waves): 512 took 3.10 ms on 4M rays,
+// 256 3.27, 1024 3.12, 2048 4.15.  Below 16 waves per CU at 512 rays each
+// (2M rays on 256 CUs) one ray per lane is faster: on the level-9 sphere
+// 0.73 against 0.84 ms at 512K rays (128 per wave) and 1.23 against 1.27 ms
+// at 1M (256 per wave), while at 2M it took 1.98 against 1.81 ms.
 #ifndef MLBVH_PLAIN_CHUNK
 #define MLBVH_PLAIN_CHUNK 512
 #endif
 constexpr uint32_t kPlainChunk = MLBVH_PLAIN_CHUNK;
+
 
 template <bool kAny, bool kCount, bool kBatched, class TS>
 __global__ void __launch_bounds__(kTraceBlock)
@@ -858,15 +866,16 @@ static int trace_impl(mlbvh_ctx* ctx, const void* d_nodes, TS tris, const void* 
     const float4* rays = (const float4*)d_rays;
     HitRec* hits = (HitRec*)d_hits;
     const uint32_t n = (uint32_t)nRays;
-    // stack walk: one chunk per wave, kPlainChunk rays or fewer so that the
-    // grid still has 16 waves per CU; else one ray per lane
-    uint32_t chunk = n / (uint32_t)(ctx->cuCount * 16) / 64 * 64;
-    chunk = chunk < 64 ? 64 : chunk > kPlainChunk ? kPlainChunk : chunk;
-    const uint32_t perBlock = kWalk == kWalkStack ? chunk * (kTraceBlock / 64) : kTraceBlock;
+    // stack walk: one chunk of kPlainChunk rays per wave once that still
+    // makes 16 waves per CU; one ray per lane (k_trace) below that, and for
+    // the other walks
+    const uint32_t chunk = kPlainChunk;
+    const bool ww = kWalk == kWalkStack && (uint64_t)n >= (uint64_t)ctx->cuCount * 16u * chunk;
+    const uint32_t perBlock = ww ? chunk * (kTraceBlock / 64) : kTraceBlock;
     const unsigned g = (unsigned)(((uint64_t)n + perBlock - 1) / perBlock);
 #define LAUNCH(A, C, CNT)                                                                                        \
     {                                                                                                            \
-        if constexpr (kWalk == kWalkStack)                                                                       \
+        if (ww)                                                                                                  \
             k_trace_ww<A, C, false><<<g, kTraceBlock, 0, s>>>(nodes, tris, nullptr, root, 1, rays, hits, n,      \
                                                               chunk, nullptr, CNT);                              \
         else                                                                                                     \

@@ -87,9 +87,10 @@ def test_trace_on_gpu_built_bvh(gpu, orc):
 
 @pytest.mark.parametrize("query", [0, 1])
 def test_trace_ray_counts_and_lane_refill(gpu, orc, query):
-    """The stack-walk trace gives each wave a chunk of rays (64 to 512 by ray
-    count) and refills idle lanes from it: ragged counts around the wave and
-    chunk sizes, up to a count that takes 512-ray chunks, give hits
+    """The stack-walk trace runs one ray per lane below 16 waves of 512 rays
+    per CU and above that gives each wave 512 rays, refilling idle lanes from
+    them: ragged counts around the wave and block sizes and up to a count
+    that takes the 512-ray chunks (2.3M, ragged last chunk) give hits
     bit-identical to the one-ray-per-lane stack-free walk; at the smaller
     counts the counters equal the oracle's stack walk, and a permutation of
     the rays (other lanes, other refill order) permutes the hits and keeps
