@@ -160,6 +160,27 @@ def batch_scene(tri_counts, ray_counts, seed: int = 0, sigma: float = 0.01):
     return np.ascontiguousarray(tris), toff, rays, roff
 
 
+def furnace_box(albedo: float = 0.5, emission: float = 1.0):
+    """The closed cube [-1, 1]^3 as 12 triangles, every face with the same
+    reflectance and emission: seen from inside, every path hits a wall at
+    every depth, so a path tracer's radiance has a closed form.  Returns
+    (tris (12, 3, 3), attribs (12, 15): shading normals, reflectance,
+    emissive)."""
+    quads = [((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1)), ((-1, -1, 1), (-1, 1, 1), (1, 1, 1), (1, -1, 1)),
+             ((-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (-1, -1, 1)), ((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1)),
+             ((-1, -1, -1), (-1, -1, 1), (1, -1, 1), (1, -1, -1)), ((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1))]
+    tris = []
+    for a, b, c, d in quads:
+        tris += [(a, b, c), (a, c, d)]
+    tris = np.asarray(tris, np.float32)
+    attribs = np.zeros((tris.shape[0], 15), np.float32)
+    attribs[:, 9:12] = albedo
+    attribs[:, 12:15] = emission
+    return tris, attribs
+
+
+# Camera inside furnace_box, off the centre
+CAMERA_FURNACE = ((0.1, 0.2, 0.3), (0.1, 0.2, -1.0), (0.0, 1.0, 0.0), 90.0)
 # Cameras of the workloads: (eye, center, up, fovy_degrees)
 CAMERA_SMALL = ((0.5, 0.5, -1.5), (0.5, 0.5, -0.5), (0.0, 1.0, 0.0), 60.0)
 CAMERA_SPHERE = ((0.0, 0.2, -1.8), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0)

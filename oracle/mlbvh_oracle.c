@@ -27,10 +27,13 @@
  *                          stackfree == stackfull == escape-link
  *   sobol / camera ....... golden vectors from the reference's own sobol.h and
  *                          camera.h compiled with g++ (oracle/_ref)
- *   AO / PT shading ...... per-step closed forms (ONB orthonormality, cosine
- *                          hemisphere unit length, packRGBA rounding) and the
- *                          AO/normal images against traced hits; PT is
- *                          "parity unpinned" beyond those pieces.
+ *   AO / PT shading ...... per-step closed forms (ONB orthonormality, packRGBA
+ *                          rounding), the AO/normal images against traced
+ *                          hits; PCG32 by the published pcg32-demo outputs
+ *                          (seed 42, sequence 54); the cosine hemisphere by
+ *                          closed-form points and its moments; the PT loop
+ *                          by a closed furnace (every path's radiance is
+ *                          1, 1.5, 1.75 at depth 1-3, 1.75 or 2 at depth 4).
  */
 #include "mlbvh_oracle.h"
 
@@ -909,6 +912,28 @@ static float pcg_uniformf(orc_pcg* p)
     float f;
     memcpy(&f, &bits, 4);
     return f - 1.0f;
+}
+
+void orc_pcg32(uint64_t seed, uint64_t seq, int n, uint32_t* out)
+{
+    orc_pcg p;
+    pcg_init(&p, seed, seq);
+    for (int i = 0; i < n; i++) out[i] = pcg_next(&p);
+}
+
+void orc_pcg32_uniformf(uint64_t seed, uint64_t seq, int n, float* out)
+{
+    orc_pcg p;
+    pcg_init(&p, seed, seq);
+    for (int i = 0; i < n; i++) out[i] = pcg_uniformf(&p);
+}
+
+void orc_hemisphere_cos(float xi0, float xi1, float* out)
+{
+    orc_v3 v = hemisphere_cos(xi0, xi1);
+    out[0] = v.x;
+    out[1] = v.y;
+    out[2] = v.z;
 }
 
 void orc_render_normal(uint32_t* pixels, int w, int h, const orc_raygen* g, uint32_t root, const orc_node* nodes,

@@ -109,6 +109,11 @@ void orc_brute_force(const orc_tri* tris, int n, const float* rays, int nRays, o
 
 /* sampling + camera, sobol.h:18-129, camera.h:10-31, minimum_lbvh.h:904-909 */
 uint32_t orc_hash_pcg(uint32_t v);
+/* PCG32 of main_gpu.cu:51-82: the first n outputs, and uniformf of them */
+void orc_pcg32(uint64_t seed, uint64_t seq, int n, uint32_t* out);
+void orc_pcg32_uniformf(uint64_t seed, uint64_t seq, int n, float* out);
+/* cosine-weighted hemisphere direction of main_gpu.cu:24-36 (y = up) */
+void orc_hemisphere_cos(float xi0, float xi1, float* out);
 void orc_sobol_2d(float* x, float* y, uint32_t index, uint32_t p0, uint32_t p1, uint32_t p2);
 void orc_raygen_lookat(orc_raygen* g, orc_v3 eye, orc_v3 center, orc_v3 up, float fovy, int w, int h);
 void orc_raygen_shoot(const orc_raygen* g, orc_v3* ro, orc_v3* rd, float u, float v);

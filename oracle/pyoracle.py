@@ -69,6 +69,9 @@ def lib():
             "orc_trace": ([P, P, c_uint32, P, c_int, c_int, c_int, P, P, c_int], None),
             "orc_brute_force": ([P, c_int, P, c_int, P], None),
             "orc_hash_pcg": ([c_uint32], c_uint32),
+            "orc_pcg32": ([c_uint64, c_uint64, c_int, P], None),
+            "orc_pcg32_uniformf": ([c_uint64, c_uint64, c_int, P], None),
+            "orc_hemisphere_cos": ([c_float, c_float, P], None),
             "orc_sobol_2d": ([POINTER(c_float), POINTER(c_float), c_uint32, c_uint32, c_uint32, c_uint32], None),
             "orc_raygen_lookat": ([P, V3, V3, V3, c_float, c_int, c_int], None),
             "orc_raygen_shoot": ([P, P, P, c_float, c_float], None),

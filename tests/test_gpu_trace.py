@@ -271,6 +271,38 @@ def test_pt_and_pack_match_oracle_statistically(gpu, orc):
     assert d.max() <= 1
 
 
+@pytest.mark.parametrize("sobol", [1, 0])
+def test_pt_furnace_closed_form(gpu, orc, sobol):
+    """Path tracing inside a closed cube whose walls all emit 1 and reflect
+    0.5: every path's radiance is exactly 0, 1, 1.5, 1.75 at max depth
+    0 .. 3 and 1.75 or 2.0 (Russian roulette at p = 0.5) at depth 4,
+    whatever direction a bounce takes, so the accumulators equal the
+    oracle's bit for bit (the roulette draws are the same Sobol / PCG
+    numbers)."""
+    tris, attribs = scenes.furnace_box()
+    nodes, root, _, _ = orc.build(tris)
+    eye, ctr, up, fov = scenes.CAMERA_FURNACE
+    w, h, spp = 48, 40, 3
+    og = orc.raygen_lookat(eye, ctr, up, math.radians(fov), w, h)
+    g = gpu.RayGenerator().lookat(eye, ctr, up, math.radians(fov), w, h)
+    nt, rt = upload_bvh(nodes, root)
+    tt = torch.from_numpy(tris).cuda()
+    at = torch.from_numpy(attribs).cuda()
+    for depth, exact in ((0, 0.0), (1, 1.0), (2, 1.5), (3, 1.75), (4, None)):
+        print(f"pt furnace sobol={sobol} depth={depth}", flush=True)
+        acc = np.zeros((w * h, 4), np.float32)
+        gacc = torch.zeros((w * h, 4), dtype=torch.float32, device="cuda")
+        for _ in range(spp):
+            acc = orc.render_pt(acc, og, w, h, root, nodes, tris, attribs, sobol, 1024, depth)
+            gpu.render_pt(gacc, w, h, g, rt, nt, tt, at, sobol, 1024, depth)
+        ga = gacc.cpu().numpy()
+        assert np.array_equal(ga, acc), depth
+        if exact is not None:
+            assert np.all(ga[:, :3] == np.float32(exact * spp)), depth
+        else:
+            assert 0.45 * spp < ((ga[:, 0] - 1.75 * spp) / 0.25).mean() < 0.55 * spp
+
+
 def test_pt_full_size_scene(gpu, orc):
     """The path-tracing configuration at full size: 64 displaced level-7
     spheres + ground + light (20,971,524 triangles), 1920x1080, 8 spp, max
