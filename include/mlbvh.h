@@ -185,9 +185,11 @@ MLBVH_API int mlbvh_build_batched(mlbvh_ctx* ctx, const void* d_triangles, const
                                   void* d_nodes, uint32_t* d_roots, void* stream);
 
 /* Persistent work-queue trace over a batched build: each Ray names its mesh
- * in its pad word (u32); hits hold mesh-local triangle indices.  Same
- * traversal as mlbvh_trace (minimum_lbvh.h:981-1061).  d_counters as in
- * mlbvh_trace. */
+ * in its pad word (u32); hits hold mesh-local triangle indices, and a ray
+ * naming no mesh of the batch gets a miss record (t = its tmax, triangle
+ * 0xFFFFFFFF).  Same traversal as mlbvh_trace (minimum_lbvh.h:981-1061).
+ * d_counters as in mlbvh_trace.  The work counter lives in ctx: calls on one
+ * ctx must not overlap. */
 MLBVH_API int mlbvh_trace_batched(mlbvh_ctx* ctx, const void* d_nodes, const void* d_triangles,
                                   const uint32_t* d_triOffsets, const uint32_t* d_roots, int nMeshes,
                                   const float* d_rays, void* d_hits, int nRays, int rayQuery,

@@ -10,9 +10,10 @@
 //     key = meshId << (32 + L) | morton << L | localIndex      (L = index bits)
 // over bits [L, L + 32 + meshBits), so every mesh's records land, sorted, in
 // its own segment [off[m], off[m+1]).  The climb then runs in mesh-local
-// coordinates.  Tracing uses a persistent kernel whose waves pull 64-ray
-// chunks from an atomic work counter; each ray names its mesh in the Ray's
-// pad word.
+// coordinates.  Tracing uses a persistent kernel (k_trace_ww in trace.hip)
+// whose waves pull chunks of rays from an atomic work counter and hand the
+// next ray of the chunk to each lane whose walk has ended; each ray names its
+// mesh in the Ray's pad word.
 #include "mlbvh_ctx.h"
 
 namespace mlbvh {
