@@ -37,6 +37,9 @@ struct SortCfg {
 constexpr SortCfg kSortCfgs[kNumSortCfgs] = {{256, 16}, {256, 32}, {512, 16}, {512, 24}, {1024, 16}};
 constexpr int kHistBlock = 256;
 
+#ifndef MLBVH_PRESCAN
+#define MLBVH_PRESCAN 1
+#endif
 constexpr uint64_t kFlagA = 1;  // tile aggregate published
 constexpr uint64_t kFlagP = 2;  // inclusive prefix published
 
@@ -76,16 +79,55 @@ __global__ void k_sort_init(uint32_t* ctrl, int nPass, int nHist, uint32_t* hist
     sort_state_init(ctrl, nPass, hist, nHist, status, statusWords);
 }
 
+// With tileCounts, blocks take whole sort tiles of tileKeys keys (tile-
+// strided) and also write each tile's counts of the first digit,
+// tileCounts[digit * nTiles + tile], for k_tile_scan.
 template <class K>
 __global__ void __launch_bounds__(kHistBlock) k_hist(const K* __restrict__ keys, uint32_t n, int startBit,
                                                      int nPass, uint32_t* __restrict__ partials,
-                                                     const uint32_t* __restrict__ guard = nullptr)
+                                                     const uint32_t* __restrict__ guard = nullptr,
+                                                     uint32_t tileKeys = 0, uint32_t nTiles = 0,
+                                                     uint32_t* __restrict__ tileCounts = nullptr)
 {
     if (guard && *guard == 0) return;
     __shared__ uint32_t s_h[8][256];
     for (int p = 0; p < 8; p++) s_h[p][threadIdx.x] = 0;
-    __syncthreads();
     const int keyBits = 8 * (int)sizeof(K);
+    if (tileCounts) {
+        __shared__ uint32_t s_t[256];  // this tile's counts of the first digit
+        s_t[threadIdx.x] = 0;
+        __syncthreads();
+        auto count = [&](K k) {
+            atomicAdd(&s_t[startBit < keyBits ? digit_of(k, startBit) : 0u], 1u);
+            for (int p = 1; p < nPass; p++) {
+                int sh = startBit + 8 * p;
+                atomicAdd(&s_h[p][sh < keyBits ? digit_of(k, sh) : 0u], 1u);
+            }
+        };
+        for (uint32_t tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+            const uint64_t t0 = (uint64_t)tile * tileKeys;
+            const uint64_t t1 = t0 + tileKeys < (uint64_t)n ? t0 + tileKeys : (uint64_t)n;
+            uint64_t i = t0 + threadIdx.x;
+            for (; i + 3 * kHistBlock < t1; i += 4 * kHistBlock) {
+                K k[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) k[u] = keys[i + u * kHistBlock];
+#pragma unroll
+                for (int u = 0; u < 4; u++) count(k[u]);
+            }
+            for (; i < t1; i += kHistBlock) count(keys[i]);
+            __syncthreads();
+            const uint32_t c = s_t[threadIdx.x];
+            tileCounts[(size_t)threadIdx.x * nTiles + tile] = c;
+            s_h[0][threadIdx.x] += c;
+            s_t[threadIdx.x] = 0;
+            __syncthreads();
+        }
+        for (int p = 0; p < nPass; p++)
+            partials[((size_t)blockIdx.x * nPass + p) * 256 + threadIdx.x] = s_h[p][threadIdx.x];
+        return;
+    }
+    __syncthreads();
     const uint32_t stride = gridDim.x * kHistBlock;
     uint32_t i = blockIdx.x * kHistBlock + threadIdx.x;
     for (; i + 3 * stride < n; i += 4 * stride) {
@@ -127,11 +169,37 @@ __global__ void __launch_bounds__(256) k_hist_reduce(const uint32_t* __restrict_
     if (s) atomicAdd(&hist[p * 256 + threadIdx.x], s);
 }
 
+// The first digit pass's inclusive prefixes filled in advance from per-tile
+// counts: status[d * nTiles + tile] = (that pass's tag, prefix flag, count
+// of digit d in tiles 0..tile).  Digit-major, so that each lane writes whole
+// lines; the other passes' look-back words ([tile][digit]) carry other tags,
+// so sharing the table is safe.  One block per digit.
+__global__ void __launch_bounds__(256) k_tile_scan(const uint32_t* __restrict__ tileCounts, uint32_t nTiles,
+                                                   uint64_t* __restrict__ status, const uint32_t* __restrict__ ctrl,
+                                                   int pass)
+{
+    __shared__ uint32_t s_tmp[4];
+    const uint32_t d = blockIdx.x;
+    const uint32_t epoch = ctrl[kCtrlEpochs + pass];
+    const uint32_t per = (nTiles + 255) / 256;
+    const uint32_t a = threadIdx.x * per < nTiles ? threadIdx.x * per : nTiles;
+    const uint32_t b = a + per < nTiles ? a + per : nTiles;
+    const uint32_t* col = tileCounts + (size_t)d * nTiles;
+    uint32_t sum = 0;
+    for (uint32_t j = a; j < b; j++) sum += col[j];
+    uint32_t run = scan256(sum, s_tmp);
+    uint64_t* out = status + (size_t)d * nTiles;
+    for (uint32_t j = a; j < b; j++) {
+        run += col[j];
+        out[j] = lb_pack(epoch, kFlagP, run);
+    }
+}
+
 template <class K, bool kVals, int kBlock, int kItems>
 __global__ void __launch_bounds__(kBlock)
     k_pass(const K* __restrict__ kin, K* __restrict__ kout, const uint32_t* __restrict__ vin,
            uint32_t* __restrict__ vout, uint32_t n, int shift, const uint32_t* __restrict__ passHist,
-           uint64_t* status, uint32_t* ctrl, int pass, const uint32_t* __restrict__ guard)
+           uint64_t* status, uint32_t* ctrl, int pass, const uint32_t* __restrict__ guard, int preScanned)
 {
     if (guard && *guard == 0) return;  // a fallback pass that is not needed
     constexpr int kTile = kBlock * kItems;
@@ -212,7 +280,9 @@ __global__ void __launch_bounds__(kBlock)
         // Decoupled look-back for digit t.
         uint64_t* my = status + (uint64_t)tile * 256 + t;
         uint32_t excl = 0;
-        if (tile == 0) {
+        if (preScanned) {  // k_tile_scan has published every tile's inclusive prefix
+            if (tile != 0) excl = (uint32_t)status[(uint64_t)t * gridDim.x + tile - 1];
+        } else if (tile == 0) {
             __hip_atomic_store(my, lb_pack(epoch, kFlagP, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_store(my, lb_pack(epoch, kFlagA, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -498,22 +568,23 @@ int ensure_partials(mlbvh_ctx* ctx, int blocks, int nPass)
 template <class K, bool kVals, int B, int I>
 static void launch_pass(hipStream_t s, uint32_t tiles, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
                         uint32_t n, int shift, const uint32_t* hist, uint64_t* status, uint32_t* ctrl, int p,
-                        const uint32_t* guard)
+                        const uint32_t* guard, int preScanned)
 {
-    k_pass<K, kVals, B, I><<<tiles, B, 0, s>>>(src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard);
+    k_pass<K, kVals, B, I><<<tiles, B, 0, s>>>(src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard,
+                                               preScanned);
 }
 
 template <class K, bool kVals>
 static void dispatch_pass(int cfg, hipStream_t s, uint32_t tiles, const K* src, K* dst, const uint32_t* vsrc,
                           uint32_t* vdst, uint32_t n, int shift, const uint32_t* hist, uint64_t* status,
-                          uint32_t* ctrl, int p, const uint32_t* guard = nullptr)
+                          uint32_t* ctrl, int p, const uint32_t* guard = nullptr, int preScanned = 0)
 {
     switch (cfg) {
-    case 0: launch_pass<K, kVals, 256, 16>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard); break;
-    case 1: launch_pass<K, kVals, 256, 32>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard); break;
-    case 2: launch_pass<K, kVals, 512, 16>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard); break;
-    case 3: launch_pass<K, kVals, 512, 24>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard); break;
-    default: launch_pass<K, kVals, 1024, 16>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard); break;
+    case 0: launch_pass<K, kVals, 256, 16>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard, preScanned); break;
+    case 1: launch_pass<K, kVals, 256, 32>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard, preScanned); break;
+    case 2: launch_pass<K, kVals, 512, 16>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard, preScanned); break;
+    case 3: launch_pass<K, kVals, 512, 24>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard, preScanned); break;
+    default: launch_pass<K, kVals, 1024, 16>(s, tiles, src, dst, vsrc, vdst, n, shift, hist, status, ctrl, p, guard, preScanned); break;
     }
 }
 
@@ -527,24 +598,40 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
     int rc;
     const int keyBits = 8 * (int)sizeof(K);
     const int hg = hist_grid(ctx, n);
-    if ((rc = ensure_partials(ctx, hg, live))) return rc;
+    // partials: hg blocks x 3 histograms, then the first pass's per-tile
+    // counts (the fallback's histograms later reuse the space)
+    const size_t partialRows = (size_t)hg * 3 + tiles > (size_t)hg * live ? (size_t)hg * 3 + tiles : (size_t)hg * live;
+    if ((rc = ensure_partials(ctx, (int)partialRows, 1))) return rc;
+    uint32_t* tileCounts = ctx->partials + (size_t)hg * 3 * 256;
     // pass slots: 0..2 = the three top digits; 3 .. 3+live-1 = the fallback sequence
     k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, 3 + live, live, ctx->hist, ctx->status, ctx->statusWords);
-    // the three top digits' histograms now; the rest only if the fallback runs
+    // the three top digits' histograms now (the rest only if the fallback
+    // runs), with the first pass's per-tile counts: its look-back table is
+    // then filled in advance and its tiles never wait on each other
+#if MLBVH_PRESCAN
+    k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - 3), 3, ctx->partials, nullptr,
+                                             (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
+    launch_hist_reduce(ctx, hg, 3, stream, live - 3);
+    k_tile_scan<<<256, 256, 0, stream>>>(tileCounts, tiles, ctx->status, ctx->ctrl, 0);
+#else
+    (void)tileCounts;
     k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - 3), 3, ctx->partials);
     launch_hist_reduce(ctx, hg, 3, stream, live - 3);
+#endif
     const uint32_t* guard = ctx->ctrl + kCtrlSortFallback;
     K* bufs[2] = {keys, ktmp};
     uint32_t* vbufs[2] = {vals, vtmp};
     for (int q = 0; q < 3; q++) {
         const int p = live - 3 + q;
         const int shift = startBit + 8 * p;
+        const int pre = MLBVH_PRESCAN && q == 0;
         if (vals)
             dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, bufs[q & 1], bufs[(q + 1) & 1], vbufs[q & 1],
-                                   vbufs[(q + 1) & 1], n, shift, ctx->hist + 256 * p, ctx->status, ctx->ctrl, q);
+                                   vbufs[(q + 1) & 1], n, shift, ctx->hist + 256 * p, ctx->status, ctx->ctrl, q,
+                                   nullptr, pre);
         else
             dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, bufs[q & 1], bufs[(q + 1) & 1], nullptr, nullptr, n,
-                                    shift, ctx->hist + 256 * p, ctx->status, ctx->ctrl, q);
+                                    shift, ctx->hist + 256 * p, ctx->status, ctx->ctrl, q, nullptr, pre);
     }
     // sort key = bits [startBit, min(startBit + 8 * live, keyBits))
     const int topBit = startBit + 8 * live < keyBits ? startBit + 8 * live : keyBits;
