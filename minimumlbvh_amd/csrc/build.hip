@@ -98,21 +98,25 @@ __global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(TS tris, int n, uint
 
 // Morton record per triangle (minimum_lbvh.cu:68-92 / minimum_lbvh.h:534-553),
 // the 4 digit histograms of the 32-bit key, and the climb's arrival words.
-// Grid-stride over a few blocks per CU so that the histograms are merged into
-// global memory by few blocks (global atomics on 1024 shared counters are the
-// cost otherwise).
+// A few blocks per CU, so that the histograms are merged into global memory
+// by few blocks (global atomics on 1024 shared counters are the cost
+// otherwise); each block takes whole sort tiles (tileKeys triangles,
+// tile-strided) and writes every tile's counts of the first digit,
+// tileCounts[digit * nTiles + tile], so the record sort's first pass can skip
+// its look-back.
 template <class TS>
 __global__ void __launch_bounds__(kBuildBlock)
     k_morton(TS tris, int n, const uint32_t* __restrict__ sceneBox, uint32_t opt,
              uint64_t* __restrict__ rec, uint32_t* __restrict__ partials, uint32_t* __restrict__ flags,
-             uint32_t* __restrict__ ctrl)
+             uint32_t* __restrict__ ctrl, uint32_t tileKeys, uint32_t nTiles, uint32_t* __restrict__ tileCounts)
 {
     __shared__ uint32_t s_h[4][256];
+    __shared__ uint32_t s_t[256];  // this tile's counts of the first digit
     for (int p = 0; p < 4; p++) s_h[p][threadIdx.x] = 0;
+    s_t[threadIdx.x] = 0;
     f3 lo, hi;
     fold_scene_box(sceneBox, &lo, &hi);  // its barrier also orders the clears above
-    for (int i = blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += gridDim.x * kBuildBlock) {
-        Tri tri = tris[i];
+    auto record = [&](const Tri& tri, uint32_t i) {
         f3 center = ((tri.v[0] + tri.v[1]) + tri.v[2]) / 3.0f;
         uint32_t morton;
         if (opt & MLBVH_BUILD_OPTION_USE_NORMAL) {
@@ -123,13 +127,32 @@ __global__ void __launch_bounds__(kBuildBlock)
         } else {
             morton = (uint32_t)(morton_in_box(lo, hi, center) >> 31);
         }
-        rec[i] = (uint64_t)morton | ((uint64_t)(uint32_t)i << 32);
-        if (i < n - 1) flags[i] = 0xFFFFFFFFu;
+        rec[i] = (uint64_t)morton | ((uint64_t)i << 32);
+        if (i < (uint32_t)n - 1) flags[i] = 0xFFFFFFFFu;
         if (i == 0) ctrl[kCtrlFrontier] = 0;
+        atomicAdd(&s_t[morton & 0xFFu], 1u);
 #pragma unroll
-        for (int p = 0; p < 4; p++) atomicAdd(&s_h[p][(morton >> (8 * p)) & 0xFFu], 1u);
+        for (int p = 1; p < 4; p++) atomicAdd(&s_h[p][(morton >> (8 * p)) & 0xFFu], 1u);
+    };
+    for (uint32_t tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+        const uint32_t t0 = tile * tileKeys;
+        const uint32_t t1 = (uint32_t)n - t0 < tileKeys ? (uint32_t)n : t0 + tileKeys;
+        uint32_t i = t0 + threadIdx.x;
+        for (; i + 3 * kBuildBlock < t1; i += 4 * kBuildBlock) {
+            Tri tr[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) tr[u] = tris[i + u * kBuildBlock];
+#pragma unroll
+            for (int u = 0; u < 4; u++) record(tr[u], i + u * kBuildBlock);
+        }
+        for (; i < t1; i += kBuildBlock) record(tris[i], i);
+        __syncthreads();
+        const uint32_t c = s_t[threadIdx.x];
+        tileCounts[(size_t)threadIdx.x * nTiles + tile] = c;
+        s_h[0][threadIdx.x] += c;
+        s_t[threadIdx.x] = 0;
+        __syncthreads();
     }
-    __syncthreads();
 #pragma unroll
     for (int p = 0; p < 4; p++) partials[((size_t)blockIdx.x * 4 + p) * 256 + threadIdx.x] = s_h[p][threadIdx.x];
 }
@@ -452,11 +475,15 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         if (rc) return rc;
         k_deltas64<<<g, kBuildBlock, 0, stream>>>(ctx->recB, ctx->idxA, n, ctx->deltas, ctx->recA);
     } else {
-        if ((rc = ensure_partials(ctx, (int)gMorton, 4))) return rc;
-        k_morton<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials,
-                                                      ctx->flags, ctx->ctrl);
-        launch_hist_reduce(ctx, (int)gMorton, 4, stream);
-        rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream);
+        const uint32_t tiles = sort_tiles(ctx, (uint32_t)n);
+        const unsigned gm = gMorton < tiles ? gMorton : tiles;
+        // partials: gm blocks x 4 histograms, then the per-tile counts
+        if ((rc = ensure_partials(ctx, (int)(gm * 4 + tiles), 1))) return rc;
+        uint32_t* tileCounts = ctx->partials + (size_t)gm * 4 * 256;
+        k_morton<<<gm, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials, ctx->flags,
+                                                 ctx->ctrl, (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
+        launch_hist_reduce(ctx, (int)gm, 4, stream);
+        rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts);
         if (rc) return rc;
         k_deltas<<<g, kBuildBlock, 0, stream>>>(ctx->recA, n, ctx->deltas);
     }

@@ -679,7 +679,7 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
 
 template <class K>
 static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, uint32_t n, int startBit,
-                     int endBit, bool histReady, hipStream_t stream)
+                     int endBit, bool histReady, hipStream_t stream, const uint32_t* readyTileCounts = nullptr)
 {
     const int keyBits = 8 * (int)sizeof(K);
     if (startBit < 0 || endBit <= startBit) {
@@ -707,25 +707,37 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
     if (!histReady && n >= kHybridMinKeys && live >= 4) return sort_hybrid(ctx, keys, vals, ktmp, vtmp, n, startBit, live,
                                                                              tiles, stream);
 
+    // per-tile counts of the first digit (from the caller's histogram kernel
+    // or from k_hist here) let the first pass skip its look-back
+    const uint32_t* tileCounts = MLBVH_PRESCAN ? readyTileCounts : nullptr;
     if (!histReady) {
         int hg = hist_grid(ctx, n);
-        if ((rc = ensure_partials(ctx, hg, live))) return rc;
+        if ((rc = ensure_partials(ctx, hg * live + (MLBVH_PRESCAN ? (int)tiles : 0), 1))) return rc;
         k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, live, live, ctx->hist, ctx->status, ctx->statusWords);
+#if MLBVH_PRESCAN
+        uint32_t* counts = ctx->partials + (size_t)hg * live * 256;
+        k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials, nullptr,
+                                                 (uint32_t)sort_tile_keys(ctx), tiles, counts);
+        tileCounts = counts;
+#else
         k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials);
+#endif
         launch_hist_reduce(ctx, hg, live, stream);
     }
+    if (tileCounts) k_tile_scan<<<256, 256, 0, stream>>>(tileCounts, tiles, ctx->status, ctx->ctrl, 0);
     K* src = keys;
     K* dst = ktmp;
     uint32_t* vsrc = vals;
     uint32_t* vdst = vtmp;
     for (int p = 0; p < live; p++) {
         int shift = startBit + 8 * p;
+        const int pre = p == 0 && tileCounts != nullptr;
         if (vals)
             dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, src, dst, vsrc, vdst, n, shift,
-                                   ctx->hist + 256 * p, ctx->status, ctx->ctrl, p);
+                                   ctx->hist + 256 * p, ctx->status, ctx->ctrl, p, nullptr, pre);
         else
             dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, src, dst, nullptr, nullptr, n, shift,
-                                    ctx->hist + 256 * p, ctx->status, ctx->ctrl, p);
+                                    ctx->hist + 256 * p, ctx->status, ctx->ctrl, p, nullptr, pre);
         K* tk = src;
         src = dst;
         dst = tk;
@@ -747,9 +759,9 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
 }
 
 int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, int startBit, int endBit,
-                     bool histReady, hipStream_t stream)
+                     bool histReady, hipStream_t stream, const uint32_t* tileCounts)
 {
-    return sort_impl<uint64_t>(ctx, keys, nullptr, tmp, nullptr, n, startBit, endBit, histReady, stream);
+    return sort_impl<uint64_t>(ctx, keys, nullptr, tmp, nullptr, n, startBit, endBit, histReady, stream, tileCounts);
 }
 
 int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,
