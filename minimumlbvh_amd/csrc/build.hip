@@ -104,16 +104,24 @@ __global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(TS tris, int n, uint
 // tile-strided) and writes every tile's counts of the first digit,
 // tileCounts[digit * nTiles + tile], so the record sort's first pass can skip
 // its look-back.
+constexpr int kMortonBlock = 1024;  // one block per sort tile still fills the CUs
+#ifndef MLBVH_MORTON_UNROLL
+#define MLBVH_MORTON_UNROLL 2
+#endif
+// triangles loaded at once per thread: 2 took 57 us at 5.24M triangles, 1
+// 59 us, 4 63 us (76 VGPRs, one block per CU)
+constexpr int kMortonUnroll = MLBVH_MORTON_UNROLL;
 template <class TS>
-__global__ void __launch_bounds__(kBuildBlock)
+__global__ void __launch_bounds__(kMortonBlock)
     k_morton(TS tris, int n, const uint32_t* __restrict__ sceneBox, uint32_t opt,
              uint64_t* __restrict__ rec, uint32_t* __restrict__ partials, uint32_t* __restrict__ flags,
              uint32_t* __restrict__ ctrl, uint32_t tileKeys, uint32_t nTiles, uint32_t* __restrict__ tileCounts)
 {
     __shared__ uint32_t s_h[4][256];
     __shared__ uint32_t s_t[256];  // this tile's counts of the first digit
-    for (int p = 0; p < 4; p++) s_h[p][threadIdx.x] = 0;
-    s_t[threadIdx.x] = 0;
+    const int t = threadIdx.x;
+    for (int i = t; i < 4 * 256; i += kMortonBlock) (&s_h[0][0])[i] = 0;
+    if (t < 256) s_t[t] = 0;
     f3 lo, hi;
     fold_scene_box(sceneBox, &lo, &hi);  // its barrier also orders the clears above
     auto record = [&](const Tri& tri, uint32_t i) {
@@ -137,24 +145,27 @@ __global__ void __launch_bounds__(kBuildBlock)
     for (uint32_t tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
         const uint32_t t0 = tile * tileKeys;
         const uint32_t t1 = (uint32_t)n - t0 < tileKeys ? (uint32_t)n : t0 + tileKeys;
-        uint32_t i = t0 + threadIdx.x;
-        for (; i + 3 * kBuildBlock < t1; i += 4 * kBuildBlock) {
-            Tri tr[4];
+        uint32_t i = t0 + (uint32_t)t;
+        for (; i + (kMortonUnroll - 1) * kMortonBlock < t1; i += kMortonUnroll * kMortonBlock) {
+            Tri tr[kMortonUnroll];
 #pragma unroll
-            for (int u = 0; u < 4; u++) tr[u] = tris[i + u * kBuildBlock];
+            for (int u = 0; u < kMortonUnroll; u++) tr[u] = tris[i + u * kMortonBlock];
 #pragma unroll
-            for (int u = 0; u < 4; u++) record(tr[u], i + u * kBuildBlock);
+            for (int u = 0; u < kMortonUnroll; u++) record(tr[u], i + u * kMortonBlock);
         }
-        for (; i < t1; i += kBuildBlock) record(tris[i], i);
+        for (; i < t1; i += kMortonBlock) record(tris[i], i);
         __syncthreads();
-        const uint32_t c = s_t[threadIdx.x];
-        tileCounts[(size_t)threadIdx.x * nTiles + tile] = c;
-        s_h[0][threadIdx.x] += c;
-        s_t[threadIdx.x] = 0;
+        if (t < 256) {
+            const uint32_t c = s_t[t];
+            tileCounts[(size_t)t * nTiles + tile] = c;
+            s_h[0][t] += c;
+            s_t[t] = 0;
+        }
         __syncthreads();
     }
+    if (t < 256)
 #pragma unroll
-    for (int p = 0; p < 4; p++) partials[((size_t)blockIdx.x * 4 + p) * 256 + threadIdx.x] = s_h[p][threadIdx.x];
+        for (int p = 0; p < 4; p++) partials[((size_t)blockIdx.x * 4 + p) * 256 + t] = s_h[p][t];
 }
 
 // The 63-bit key variant (MLBVH_BUILD_OPTION_MORTON64, no reference
@@ -480,7 +491,7 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         // partials: gm blocks x 4 histograms, then the per-tile counts
         if ((rc = ensure_partials(ctx, (int)(gm * 4 + tiles), 1))) return rc;
         uint32_t* tileCounts = ctx->partials + (size_t)gm * 4 * 256;
-        k_morton<<<gm, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials, ctx->flags,
+        k_morton<<<gm, kMortonBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials, ctx->flags,
                                                  ctx->ctrl, (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
         launch_hist_reduce(ctx, (int)gm, 4, stream);
         rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts);
