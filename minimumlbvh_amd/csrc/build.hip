@@ -257,6 +257,7 @@ __global__ void __launch_bounds__(kLocalLeaves)
     __shared__ uint8_t s_d[kLocalLeaves + 1];
     __shared__ uint8_t s_pm[kLocalLeaves];      // s_pm[j] = max s_d[0 .. j]
     __shared__ uint8_t s_sm[kLocalLeaves + 1];  // s_sm[j] = max s_d[j .. kLocalLeaves]
+    __shared__ uint8_t s_loc[kLocalLeaves];     // internal node b0 + j lies inside the block (the predicate above)
     __shared__ uint32_t s_wmax[2][kWaves];
     __shared__ uint32_t s_flag[kLocalLeaves];   // arrival word of local node b0 + slot
     __shared__ Box s_box[kLocalLeaves][2];      // the node's child boxes
@@ -295,10 +296,21 @@ __global__ void __launch_bounds__(kLocalLeaves)
     if (lane == 63) s_wmax[0][wave] = pm;
     if (lane == 0) s_wmax[1][wave] = sm;
     __syncthreads();
-    for (int w = 0; w < wave; w++) pm = pm > s_wmax[0][w] ? pm : s_wmax[0][w];
-    for (int w = wave + 1; w < kWaves; w++) sm = sm > s_wmax[1][w] ? sm : s_wmax[1][w];
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {  // all reads at once, no per-wave trip count
+        const uint32_t a = s_wmax[0][w], b = s_wmax[1][w];
+        if (w < wave) pm = pm > a ? pm : a;
+        if (w > wave) sm = sm > b ? sm : b;
+    }
     s_pm[t] = (uint8_t)pm;
     s_sm[t + 1] = (uint8_t)sm;
+    __syncthreads();
+    // the block-local test of each internal node of the block, once, so that a
+    // climb step reads it together with its two deltas
+    if ((uint32_t)t + 1 < m) {
+        const uint32_t dp = s_d[t + 1];
+        s_loc[t] = (uint8_t)((b0 == 0 || (uint32_t)s_pm[t] >= dp) && (b1 == nInternals || (uint32_t)s_sm[t + 2] > dp));
+    }
     __syncthreads();
 
     const uint32_t pos = b0 + (uint32_t)t;
@@ -313,16 +325,17 @@ __global__ void __launch_bounds__(kLocalLeaves)
                 rootHere = true;
                 break;
             }
-            uint32_t dL = 0 < lo ? (uint32_t)s_d[lo - b0] : 0xFFFFFFFFu;
-            uint32_t dR = hi < nInternals ? (uint32_t)s_d[hi - b0 + 1] : 0xFFFFFFFFu;
+            // four independent LDS reads (indices stay inside the block: the
+            // range [lo, hi] only grows through block-local parents)
+            const uint32_t vL = s_d[lo - b0], vR = s_d[hi - b0 + 1];
+            const uint32_t lL = s_loc[lo > b0 ? lo - 1 - b0 : 0], lR = s_loc[hi - b0];
+            const uint32_t dL = 0 < lo ? vL : 0xFFFFFFFFu;
+            const uint32_t dR = hi < nInternals ? vR : 0xFFFFFFFFu;
+            const bool locL = lo > b0 && lL != 0;  // parent lo - 1
+            const bool locR = hi < b1 && lR != 0;  // parent hi
             int goLeft = dL < dR ? 1 : 0;
             uint32_t parent = goLeft ? lo - 1 : hi;
-            bool local = goLeft ? lo > b0 : hi < b1;
-            if (local) {
-                uint32_t jp = parent - b0 + 1;
-                uint32_t dp = s_d[jp];
-                local = (b0 == 0 || (uint32_t)s_pm[jp - 1] >= dp) && (b1 == nInternals || (uint32_t)s_sm[jp + 1] > dp);
-            }
+            const bool local = goLeft ? locL : locR;
             if (!local) {
                 emit = true;
                 break;
