@@ -113,13 +113,13 @@ constexpr uint32_t kTraceRefill = MLBVH_TRACE_REFILL;
 // (the hardware dispatcher balances the waves): 512 took 3.10 ms on 4M rays,
 // 256 3.27, 1024 3.12, 2048 4.15.  Below 16 waves per CU at 512 rays each
 // (2M rays on 256 CUs) one ray per lane is faster: on the level-9 sphere
-// 0.73 against 0.84 ms at 512K rays (128 per wave) and 1.23 against 1.27 ms
-// at 1M (256 per wave), while at 2M it took 1.98 against 1.81 ms.
+// it took 0.73 against 0.84 ms at 512K rays (chunks of 128) and 1.23
+// against 1.27 ms at 1M (chunks of 256), while at 2M rays it took 1.98
+// against 1.81 ms with chunks of 512.
 #ifndef MLBVH_PLAIN_CHUNK
 #define MLBVH_PLAIN_CHUNK 512
 #endif
 constexpr uint32_t kPlainChunk = MLBVH_PLAIN_CHUNK;
-
 
 template <bool kAny, bool kCount, bool kBatched, class TS>
 __global__ void __launch_bounds__(kTraceBlock)
