@@ -233,7 +233,8 @@ def main():
             "key_bits": 64,
             "ms": round(sort_avg, 4),
             "gkeys_per_s": round(N / (sort_avg / 1e3) / 1e9, 2),
-            "route": "3 top-digit onesweep passes + segment finish (full 8-pass fallback only for long runs of "
+            "route": "3 top-digit onesweep passes (the first reads tile prefixes scanned from the histogram read, "
+                     "no look-back) + segment finish (full 8-pass fallback only for long runs of "
                      "equal 24-bit prefix)",
             "target_ms": round((2 * P + 1) * 8 * N / (0.6 * HBM_PEAK_GBS * 1e9) * 1e3, 4),
             "lsd8_algorithmic_bytes": sort_bytes,
