@@ -301,10 +301,14 @@ def run_pt(reps: int, out: dict, spp: int = 8, max_depth: int = 4):
 
 
 def run_diff(steps: int, out: dict):
-    """C4: Adam steps (fwd + bwd + update) against forward-only time."""
+    """C4: Adam steps (fwd + bwd + update) against forward-only time.  One
+    untimed step first: the first launch of each backward, refit and rebuild
+    kernel loads its code, which is not per-step work."""
     sc = DiffScene()
     fwd_ms = events_ms(sc.forward, 20, 2)
     l0 = float(sc.forward())
+    sc.step()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev_s, ev_e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev_s.record()
