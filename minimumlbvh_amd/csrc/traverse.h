@@ -194,8 +194,8 @@ __device__ inline void traverse(const Node* __restrict__ nodes, TS tris, uint32_
 // depth-first order, and the last of them wins in both.
 // 16 entries (the oracle's ORC_LANE_STACK, so the counters compare): with 12
 // the 1080p normal render of the level-9 sphere took 2.95 ms against 0.98 ms
-// (stack-free restarts), with 20 the same time, with 24 the batched trace
-// 33.7 ms against 28.5 ms (fewer blocks per CU).
+// and the 4M-ray trace 3.61 against 3.05 ms, with 20 the same times, with 24
+// the batched trace 33.7 ms against 28.5 ms (fewer blocks per CU).
 constexpr uint32_t kLaneStack = 16;
 template <bool kAny, bool kCount, int kStride, class TS>
 __device__ inline void traverse_stack(const Node* __restrict__ nodes, TS tris, uint32_t root, f3 ro, f3 rd, f3 inv,
