@@ -373,11 +373,13 @@ __device__ inline uint32_t prefix24(K k, int shift)
 }
 
 // The order of a key among keys sharing its 24-bit prefix: its sort bits
-// below the prefix (at most 40) above its window position (12 bits).
+// below the prefix, [startBit, prefixShift), moved down to bit 0 (at most
+// 8 * (8 - 3) = 40 bits), above its window position (12 bits).  The word then
+// stays below bit 52, where the segment start goes.
 template <class K>
-__device__ inline uint64_t order_word(K k, K lowMask, int pos)
+__device__ inline uint64_t order_word(K k, K lowMask, int startBit, int pos)
 {
-    return ((uint64_t)(k & lowMask) << 12) | (uint64_t)(uint32_t)pos;
+    return ((uint64_t)((k & lowMask) >> startBit) << 12) | (uint64_t)(uint32_t)pos;
 }
 
 // sortMask selects the key bits [startBit, startBit + 8 * passes) that order
@@ -397,7 +399,8 @@ constexpr uint64_t kFinishFar = 0xFFFull;
 template <class K, bool kVals>
 __global__ void __launch_bounds__(kFinishBlock)
     k_segment_finish(const K* __restrict__ src, K* __restrict__ dst, const uint32_t* __restrict__ vsrc,
-                     uint32_t* __restrict__ vdst, uint32_t n, int prefixShift, K sortMask, uint32_t* ctrl)
+                     uint32_t* __restrict__ vdst, uint32_t n, int startBit, int prefixShift, K sortMask,
+                     uint32_t* ctrl)
 {
     constexpr int kWin = kFinishChunk + 2 * kFinishHalo;
     constexpr int kRows = kWin / 64;
@@ -481,7 +484,7 @@ __global__ void __launch_bounds__(kFinishBlock)
         // Inside a segment the prefix is common, so keys order by their sort
         // bits below the prefix, ties by window position: one comparison of
         // (low bits << 12 | position), window positions being < 4096.
-        mine[u] = ((uint64_t)(isShort ? a : (int)kFinishFar) << 52) | order_word(ld[u], lowMask, w);
+        mine[u] = ((uint64_t)(isShort ? a : (int)kFinishFar) << 52) | order_word(ld[u], lowMask, startBit, w);
         s_ow[w] = mine[u];
         segA[u] = a;
         segE[u] = e;
@@ -640,11 +643,11 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
     const int prefixShift = startBit + 8 * (live - 3);
     const unsigned fb = (unsigned)((n + kFinishChunk - 1) / kFinishChunk);
     if (vals)
-        k_segment_finish<K, true><<<fb, kFinishBlock, 0, stream>>>(ktmp, keys, vtmp, vals, n, prefixShift, sortMask,
-                                                                   ctx->ctrl);
+        k_segment_finish<K, true><<<fb, kFinishBlock, 0, stream>>>(ktmp, keys, vtmp, vals, n, startBit, prefixShift,
+                                                                   sortMask, ctx->ctrl);
     else
-        k_segment_finish<K, false><<<fb, kFinishBlock, 0, stream>>>(ktmp, keys, nullptr, nullptr, n, prefixShift,
-                                                                    sortMask, ctx->ctrl);
+        k_segment_finish<K, false><<<fb, kFinishBlock, 0, stream>>>(ktmp, keys, nullptr, nullptr, n, startBit,
+                                                                    prefixShift, sortMask, ctx->ctrl);
     // fallback: histograms of the lower digits (of the three-pass state, which
     // the finish does not modify), then every pass from that state
     k_hist<K><<<hg, kHistBlock, 0, stream>>>(ktmp, n, startBit, live - 3, ctx->partials, guard);

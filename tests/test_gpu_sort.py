@@ -198,10 +198,14 @@ def test_hybrid_segment_shapes(gpu, orc, case):
 
 
 @pytest.mark.parametrize("dtype,start,end", [(np.uint64, 4, 60), (np.uint64, 17, 41), (np.uint32, 3, 29),
-                                             (np.uint32, 0, 32), (np.uint64, 40, 64)])
+                                             (np.uint32, 0, 32), (np.uint64, 40, 64), (np.uint64, 4, 64),
+                                             (np.uint64, 1, 64), (np.uint64, 9, 64), (np.uint64, 7, 63),
+                                             (np.uint32, 7, 32)])
 def test_hybrid_bit_ranges(gpu, orc, dtype, start, end):
-    """The hybrid's 16-bit prefix is the two highest digits of the requested
-    range (digits past endBit included, as in every pass)."""
+    """The hybrid's 24-bit prefix is the three highest digits of the requested
+    range (digits past endBit included, as in every pass).  Ranges whose
+    start is not a multiple of 8 and whose top digit reaches bit 63 give the
+    finish kernel 40 sort bits below the prefix: the widest order word."""
     n = 600_011
     print(f"hybrid sort {np.dtype(dtype).name} [{start},{end}) n={n}", flush=True)
     r = np.random.default_rng(start * 100 + end)
