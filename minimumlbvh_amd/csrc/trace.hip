@@ -11,6 +11,7 @@
 // walks stay available through mlbvh_trace_stackfree /
 // mlbvh_trace_escape_link.
 #include "mlbvh_ctx.h"
+#include "mlbvh_math.h"
 #include "traverse.h"
 
 namespace mlbvh {
@@ -252,8 +253,10 @@ __device__ inline f3 hemisphere_cos(float xi0, float xi1)
     const float kPi = 3.14159265358979323846f;
     float phi = (xi0 * 2.0f) * kPi;
     float r = sqrtf(xi1);
-    float x = cosf(phi) * r;
-    float z = sinf(phi) * r;
+    float sp, cp;
+    fm::sincos(phi, &sp, &cp);  // in place of the reference's cosf / sinf
+    float x = cp * r;
+    float z = sp * r;
     float y = sqrtf(fmaxf(1.0f - r * r, 0.0f));
     return mk(x, y, z);
 }
@@ -569,7 +572,7 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
             const float step = 1.0f / (float)nSample;
             float ao = 0.0f;
             for (int k = 0; k < occluded; k++) ao += step;
-            float brightness = powf(1.0f - ao, 1.0f / 2.2f);
+            float brightness = fm::pow_pos(1.0f - ao, 1.0f / 2.2f);
             pixels[pixel] = pack_rgba(brightness, brightness, brightness);
             if (aoOut) aoOut[pixel] = ao;
         }
@@ -723,7 +726,7 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
             const float step = 1.0f / (float)nSample;
             float ao = 0.0f;
             for (uint32_t k = 0; k < occluded; k++) ao += step;
-            float brightness = powf(1.0f - ao, 1.0f / 2.2f);
+            float brightness = fm::pow_pos(1.0f - ao, 1.0f / 2.2f);
             pixels[pixel] = pack_rgba(brightness, brightness, brightness);
             if (aoOut) aoOut[pixel] = ao;
         }
@@ -824,7 +827,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_pack(uint32_t* pixels, const fl
     int i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     float4 a = acc[i];
-    pixels[i] = pack_rgba(powf(a.x / a.w, 1.0f / 2.2f), powf(a.y / a.w, 1.0f / 2.2f), powf(a.z / a.w, 1.0f / 2.2f));
+    pixels[i] = pack_rgba(fm::pow_pos(a.x / a.w, 1.0f / 2.2f), fm::pow_pos(a.y / a.w, 1.0f / 2.2f),
+                          fm::pow_pos(a.z / a.w, 1.0f / 2.2f));
 }
 
 static int check_scene(mlbvh_ctx* ctx, const void* root, const void* nodes, const void* tris, const char* who)

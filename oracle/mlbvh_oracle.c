@@ -870,12 +870,129 @@ uint32_t orc_pack_rgba(float r, float g, float b, float a) /* main_gpu.cu:17-22 
     return ((uint32_t)z << 16) | ((uint32_t)y << 8) | (uint32_t)x;
 }
 
+/* ---------------------------------------------------------------------- */
+/* sin / cos / pow in place of libm's (main_gpu.cu:30-31, :174, :283).      */
+/* Same operations in the same order as the kernels' functions, so host and */
+/* GPU agree bit for bit.  Published methods:                               */
+/*  - sin/cos: Cody-Waite reduction (Cody & Waite 1980) by pi/2 split into  */
+/*    P1 + P2 + P3 (P1, P2: 13 significant bits, so k*P1, k*P2 are exact    */
+/*    for |k| < 2^11), then the Taylor polynomials of sin to r^9 and cos to */
+/*    r^10 on |r| <= pi/4, Horner form, quadrant swap/sign.                 */
+/*  - pow(x, y), x >= 0, y > 0: 2^(y log2 x); log2 x = e + ln(m)/ln 2 with  */
+/*    m in [sqrt(1/2), sqrt 2] and ln m = 2 atanh((m-1)/(m+1)) by its odd   */
+/*    series to s^9; y*e split exactly (y cut to 12 significant bits);      */
+/*    2^f, |f| <= 1/2, by the Taylor series to f^7 with coefficients        */
+/*    (ln 2)^n / n!; scaled by 2^k in two exact steps.                      */
+/* Pinned by tests/test_oracle_math.py: the constants against their exact   */
+/* values, and the functions against float64 sin / cos / pow.               */
+/* ---------------------------------------------------------------------- */
+static inline float f_bits(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static inline uint32_t bits_f(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+
+void orc_sincos(float x, float* s, float* c)
+{
+    const float P1 = 0x1.921p+0f, P2 = 0x1.f6ap-13f, P3 = 0x1.110b46p-26f;
+    const float kf = rintf(x * 0x1.45f306p-1f);
+    const int k = (int)kf;
+    const float r = ((x - kf * P1) - kf * P2) - kf * P3;
+    const float r2 = r * r;
+    const float sp = r + (r * r2) * (-0x1.555556p-3f + r2 * (0x1.111112p-7f + r2 * (-0x1.a01a02p-13f +
+                                                                                       r2 * 0x1.71de3ap-19f)));
+    const float cp = 1.0f + r2 * (-0x1p-1f + r2 * (0x1.555556p-5f + r2 * (-0x1.6c16c2p-10f +
+                                                                           r2 * (0x1.a01a02p-16f +
+                                                                                 r2 * -0x1.27e4fcp-22f))));
+    switch (k & 3) {
+        case 0: *s = sp; *c = cp; break;
+        case 1: *s = cp; *c = -sp; break;
+        case 2: *s = -sp; *c = -cp; break;
+        default: *s = -cp; *c = sp; break;
+    }
+}
+
+static float log2_parts(float x, int* eOut) /* x > 0, finite: log2 x = e + L, |L| <= 1/2 */
+{
+    uint32_t b = bits_f(x);
+    int e = 0;
+    if (b < 0x00800000u) {
+        x = x * 0x1p24f;
+        b = bits_f(x);
+        e = -24;
+    }
+    e += (int)(b >> 23) - 127;
+    float m = f_bits((b & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 0x1.6a09e6p+0f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    const float lnm = s * (2.0f + s2 * (0x1.555556p-1f + s2 * (0x1.99999ap-2f + s2 * (0x1.24924ap-2f +
+                                                                                       s2 * 0x1.c71c72p-3f))));
+    *eOut = e;
+    return lnm * 0x1.715476p+0f;
+}
+
+static float exp2_parts(float kf, float f) /* 2^(k + f), k integer-valued, |f| <= 1/2 */
+{
+    if (kf > 128.0f) return f_bits(0x7F800000u);
+    if (kf < -151.0f) return 0.0f;
+    const float p = 1.0f + f * (0x1.62e430p-1f + f * (0x1.ebfbe0p-3f + f * (0x1.c6b08ep-5f + f * (0x1.3b2ab6p-7f +
+                    f * (0x1.5d87fep-10f + f * (0x1.430912p-13f + f * 0x1.ffcbfcp-17f))))));
+    const int k = (int)kf;
+    const int k1 = k >> 1, k2 = k - k1;
+    return (p * f_bits((uint32_t)(k1 + 127) << 23)) * f_bits((uint32_t)(k2 + 127) << 23);
+}
+
+float orc_log2_pos(float x) /* x > 0, finite */
+{
+    int e;
+    const float L = log2_parts(x, &e);
+    return (float)e + L;
+}
+
+float orc_exp2_fin(float z) /* finite z */
+{
+    const float kf = rintf(z);
+    return exp2_parts(kf, z - kf);
+}
+
+float orc_pow_pos(float x, float y) /* y > 0; x <= 0 gives 0 (libm: NaN for x < 0, which packs to the same 0) */
+{
+    if (x != x) return x;
+    if (!(x > 0.0f)) return 0.0f;
+    if (x == f_bits(0x7F800000u)) return x;
+    int e;
+    const float L = log2_parts(x, &e);
+    const float yh = f_bits(bits_f(y) & 0xFFFFF000u); /* 12 significant bits: yh * e is exact */
+    const float yl = y - yh;
+    const float fe = (float)e;
+    const float zh = yh * fe;
+    float kf = rintf(zh);
+    float f = (zh - kf) + (yl * fe + y * L);
+    const float k2 = rintf(f);
+    f = f - k2;
+    kf = kf + k2;
+    return exp2_parts(kf, f);
+}
+
+void orc_sincos_batch(const float* x, int n, float* s, float* c)
+{
+    for (int i = 0; i < n; i++) orc_sincos(x[i], &s[i], &c[i]);
+}
+
+void orc_pow_batch(const float* x, int n, float y, float* out)
+{
+    for (int i = 0; i < n; i++) out[i] = orc_pow_pos(x[i], y);
+}
+
 static orc_v3 hemisphere_cos(float xi0, float xi1) /* main_gpu.cu:24-36 */
 {
     float phi = xi0 * 2.0f * ORC_PI;
     float r = sqrtf(xi1);
-    float x = cosf(phi) * r;
-    float z = sinf(phi) * r;
+    float sp, cp;
+    orc_sincos(phi, &sp, &cp);
+    float x = cp * r;
+    float z = sp * r;
     float y = sqrtf(fmaxf(1.0f - r * r, 0.0f));
     return V(x, y, z);
 }
@@ -887,6 +1004,29 @@ static void onb(orc_v3 z, orc_v3* xaxis, orc_v3* yaxis) /* Duff et al. 2017, mai
     const float b = z.x * z.y * a;
     *xaxis = V(1.0f + sign * z.x * z.x * a, sign * b, -sign * z.x);
     *yaxis = V(b, sign + z.y * z.y * a, -z.y);
+}
+
+/* dir = xaxis * dl.x + zaxis * dl.z + n * dl.y, main_gpu.cu:153 and :235 */
+static inline orc_v3 sample_dir(orc_v3 n, orc_v3 xaxis, orc_v3 zaxis, float r0, float r1)
+{
+    orc_v3 dl = hemisphere_cos(r0, r1);
+    return vadd(vadd(vmuls(xaxis, dl.x), vmuls(zaxis, dl.z)), vmuls(n, dl.y));
+}
+
+void orc_onb(const float* z, float* out)
+{
+    orc_v3 x, y;
+    onb(V(z[0], z[1], z[2]), &x, &y);
+    out[0] = x.x, out[1] = x.y, out[2] = x.z;
+    out[3] = y.x, out[4] = y.y, out[5] = y.z;
+}
+
+void orc_cosine_dir(const float* n, float xi0, float xi1, float* out)
+{
+    orc_v3 nn = V(n[0], n[1], n[2]), x, y;
+    onb(nn, &x, &y);
+    orc_v3 d = sample_dir(nn, x, y, xi0, xi1);
+    out[0] = d.x, out[1] = d.y, out[2] = d.z;
 }
 
 typedef struct { uint64_t state, inc; } orc_pcg; /* main_gpu.cu:51-82 */
@@ -1197,8 +1337,7 @@ void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_rayge
                     r0 = pcg_uniformf(&rng);
                     r1 = pcg_uniformf(&rng);
                 }
-                orc_v3 dl = hemisphere_cos(r0, r1);
-                orc_v3 dir = vadd(vadd(vmuls(xaxis, dl.x), vmuls(zaxis, dl.z)), vmuls(n, dl.y));
+                orc_v3 dir = sample_dir(n, xaxis, zaxis, r0, r1);
                 int occluded;
                 if (walk >= 2) {
                     occluded = orc_any_hit_stack(nodes, tris, root, ao_origin, dir, orc_inv_rd(dir), radius, cnt,
@@ -1215,7 +1354,7 @@ void orc_render_ao(uint32_t* pixels, float* aoOut, int w, int h, const orc_rayge
             }
             cFetch += cnt[0];
             cTest += cnt[1];
-            float brightness = powf(1.0f - ao, 1.0f / 2.2f);
+            float brightness = orc_pow_pos(1.0f - ao, 1.0f / 2.2f);
             pixels[pixel] = orc_pack_rgba(brightness, brightness, brightness, 1.0f);
             if (aoOut) aoOut[pixel] = ao;
         }
@@ -1269,9 +1408,8 @@ void orc_render_pt(float* acc4, int w, int h, const orc_raygen* g, uint32_t root
                     color = vadd(color, vmul(tp, at->emissive));
                     orc_v3 xaxis, zaxis;
                     onb(hn, &xaxis, &zaxis);
-                    orc_v3 dl = hemisphere_cos(r0, r1);
                     orc_v3 nro = vadd(hp, vmuls(hn, 0.0001f));
-                    orc_v3 nrd = vadd(vadd(vmuls(xaxis, dl.x), vmuls(zaxis, dl.z)), vmuls(hn, dl.y));
+                    orc_v3 nrd = sample_dir(hn, xaxis, zaxis, r0, r1);
                     tp = vmul(tp, at->reflectance);
                     float maxComp = hmax(hmax(tp.x, tp.y), tp.z);
                     float p = hmin(maxComp / 0.25f, 1.0f);
@@ -1303,8 +1441,8 @@ void orc_pack(uint32_t* pixels, const float* acc, int n) /* main_gpu.cu:273-286 
 {
     for (int i = 0; i < n; i++) {
         const float* a = acc + 4 * (size_t)i;
-        pixels[i] = orc_pack_rgba(powf(a[0] / a[3], 1.0f / 2.2f), powf(a[1] / a[3], 1.0f / 2.2f),
-                                  powf(a[2] / a[3], 1.0f / 2.2f), 1.0f);
+        pixels[i] = orc_pack_rgba(orc_pow_pos(a[0] / a[3], 1.0f / 2.2f), orc_pow_pos(a[1] / a[3], 1.0f / 2.2f),
+                                  orc_pow_pos(a[2] / a[3], 1.0f / 2.2f), 1.0f);
     }
 }
 

@@ -114,6 +114,18 @@ void orc_pcg32(uint64_t seed, uint64_t seq, int n, uint32_t* out);
 void orc_pcg32_uniformf(uint64_t seed, uint64_t seq, int n, float* out);
 /* cosine-weighted hemisphere direction of main_gpu.cu:24-36 (y = up) */
 void orc_hemisphere_cos(float xi0, float xi1, float* out);
+/* Duff et al. 2017 orthonormal basis of main_gpu.cu:39-45: out = xaxis[3], yaxis[3] for the unit normal z */
+void orc_onb(const float* z, float* out);
+/* the world direction the AO and PT kernels build for unit normal n from (xi0, xi1): the ONB of n and the
+   cosine-hemisphere sample, x*dl.x + y*dl.z + n*dl.y (main_gpu.cu:139-153, :231-235) */
+void orc_cosine_dir(const float* n, float xi0, float xi1, float* out);
+/* sin / cos / log2 / exp2 / pow in place of libm's (main_gpu.cu:30-31, :174, :283); pow for y > 0 */
+void orc_sincos(float x, float* s, float* c);
+float orc_log2_pos(float x);
+float orc_exp2_fin(float z);
+float orc_pow_pos(float x, float y);
+void orc_sincos_batch(const float* x, int n, float* s, float* c);
+void orc_pow_batch(const float* x, int n, float y, float* out);
 void orc_sobol_2d(float* x, float* y, uint32_t index, uint32_t p0, uint32_t p1, uint32_t p2);
 void orc_raygen_lookat(orc_raygen* g, orc_v3 eye, orc_v3 center, orc_v3 up, float fovy, int w, int h);
 void orc_raygen_shoot(const orc_raygen* g, orc_v3* ro, orc_v3* rd, float u, float v);

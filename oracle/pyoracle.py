@@ -72,6 +72,14 @@ def lib():
             "orc_pcg32": ([c_uint64, c_uint64, c_int, P], None),
             "orc_pcg32_uniformf": ([c_uint64, c_uint64, c_int, P], None),
             "orc_hemisphere_cos": ([c_float, c_float, P], None),
+            "orc_onb": ([P, P], None),
+            "orc_cosine_dir": ([P, c_float, c_float, P], None),
+            "orc_sincos": ([c_float, POINTER(c_float), POINTER(c_float)], None),
+            "orc_log2_pos": ([c_float], c_float),
+            "orc_exp2_fin": ([c_float], c_float),
+            "orc_pow_pos": ([c_float, c_float], c_float),
+            "orc_sincos_batch": ([P, c_int, P, P], None),
+            "orc_pow_batch": ([P, c_int, c_float, P], None),
             "orc_sobol_2d": ([POINTER(c_float), POINTER(c_float), c_uint32, c_uint32, c_uint32, c_uint32], None),
             "orc_raygen_lookat": ([P, V3, V3, V3, c_float, c_int, c_int], None),
             "orc_raygen_shoot": ([P, P, P, c_float, c_float], None),
@@ -204,6 +212,39 @@ def sobol_2d(index, p0, p1, p2):
     x, y = c_float(), c_float()
     lib().orc_sobol_2d(ctypes.byref(x), ctypes.byref(y), index, p0, p1, p2)
     return np.float32(x.value), np.float32(y.value)
+
+
+def onb(n):
+    """(xaxis, yaxis) of the Duff et al. 2017 basis for the unit normal n."""
+    out = np.zeros(6, np.float32)
+    lib().orc_onb(_p(_f32(n)), _p(out))
+    return out[:3], out[3:]
+
+
+def cosine_dir(n, xi0, xi1):
+    out = np.zeros(3, np.float32)
+    lib().orc_cosine_dir(_p(_f32(n)), xi0, xi1, _p(out))
+    return out
+
+
+def sincos(x):
+    s, c = c_float(), c_float()
+    lib().orc_sincos(x, ctypes.byref(s), ctypes.byref(c))
+    return np.float32(s.value), np.float32(c.value)
+
+
+def sincos_batch(x):
+    x = _f32(x)
+    s, c = np.zeros_like(x), np.zeros_like(x)
+    lib().orc_sincos_batch(_p(x), x.size, _p(s), _p(c))
+    return s, c
+
+
+def pow_batch(x, y):
+    x = _f32(x)
+    out = np.zeros_like(x)
+    lib().orc_pow_batch(_p(x), x.size, y, _p(out))
+    return out
 
 
 def raygen_lookat(eye, center, up, fovy, w, h):

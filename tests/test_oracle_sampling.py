@@ -169,3 +169,199 @@ def test_pt_furnace_closed_form(orc, sobol):
         else:
             assert np.all((paths == np.float32(1.75)) | (paths == np.float32(2.0)))
             assert 0.45 < (paths == np.float32(2.0)).mean() < 0.55
+
+
+def _unit_normals(n, seed):
+    r = np.random.default_rng(seed)
+    v = r.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    special = [(0, 0, 1), (0, 0, -1), (1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (0.6, 0, 0.8), (0, 0.6, -0.8),
+               (1e-4, 0, -1.0), (0, 0, 0), (0, 0, 0)]
+    v = np.vstack([v, special]).astype(np.float32)
+    v[-2] = (0.0, 1.0, -0.0)  # n.z = -0: sign -1
+    v[-1] = (1.0, 0.0, 0.0)
+    v = (v / np.linalg.norm(v.astype(np.float64), axis=1, keepdims=True)).astype(np.float32)
+    return v
+
+
+def test_onb_is_orthonormal_and_right_handed(orc):
+    """GetOrthonormalBasis (Duff et al. 2017, main_gpu.cu:39-45) on random
+    unit normals and the poles, axes and n.z = +-0: x.n = y.n = x.y = 0,
+    |x| = |y| = 1 and x cross y = n, to float rounding."""
+    for n in _unit_normals(2000, 5):
+        x, y = orc.onb(n)
+        x, y, n64 = x.astype(np.float64), y.astype(np.float64), n.astype(np.float64)
+        assert abs(x @ n64) < 4e-7 and abs(y @ n64) < 4e-7 and abs(x @ y) < 4e-7, n
+        assert abs(np.linalg.norm(x) - 1) < 4e-7 and abs(np.linalg.norm(y) - 1) < 4e-7, n
+        assert np.abs(np.cross(x, y) - n64).max() < 6e-7, n
+    # the pole cases in closed form
+    x, y = orc.onb(np.array([0, 0, 1], np.float32))
+    assert x.tolist() == [1, 0, 0] and y.tolist() == [0, 1, 0]
+    x, y = orc.onb(np.array([0, 0, -1], np.float32))
+    assert x.tolist() == [1, 0, 0] and y.tolist() == [0, -1, 0]
+
+
+def test_cosine_directions_around_a_normal(orc):
+    """The AO / PT bounce direction (main_gpu.cu:152-153, :231-235): for any
+    unit normal the direction is unit length, in the normal's hemisphere
+    (d.n = sqrt(1 - xi1)), and over a midpoint grid of (xi0, xi1) it has the
+    cosine distribution's moments E[d.n] = 2/3, E[d] = (2/3) n, and
+    P(d.n >= c) = 1 - c^2."""
+    g = (np.arange(48) + 0.5) / 48
+    for n in _unit_normals(12, 6):
+        d = np.array([orc.cosine_dir(n, a, b) for a in g for b in g], np.float64)
+        n64 = n.astype(np.float64)
+        assert np.abs(np.linalg.norm(d, axis=1) - 1).max() < 1e-6
+        dn = d @ n64
+        assert dn.min() >= -1e-7
+        want = np.sqrt(1 - np.repeat(g[None, :], 48, axis=0).ravel())
+        assert np.abs(dn - want).max() < 2e-6
+        assert abs(dn.mean() - 2 / 3) < 2e-4
+        assert np.abs(d.mean(axis=0) - (2 / 3) * n64).max() < 2e-4
+        for c in (0.25, 0.5, 0.8):
+            assert abs((dn >= c).mean() - (1 - c * c)) < 0.03
+
+
+def _quad(y, half, up):
+    """Square at height y in the xz plane; `up` picks the winding, so the
+    geometric normal (v1-v0)x(v2-v1) points +y or -y."""
+    a, b, c, d = (-half, y, -half), (half, y, -half), (half, y, half), (-half, y, half)
+    tris = [(a, d, c), (a, c, b)] if up else [(a, b, c), (a, c, d)]
+    return np.asarray(tris, np.float32)
+
+
+def _render_ao(orc, tris, cam, w, h, n_sample, radius):
+    nodes, root, _, _ = orc.build(tris)
+    eye, ctr, up, fov = cam
+    g = orc.raygen_lookat(eye, ctr, up, math.radians(fov), w, h)
+    px, ao, _ = orc.render_ao(g, w, h, root, nodes, tris, 1, n_sample, radius, 0, primary_stack=True)
+    return px, ao
+
+
+@pytest.mark.parametrize("n_sample", [4, 16])
+def test_ao_closed_box_is_fully_occluded(orc, n_sample):
+    """Inside the closed cube [-1, 1]^3 every primary ray hits and, with a
+    radius above the cube's diagonal, every AO ray hits a wall: ao = N * 1/N
+    = 1 exactly and the pixel is black.  This fails if the normal were not
+    turned toward the ray (the AO origin would sit 0.001 outside the box and
+    every ray would leave it)."""
+    from minimumlbvh_amd import scenes
+
+    tris, _ = scenes.furnace_box()
+    px, ao = _render_ao(orc, tris, scenes.CAMERA_FURNACE, 40, 32, n_sample, 4.0)
+    assert np.all(ao == 1.0) and np.all(px == 0)
+
+
+@pytest.mark.parametrize("up", [True, False])
+def test_ao_lone_quad_is_unoccluded(orc, up):
+    """A lone square seen from above: every AO ray leaves through the
+    normal's hemisphere and hits nothing, so ao = 0 and the pixel is white,
+    whichever way the square is wound.  A direction built on the wrong side
+    of the surface would hit the square itself."""
+    tris = _quad(0.0, 1.0, up)
+    cam = ((0.0, 1.0, 0.0), (0.0, 0.0, 0.001), (0.0, 0.0, 1.0), 60.0)
+    px, ao = _render_ao(orc, tris, cam, 48, 48, 16, 0.5)
+    hit = ao >= 0
+    assert hit.sum() > 1000
+    assert np.all(ao[hit] == 0.0) and np.all(px[hit] == 0xFFFFFF) and np.all(px[~hit] == 0)
+
+
+def test_ao_between_parallel_planes_closed_form(orc):
+    """Floor y = 0 and ceiling y = h, seen from between them: an AO ray from
+    the floor (origin lifted by 0.001 along the normal, length = radius)
+    reaches the ceiling iff cos(theta) >= c = (h - 0.001) / radius, which
+    under the cosine distribution has probability 1 - c^2.  With h = 0.101
+    and radius 0.2, E[ao] = 0.75; without the lift it would be 0.745, with
+    the lift reversed 0.7399.  The pixel is the 8-bit level of
+    (1 - ao)^(1/2.2) (float64 power, independent of the kernels' pow)."""
+    h, radius = 0.101, 0.2
+    tris = np.concatenate([_quad(0.0, 50.0, True), _quad(h, 50.0, False)])
+    cam = ((0.0, h / 2, 0.0), (0.0, 0.0, 0.001), (0.0, 0.0, 1.0), 60.0)
+    px, ao = _render_ao(orc, tris, cam, 128, 128, 16, radius)
+    assert np.all(ao >= 0)
+    c = (h - 0.001) / radius
+    assert abs(ao.mean() - (1 - c * c)) < 0.0025, ao.mean()
+    want = np.clip(np.trunc(np.power(1.0 - ao.astype(np.float64), 1 / 2.2) * 255 + 0.5), 0, 255).astype(np.uint32)
+    assert np.array_equal(px & 0xFF, want) and np.array_equal(px >> 8 & 0xFF, want) and np.array_equal(px >> 16, want)
+
+
+@pytest.mark.parametrize("up", [True, False])
+def test_normal_image_of_an_axis_aligned_quad(orc, up):
+    """main_gpu.cu:84-109: the pixel is packRGBA((n + 1) / 2) of the
+    unflipped unit geometric normal: (0, +-1, 0) gives (128, 255, 128) or
+    (128, 0, 128); misses are black."""
+    tris = _quad(0.0, 1.0, up)
+    nodes, root, _, _ = orc.build(tris)
+    g = orc.raygen_lookat((0.0, 1.0, 0.0), (0.0, 0.0, 0.001), (0.0, 0.0, 1.0), math.radians(60), 40, 40)
+    ng = np.cross(tris[0, 1] - tris[0, 0], tris[0, 2] - tris[0, 1]).astype(np.float64)
+    n = ng / np.linalg.norm(ng)
+    assert np.allclose(n, [0, 1 if up else -1, 0])
+    rgb = np.trunc((n + 1) / 2 * 255 + 0.5).astype(np.uint32)
+    want = rgb[2] << 16 | rgb[1] << 8 | rgb[0]
+    px = orc.render_normal(g, 40, 40, root, nodes, tris)
+    assert (px == want).sum() > 1000 and np.all((px == want) | (px == 0))
+
+
+def _planes_scene(h, half, floor_albedo, light):
+    """Diffuse floor at y = 0 and an emissive, black ceiling at y = h."""
+    tris = np.concatenate([_quad(0.0, half, True), _quad(h, half, True)])
+    attribs = np.zeros((4, 15), np.float32)
+    attribs[:2, 9:12] = floor_albedo
+    attribs[2:, 12:15] = light
+    return tris, attribs
+
+
+@pytest.mark.parametrize("up", [True, False])
+def test_pt_one_bounce_to_an_emissive_ceiling(orc, up):
+    """Path tracing (main_gpu.cu:179-271) with max depth 2 under an infinite
+    emissive ceiling: the floor emits nothing, its bounce (p = 1 for albedo
+    0.5) always reaches the ceiling, so every path carries exactly
+    albedo * Le.  A bounce into the floor's lower hemisphere, or an origin
+    offset to the wrong side, would miss and give 0."""
+    light = np.array([1.0, 2.0, 3.0], np.float32)
+    tris, attribs = _planes_scene(0.05, 1000.0, 0.5, light)
+    if not up:
+        tris[:2] = tris[:2, ::-1]
+    nodes, root, _, _ = orc.build(tris)
+    g = orc.raygen_lookat((0.0, 0.025, 0.0), (0.0, 0.0, 0.001), (0.0, 0.0, 1.0), math.radians(60), 24, 24)
+    acc = np.zeros((24 * 24, 4), np.float32)
+    for _ in range(4):
+        acc = orc.render_pt(acc, g, 24, 24, root, nodes, tris, attribs, 1, 1024, 2)
+    assert np.array_equal(acc, np.tile(np.append(4 * 0.5 * light, 4).astype(np.float32), (24 * 24, 1)))
+
+
+def _square_form_factor(a, h):
+    """Cosine-weighted fraction of directions from a point that hit a
+    parallel square of side a centred at height h above it: four corner
+    rectangles of the differential-area-to-parallel-rectangle form factor."""
+    A = B = (a / 2) / h
+
+    def corner(A, B):
+        sa, sb = math.sqrt(1 + A * A), math.sqrt(1 + B * B)
+        return (A / sa * math.atan(B / sa) + B / sb * math.atan(A / sb)) / (2 * math.pi)
+
+    return 4 * corner(A, B)
+
+
+def test_pt_direct_light_form_factor(orc):
+    """A floor point under a square light of side a at height h receives
+    albedo * Le * F with F the point-to-square form factor: the path
+    tracer's bounce directions have the cosine distribution.  The camera
+    sees a 0.004-wide patch of floor under the light's centre."""
+    h, a = 0.2, 0.3
+    F = _square_form_factor(a, h)
+    floor = _quad(0.0, 50.0, True)
+    light = _quad(h, a / 2, False)
+    tris = np.concatenate([floor, light])
+    attribs = np.zeros((4, 15), np.float32)
+    attribs[:2, 9:12] = 0.5
+    attribs[2:, 12:15] = 1.0
+    nodes, root, _, _ = orc.build(tris)
+    w = hh = 64
+    g = orc.raygen_lookat((0.0, 0.1, 0.0), (0.0, 0.0, 1e-4), (0.0, 0.0, 1.0), math.radians(2.3), w, hh)
+    acc = np.zeros((w * hh, 4), np.float32)
+    spp = 16
+    for _ in range(spp):
+        acc = orc.render_pt(acc, g, w, hh, root, nodes, tris, attribs, 1, 1024, 2)
+    mean = acc[:, 0].sum() / acc[:, 3].sum()
+    assert abs(mean - 0.5 * F) < 0.004, (mean, 0.5 * F)
