@@ -18,6 +18,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "mlbvh.h"
 #include "tinyhiponesweep.h"
@@ -59,6 +60,31 @@ MINIMUM_LBVH_DEVICE inline float3 hmin(float3 a, float3 b)
 MINIMUM_LBVH_DEVICE inline float3 hmax(float3 a, float3 b)
 {
     return make_float3(hmax(a.x, b.x), hmax(a.y, b.y), hmax(a.z, b.z));
+}
+// Merge of two boxes: the host rule above keeps the second operand on ties,
+// so -0 against +0 would store a sign that depends on which child merges
+// first.  Ties keep -0 for a lower bound and +0 for an upper one (the
+// libmlbvh kernels and its checker do the same), so a node's box has the same
+// bits whatever the order; any other input gives the host rule's bits.
+MINIMUM_LBVH_DEVICE inline float umin(float a, float b)
+{
+    uint32_t ua, ub;
+    memcpy(&ua, &a, 4);
+    memcpy(&ub, &b, 4);
+    uint32_t u = ua | ub;
+    float t;
+    memcpy(&t, &u, 4);
+    return a < b ? a : (b < a ? b : t);
+}
+MINIMUM_LBVH_DEVICE inline float umax(float a, float b)
+{
+    uint32_t ua, ub;
+    memcpy(&ua, &a, 4);
+    memcpy(&ub, &b, 4);
+    uint32_t u = ua & ub;
+    float t;
+    memcpy(&t, &u, 4);
+    return a > b ? a : (b > a ? b : t);
 }
 // float -> int truncation with the reference host's (x86 cvttss2si) rule
 MINIMUM_LBVH_DEVICE inline int f2i(float f)
@@ -204,8 +230,8 @@ struct AABB {
     }
     MINIMUM_LBVH_DEVICE void extend(const AABB& b)
     {
-        lower = hmin(lower, b.lower);
-        upper = hmax(upper, b.upper);
+        lower = make_float3(umin(lower.x, b.lower.x), umin(lower.y, b.lower.y), umin(lower.z, b.lower.z));
+        upper = make_float3(umax(upper.x, b.upper.x), umax(upper.y, b.upper.y), umax(upper.z, b.upper.z));
     }
     MINIMUM_LBVH_DEVICE uint64_t encodeMortonCode(float3 p) const
     {

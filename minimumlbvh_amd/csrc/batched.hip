@@ -326,8 +326,8 @@ __global__ void __launch_bounds__(kBatchLeaves)
             s_context[slot] = hi < nInternals ? hi : 0xFFFFFFFFu;
             node = parent;
             Box other = s_box[slot][goLeft ^ 1];
-            box.lo = hmin3(box.lo, other.lo);
-            box.hi = hmax3(box.hi, other.hi);
+            box.lo = umin3(box.lo, other.lo);
+            box.hi = umax3(box.hi, other.hi);
         }
     }
     if (rootHere) roots[mesh] = node;
@@ -392,8 +392,8 @@ __global__ void __launch_bounds__(kBatchBlock)
             box = tri_box(tris[base + (node & ~kLeafBit)]);
         } else {
             const Node& nd = nodes[node];
-            box.lo = hmin3(nd.box[0].lo, nd.box[1].lo);
-            box.hi = hmax3(nd.box[0].hi, nd.box[1].hi);
+            box.lo = umin3(nd.box[0].lo, nd.box[1].lo);
+            box.hi = umax3(nd.box[0].hi, nd.box[1].hi);
         }
         climb_from(off[mesh + 1] - base, lo, hi, node, box, deltasAll + base, nodes, flagsAll + (base - mesh),
                    roots + mesh);

@@ -352,8 +352,8 @@ __global__ void __launch_bounds__(kLocalLeaves)
             s_context[slot] = hi < nInternals ? hi : 0xFFFFFFFFu;
             node = parent;
             Box other = s_box[slot][goLeft ^ 1];
-            box.lo = hmin3(box.lo, other.lo);
-            box.hi = hmax3(box.hi, other.hi);
+            box.lo = umin3(box.lo, other.lo);
+            box.hi = umax3(box.hi, other.hi);
         }
     }
     if (rootHere) *root = node;
@@ -418,8 +418,8 @@ __global__ void __launch_bounds__(kBuildBlock)
             box = tri_box(tris[node & ~kLeafBit]);
         } else {
             const Node& nd = nodes[node];
-            box.lo = hmin3(nd.box[0].lo, nd.box[1].lo);
-            box.hi = hmax3(nd.box[0].hi, nd.box[1].hi);
+            box.lo = umin3(nd.box[0].lo, nd.box[1].lo);
+            box.hi = umax3(nd.box[0].hi, nd.box[1].hi);
         }
         climb_from(n, lo, hi, node, box, deltas, nodes, flags, root);
     }

@@ -73,6 +73,22 @@ __device__ inline float hmin(float a, float b) { return a < b ? a : b; }
 __device__ inline float hmax(float a, float b) { return a > b ? a : b; }
 __device__ inline f3 hmin3(f3 a, f3 b) { return mk(hmin(a.x, b.x), hmin(a.y, b.y), hmin(a.z, b.z)); }
 __device__ inline f3 hmax3(f3 a, f3 b) { return mk(hmax(a.x, b.x), hmax(a.y, b.y), hmax(a.z, b.z)); }
+// The climb's merge of two child boxes.  The host rule keeps the second
+// operand on ties, so with -0 on one side and +0 on the other the stored sign
+// would depend on which child's thread merges first (the reference's GPU and
+// parallel CPU builds are order-dependent there).  Ties keep -0 for a lower
+// bound and +0 for an upper one (bitwise OR / AND of equal values): every
+// other input gives the host rule's bits, and the merge is symmetric.
+__device__ inline float umin(float a, float b)
+{
+    return a < b ? a : (b < a ? b : __uint_as_float(__float_as_uint(a) | __float_as_uint(b)));
+}
+__device__ inline float umax(float a, float b)
+{
+    return a > b ? a : (b > a ? b : __uint_as_float(__float_as_uint(a) & __float_as_uint(b)));
+}
+__device__ inline f3 umin3(f3 a, f3 b) { return mk(umin(a.x, b.x), umin(a.y, b.y), umin(a.z, b.z)); }
+__device__ inline f3 umax3(f3 a, f3 b) { return mk(umax(a.x, b.x), umax(a.y, b.y), umax(a.z, b.z)); }
 
 // float -> int truncation with x86-64 cvttss2si semantics (out of range and
 // NaN give INT_MIN), the behaviour of the reference's host make_int3(float3).
@@ -243,8 +259,8 @@ __device__ inline bool climb_from(uint32_t n, uint32_t lo, uint32_t hi, uint32_t
         p->context = hi < nInternals ? hi : 0xFFFFFFFFu;
         node = parent;
         Box other = load_box_coherent(&p->box[goLeft ^ 1]);
-        box.lo = hmin3(box.lo, other.lo);
-        box.hi = hmax3(box.hi, other.hi);
+        box.lo = umin3(box.lo, other.lo);
+        box.hi = umax3(box.hi, other.hi);
     }
     *root = node;
     nodes[node & ~kLeafBit].parent = kInvalidNode;

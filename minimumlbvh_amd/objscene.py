@@ -25,6 +25,7 @@ reads out of bounds there).
 """
 from __future__ import annotations
 
+import gzip
 import os
 from dataclasses import dataclass, field
 
@@ -86,7 +87,15 @@ def _triangulate(pos: list) -> list:
     return [(0, j, j + 1) for j in range(1, k - 1)]
 
 
+def _open_text(path: str):
+    if path.endswith(".gz"):
+        return gzip.open(path, "rt", encoding="utf-8", errors="replace")
+    return open(path, "r", encoding="utf-8", errors="replace")
+
+
 def load_obj(path: str) -> ObjScene:
+    """Reads an OBJ file (plain, or gzip-compressed when the name ends in
+    .gz); `mtllib` files are looked up beside it."""
     verts: list = []
     normals: list = []
     materials: list = []
@@ -95,7 +104,7 @@ def load_obj(path: str) -> ObjScene:
     tris, attrs, mids = [], [], []
     nfaces = 0
     base = os.path.dirname(os.path.abspath(path))
-    with open(path, "r", encoding="utf-8", errors="replace") as fh:
+    with _open_text(path) as fh:
         for line in fh:
             tok = line.split()
             if not tok or tok[0].startswith("#"):

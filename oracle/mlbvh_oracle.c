@@ -188,6 +188,8 @@ void orc_decode_batch(const uint64_t* codes, uint32_t* xyz, int n)
 /* ---------------------------------------------------------------------- */
 /* AABB + Morton keys, minimum_lbvh.h:212-252, 522-553                     */
 /* ---------------------------------------------------------------------- */
+static inline float f_bits_u(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static inline uint32_t bits_u(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
 static inline void aabb_empty(orc_aabb* b)
 {
     b->lower = V(FLT_MAX_H, FLT_MAX_H, FLT_MAX_H);
@@ -198,10 +200,29 @@ static inline void aabb_extend_p(orc_aabb* b, orc_v3 p)
     b->lower = vmin(b->lower, p);
     b->upper = vmax(b->upper, p);
 }
+/* The climb's merge of a child box into the other (AABB::extend(AABB),
+   minimum_lbvh.h:205-209 with the host fminf/fmaxf), except on ties: the
+   host rule keeps the second operand, so -0 against +0 would give a sign
+   that depends on which child merges (order-dependent in the reference's GPU
+   and parallel CPU builds).  Ties keep -0 for a lower bound and +0 for an
+   upper one, which makes the merge symmetric; every other input gives the
+   host rule's bits. */
+static inline float umin(float a, float b)
+{
+    if (a < b) return a;
+    if (b < a) return b;
+    return f_bits_u(bits_u(a) | bits_u(b));
+}
+static inline float umax(float a, float b)
+{
+    if (a > b) return a;
+    if (b > a) return b;
+    return f_bits_u(bits_u(a) & bits_u(b));
+}
 static inline void aabb_extend_b(orc_aabb* b, const orc_aabb* o)
 {
-    b->lower = vmin(b->lower, o->lower);
-    b->upper = vmax(b->upper, o->upper);
+    b->lower = V(umin(b->lower.x, o->lower.x), umin(b->lower.y, o->lower.y), umin(b->lower.z, o->lower.z));
+    b->upper = V(umax(b->upper.x, o->upper.x), umax(b->upper.y, o->upper.y), umax(b->upper.z, o->upper.z));
 }
 
 void orc_scene_aabb(const orc_tri* tris, int n, orc_aabb* out) /* :522-532 */

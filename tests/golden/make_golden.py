@@ -1,10 +1,14 @@
 """Regenerates the golden vectors in this folder from the reference's own
 sobol.h and camera.h, compiled from their sources by `make -C oracle ref`
-into oracle/_ref/ref_driver.  Run in a checkout that has the reference
-sources; the .npz outputs are committed so the tests need no reference.
+into oracle/_ref/ref_driver, and copies the reference demo's Cornell box
+scene (bin/assets/cornelbox.obj, gzip-compressed, and cornelbox.mtl; CC BY
+3.0, attribution in cornelbox/LICENSE.txt) into cornelbox/.  Run in a
+checkout that has the reference sources; the outputs are committed so the
+tests need no reference.
 
     python tests/golden/make_golden.py
 """
+import gzip
 import math
 import os
 import subprocess
@@ -16,6 +20,28 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+REF_ASSETS = "/root/reference/bin/assets"
+
+
+def copy_cornell_box():
+    out = os.path.join(HERE, "cornelbox")
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(REF_ASSETS, "cornelbox.obj"), "rb") as fh:
+        data = fh.read()
+    with open(os.path.join(out, "cornelbox.obj.gz"), "wb") as fh:
+        # mtime 0: the same bytes on every run
+        with gzip.GzipFile(filename="cornelbox.obj", mode="wb", fileobj=fh, compresslevel=9, mtime=0) as gz:
+            gz.write(data)
+    with open(os.path.join(REF_ASSETS, "cornelbox.mtl"), "rb") as src, \
+            open(os.path.join(out, "cornelbox.mtl"), "wb") as dst:
+        dst.write(src.read())
+    lic = ["Scene data of the MinimumLBVH demo (bin/assets/cornelbox.obj, cornelbox.mtl),",
+           "stored here as a test fixture.  Its licence and attribution files follow.", ""]
+    for name in ("cornelbox_copyright.txt", "Head_Infinite-Scan_License.txt"):
+        with open(os.path.join(REF_ASSETS, "license", name), "r", encoding="utf-8", errors="replace") as fh:
+            lic += [f"===== {name} =====", fh.read().rstrip(), ""]
+    with open(os.path.join(out, "LICENSE.txt"), "w") as fh:
+        fh.write("\n".join(lic) + "\n")
 
 
 def run(mode, inputs, out_cols):
@@ -53,6 +79,8 @@ def main():
     out = run("camera", cams, 15)
     np.savez(os.path.join(HERE, "camera.npz"), inputs=cams, outputs=out)
     print("wrote", len(q), "sobol and", len(cams), "camera vectors")
+    copy_cornell_box()
+    print("copied the Cornell box scene")
 
 
 if __name__ == "__main__":

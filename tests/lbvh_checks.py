@@ -126,6 +126,18 @@ class _MinTable:
         return out
 
 
+def _ordered(f: np.ndarray) -> np.ndarray:
+    """float32 -> uint32 keys whose integer order is the float order with
+    -0 below +0 (NaN excluded)."""
+    b = np.ascontiguousarray(f, np.float32).view(np.uint32)
+    return np.where(b >> 31 == 1, ~b, b | np.uint32(0x80000000)).astype(np.uint32)
+
+
+def _unordered(k: np.ndarray) -> np.ndarray:
+    k = k.astype(np.uint32)
+    return np.where(k >> 31 == 1, k & np.uint32(0x7FFFFFFF), ~k).astype(np.uint32).view(np.float32)
+
+
 def ranges(d: np.ndarray):
     """lo/hi leaf range of every internal node via monotonic stacks."""
     n1 = d.shape[0]
@@ -171,12 +183,19 @@ def expected_tree(tris: np.ndarray, order: np.ndarray, d: np.ndarray):
         parent[c[internal]] = p[internal]
     root = int(am.query(np.array([0]), np.array([n1 - 1]))[0]) if n1 > 0 else 0
 
-    tmin = t.min(axis=1)[idx]
-    tmax = t.max(axis=1)[idx]
-    mn = _MinTable(tmin, np.minimum)
-    mx = _MinTable(tmax, np.maximum)
-    box0 = np.concatenate([mn.query(lo, p), mx.query(lo, p)], axis=1)
-    box1 = np.concatenate([mn.query(p + 1, hi), mx.query(p + 1, hi)], axis=1)
+    # leaf boxes: the host rule a<b?a:b over v0, v1, v2 from +-FLT_MAX
+    # (AABB::extend(float3)); subtree unions: min / max in the order where
+    # -0 < +0 (the merge's tie rule), taken on order-preserving integer keys
+    lo_b = np.full(t.shape[:1] + (3,), np.float32(3.402823466e38), np.float32)
+    hi_b = np.full(t.shape[:1] + (3,), np.float32(-3.402823466e38), np.float32)
+    for k in range(3):
+        v = t[:, k, :]
+        lo_b = np.where(lo_b < v, lo_b, v)
+        hi_b = np.where(hi_b > v, hi_b, v)
+    mn = _MinTable(_ordered(lo_b[idx]), np.minimum)
+    mx = _MinTable(_ordered(hi_b[idx]), np.maximum)
+    box0 = np.concatenate([_unordered(mn.query(lo, p)), _unordered(mx.query(lo, p))], axis=1)
+    box1 = np.concatenate([_unordered(mn.query(p + 1, hi)), _unordered(mx.query(p + 1, hi))], axis=1)
 
     nodes = np.zeros((n1, 16), np.uint32)
     nodes[:, 0] = c0.astype(np.uint32)

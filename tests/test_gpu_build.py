@@ -37,9 +37,23 @@ def test_build_use_normal_bit_exact(gpu, orc):
     assert groot == oroot and np.array_equal(gn, on)
 
 
-@pytest.mark.parametrize("case", ["all_equal", "flat_axis", "duplicates", "tiny_extent"])
+def signed_zero_triangles(n, seed):
+    """Random triangles with 30 % of their coordinates set to -0 or +0."""
+    r = np.random.default_rng(seed)
+    t = scenes.random_triangles(n, seed)
+    snap = r.random(t.shape) < 0.3
+    t[snap] = np.where(r.random(int(snap.sum())) < 0.5, np.float32(-0.0), np.float32(0.0))
+    return t
+
+
+@pytest.mark.parametrize("case", ["all_equal", "flat_axis", "duplicates", "tiny_extent", "signed_zero"])
 def test_build_degenerate(gpu, orc, case):
-    if case == "all_equal":
+    """signed_zero: merges that meet -0 and +0 keep -0 below and +0 above,
+    whichever child's thread arrives second (both climb phases: 300,001
+    leaves span many 1024-leaf blocks)."""
+    if case == "signed_zero":
+        tris = signed_zero_triangles(300_001, 12)
+    elif case == "all_equal":
         tris = np.tile(scenes.random_triangles(1, 1), (5000, 1, 1))
     elif case == "flat_axis":
         tris = scenes.random_triangles(20000, 2)
@@ -54,12 +68,21 @@ def test_build_degenerate(gpu, orc, case):
     assert groot == oroot and np.array_equal(gn, on)
 
 
-def test_build_deterministic_over_repeats(gpu):
-    tris = scenes.random_triangles(200_000, 9)
-    first, r0, _, _ = gpu_build(gpu, tris)
-    for _ in range(5):
-        again, r1, _, _ = gpu_build(gpu, tris)
-        assert r1 == r0 and np.array_equal(again, first)
+@pytest.mark.parametrize("n,reps", [(200_000, 5), (2_000_003, 40)])
+def test_build_deterministic_over_repeats(gpu, n, reps):
+    """The climb's cross-block hand-off (device-scope arrival words) gives
+    the same node array on every build, compared on the device; signed zeros
+    included, so a box merged in the other order would show."""
+    tris = signed_zero_triangles(n, 9) if n > 1_000_000 else scenes.random_triangles(n, 9)
+    t = torch.from_numpy(tris).cuda()
+    b = gpu.BVHGPUBuilder()
+    sorter = gpu.OnesweepSort()
+    b.build(t, n, 0, sorter)
+    first, r0 = b.m_internals.clone(), b.m_rootNode.clone()
+    for k in range(reps):
+        b.m_internals.zero_()
+        b.build(t, n, 0, sorter)
+        assert torch.equal(b.m_rootNode, r0) and torch.equal(b.m_internals, first), k
 
 
 def test_build_icosphere_level9_full_size(gpu, orc):

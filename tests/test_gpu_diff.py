@@ -129,7 +129,10 @@ def test_fd_1000_vertices_full_workload(gpu, orc):
     fp32 adjoint to 1e-3 (relative L2 over the 1,000 vertices); their
     Richardson extrapolation (eps 1e-4, 5e-5) removes the eps^2 truncation
     term and must match the fp64 analytic adjoint to 1e-5 and the GPU adjoint
-    to 1e-4."""
+    to 1e-4 (L2), and the GPU adjoint per vertex: at least 99 % within 1e-3,
+    none beyond 1e-2.  Per vertex, plain central differences at eps 1e-4 are
+    off by their truncation term (up to ~1e-1 of a vertex's gradient, the
+    same for the fp64 analytic adjoint), which the last check shows."""
     v, f = scenes.displaced_icosphere(7, seed=0)
     dirs = v / np.linalg.norm(v, axis=1, keepdims=True)
     vt_np = (v * scenes.displacement(dirs, 99, 0.02)[:, None].astype(np.float32)).astype(np.float32)
@@ -190,10 +193,25 @@ def test_fd_1000_vertices_full_workload(gpu, orc):
     rel_ref = np.linalg.norm(ref[pick] - rich) / nr
     rel_gpu_r = np.linalg.norm(grad[pick] - rich) / nr
     rel_gpu = np.linalg.norm(grad[pick] - fd) / np.linalg.norm(fd)
-    per = np.linalg.norm(grad[pick] - rich, axis=1) / np.maximum(np.linalg.norm(rich, axis=1), 1e-12)
+    # Per vertex (floor: 1 % of the median gradient magnitude).  Central
+    # differences at eps 1e-4 carry their O(eps^2) truncation term vertex by
+    # vertex (the fp64 analytic adjoint differs from them exactly as much as
+    # the GPU's does), so the per-vertex bar is taken against their
+    # Richardson extrapolation.
+    nr_v = np.linalg.norm(rich, axis=1)
+    floor = 0.01 * np.median(nr_v)
+    per = np.linalg.norm(grad[pick] - rich, axis=1) / np.maximum(nr_v, floor)
+    per_fd = np.linalg.norm(grad[pick] - fd, axis=1) / np.maximum(nr_v, floor)
+    per_fd_ref = np.linalg.norm(ref[pick] - fd, axis=1) / np.maximum(nr_v, floor)
     print(f"FD check, 1000 of {smooth.size} kink-free vertices: GPU vs central eps 1e-4 rel L2 {rel_gpu:.3e}; "
-          f"vs Richardson: fp64 adjoint {rel_ref:.3e}, GPU adjoint {rel_gpu_r:.3e}, median per vertex "
-          f"{np.median(per):.3e}")
+          f"vs Richardson: fp64 adjoint {rel_ref:.3e}, GPU adjoint {rel_gpu_r:.3e}; per vertex vs Richardson: "
+          f"median {np.median(per):.3e}, p99 {np.percentile(per, 99):.3e}, max {per.max():.3e}; per vertex vs "
+          f"central FD: GPU max {per_fd.max():.3e}, fp64 adjoint max {per_fd_ref.max():.3e}, "
+          f"largest gap {np.abs(per_fd - per_fd_ref).max():.3e}", flush=True)
     assert rel_gpu <= 1e-3
     assert rel_ref <= 1e-5
     assert rel_gpu_r <= 1e-4
+    assert (per <= 1e-3).mean() >= 0.99
+    assert per.max() <= 1e-2
+    # the per-vertex spread against plain central differences is FD's own
+    assert np.abs(per_fd - per_fd_ref).max() <= 2e-3

@@ -11,6 +11,8 @@ import pytest
 from minimumlbvh_amd import objscene
 
 REF_ASSET = "/root/reference/bin/assets/cornelbox.obj"
+# the same scene, committed as a fixture (tests/golden/make_golden.py)
+FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "cornelbox", "cornelbox.obj.gz")
 
 SMALL_OBJ = """# known-answer scene
 mtllib small.mtl
@@ -117,12 +119,11 @@ def test_generated_cornell_box(tmp_path, orc):
     assert orc.validate(root, nodes, d)
 
 
-@pytest.mark.skipif(not os.path.exists(REF_ASSET), reason="reference asset not present")
 def test_reference_cornell_box_asset(orc):
-    """The reference's bin/assets/cornelbox.obj: 8,858 quads (17,716
-    triangles), 9 materials in .mtl order, the ceiling light's Ke on its two
-    triangles, 8,842 quads of the head scan on defaultMat."""
-    sc = objscene.load_obj(REF_ASSET)
+    """The reference demo's cornelbox.obj (committed fixture): 8,858 quads
+    (17,716 triangles), 9 materials in .mtl order, the ceiling light's Ke on
+    its two triangles, 8,842 quads of the head scan on defaultMat."""
+    sc = objscene.load_obj(FIXTURE)
     assert sc.faces == 8858 and sc.triangles.shape[0] == 17716
     assert [m.name for m in sc.materials] == ["backWall", "ceiling", "defaultMat", "floor", "leftWall", "light",
                                               "rightWall", "shortBox", "tallBox"]
@@ -135,3 +136,15 @@ def test_reference_cornell_box_asset(orc):
     assert np.isfinite(sc.triangles).all() and np.isfinite(sc.attribs).all()
     nodes, root, _, d = orc.build(sc.triangles)
     assert orc.validate(root, nodes, d)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_ASSET), reason="reference asset not present")
+def test_fixture_is_the_reference_asset():
+    """Where the reference tree is present: the fixture decompresses to the
+    reference's own file, byte for byte, and its .mtl is the same."""
+    import gzip
+
+    with gzip.open(FIXTURE, "rb") as fh, open(REF_ASSET, "rb") as ref:
+        assert fh.read() == ref.read()
+    with open(FIXTURE.replace(".obj.gz", ".mtl"), "rb") as a, open(REF_ASSET.replace(".obj", ".mtl"), "rb") as b:
+        assert a.read() == b.read()

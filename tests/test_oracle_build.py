@@ -129,6 +129,22 @@ def test_build_degenerate_inputs(orc):
     _check_against_characterisation(orc, t)
 
 
+def test_build_signed_zero_coordinates(orc):
+    """Vertices on the coordinate planes written as -0 or +0 (the reference
+    Cornell box's OBJ has both): node boxes take -0 as the lower and +0 as
+    the upper bound wherever a merge meets both, so the serial climb, the
+    parallel one and the characterisation give the same bits."""
+    r = np.random.default_rng(41)
+    t = scenes.random_triangles(20000, 41)
+    snap = r.random(t.shape) < 0.3
+    t[snap] = np.where(r.random(int(snap.sum())) < 0.5, np.float32(-0.0), np.float32(0.0))
+    nodes, root, d = _check_against_characterisation(orc, t)
+    assert (nodes[:, 2:14] == 0x80000000).any() and (nodes[:, 2:14] == 0).any()
+    for threads in (1, 3, 8):
+        b = orc.build(t, parallel=True, nthreads=threads)
+        assert np.array_equal(b[0], nodes) and b[1] == root
+
+
 def test_build_serial_equals_parallel(orc):
     tris = scenes.random_triangles(65536, 0)
     a = orc.build(tris, parallel=True)
