@@ -111,6 +111,18 @@ MLBVH_API int mlbvh_build(mlbvh_ctx* ctx, const void* d_triangles, int n, uint32
 MLBVH_API int mlbvh_build_ex(mlbvh_ctx* ctx, const void* d_triangles, int n, uint32_t buildOption, void* d_nodes, void* d_root,
                    uint64_t* d_sorted_records, uint8_t* d_deltas, void* stream);
 
+/* mlbvh_build_ex without any device allocation.  No reference counterpart:
+ * it exists for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Results are bit-identical
+ * to mlbvh_build_ex.  Its scratch is one static device array per GPU (about
+ * 44 MiB, held from library load to process exit), set up by
+ * mlbvh_ctx_create; n is limited to 2^20 triangles (MLBVH_E_INVALID_ARG
+ * above).  All calls on one GPU share that array: calls that overlap in time
+ * on one GPU race, so order them on one stream. */
+MLBVH_API int mlbvh_build_ex_noalloc(mlbvh_ctx* ctx, const void* d_triangles, int n, uint32_t buildOption,
+                                     void* d_nodes, void* d_root, uint64_t* d_sorted_records, uint8_t* d_deltas,
+                                     void* stream);
+
 /* Refit: recompute every node AABB for moved vertices with the topology
  * fixed by an earlier build's sorted records and deltas (re-runs the
  * agglomerative climb of minimum_lbvh.h:284-361 on unchanged deltas).  No

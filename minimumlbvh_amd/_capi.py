@@ -38,6 +38,7 @@ SIGNATURES = {
     "mlbvh_sort_u64": ([_P, _P, _P, _P, _P, c_uint32, c_int, c_int, _P], c_int),
     "mlbvh_build": ([_P, _P, c_int, c_uint32, _P, _P, _P], c_int),
     "mlbvh_build_ex": ([_P, _P, c_int, c_uint32, _P, _P, _P, _P, _P], c_int),
+    "mlbvh_build_ex_noalloc": ([_P, _P, c_int, c_uint32, _P, _P, _P, _P, _P], c_int),
     "mlbvh_refit": ([_P, _P, c_int, _P, _P, _P, _P, _P], c_int),
     "mlbvh_build_indexed": ([_P, _P, c_int, _P, c_int, c_uint32, _P, _P, _P, _P, _P], c_int),
     "mlbvh_refit_indexed": ([_P, _P, c_int, _P, c_int, _P, _P, _P, _P, _P], c_int),

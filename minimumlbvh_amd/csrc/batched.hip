@@ -437,12 +437,12 @@ extern "C" int mlbvh_build_batched(mlbvh_ctx* ctx, const void* d_triangles, cons
     int rc = set_device(ctx);
     if (rc) return rc;
     size_t total = totalTriangles;
-    if ((rc = ensure(&ctx->recA, &ctx->recCap, total, "hipMalloc(records)"))) return rc;
-    if ((rc = ensure(&ctx->recB, &ctx->recBCap, total, "hipMalloc(records tmp)"))) return rc;
-    if ((rc = ensure(&ctx->deltas, &ctx->deltasCap, total, "hipMalloc(deltas)"))) return rc;
-    if ((rc = ensure(&ctx->flags, &ctx->flagsCap, total, "hipMalloc(flags)"))) return rc;
-    if ((rc = ensure(&ctx->meshBox, &ctx->meshBoxCap, (size_t)nMeshes * 6, "hipMalloc(meshBox)"))) return rc;
-    if ((rc = ensure(&ctx->frontier, &ctx->frontierCap, total * 4, "hipMalloc(climb frontier)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->recA, &ctx->recCap, total, "hipMalloc(records)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->recB, &ctx->recBCap, total, "hipMalloc(records tmp)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->deltas, &ctx->deltasCap, total, "hipMalloc(deltas)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->flags, &ctx->flagsCap, total, "hipMalloc(flags)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->meshBox, &ctx->meshBoxCap, (size_t)nMeshes * 6, "hipMalloc(meshBox)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->frontier, &ctx->frontierCap, total * 4, "hipMalloc(climb frontier)"))) return rc;
     if ((rc = ensure_status(ctx, sort_tiles(ctx, totalTriangles)))) return rc;
     unsigned g = (unsigned)((total + kBatchBlock - 1) / kBatchBlock);
 // grid-stride key kernel: 8 blocks per CU built the 4096-mesh batch in

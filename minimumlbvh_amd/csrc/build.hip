@@ -438,7 +438,7 @@ template <class TS>
 static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, const uint8_t* deltas,
                         Node* nodes, uint32_t* root, hipStream_t stream)
 {
-    int rc = ensure(&ctx->frontier, &ctx->frontierCap, (size_t)n * 3, "hipMalloc(climb frontier)");
+    int rc = ensure(ctx, &ctx->frontier, &ctx->frontierCap, (size_t)n * 3, "hipMalloc(climb frontier)");
     if (rc) return rc;
     unsigned blocks = (unsigned)(((size_t)n + kLocalLeaves - 1) / kLocalLeaves);
     k_climb_local<<<blocks, kLocalLeaves, 0, stream>>>(tris, (uint32_t)n, rec, deltas, nodes, ctx->frontier,
@@ -475,10 +475,10 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
     int rc = set_device(ctx);
     if (rc) return rc;
     size_t nn = (size_t)n;
-    if ((rc = ensure(&ctx->recA, &ctx->recCap, nn, "hipMalloc(records)"))) return rc;
-    if ((rc = ensure(&ctx->recB, &ctx->recBCap, nn, "hipMalloc(records tmp)"))) return rc;
-    if ((rc = ensure(&ctx->deltas, &ctx->deltasCap, nn, "hipMalloc(deltas)"))) return rc;
-    if ((rc = ensure(&ctx->flags, &ctx->flagsCap, nn, "hipMalloc(flags)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->recA, &ctx->recCap, nn, "hipMalloc(records)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->recB, &ctx->recBCap, nn, "hipMalloc(records tmp)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->deltas, &ctx->deltasCap, nn, "hipMalloc(deltas)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->flags, &ctx->flagsCap, nn, "hipMalloc(flags)"))) return rc;
 
     Node* nodes = (Node*)d_nodes;
     uint32_t* root = (uint32_t*)d_root;
@@ -490,8 +490,8 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
     k_scene_aabb<<<gAabb, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox);
     unsigned gMorton = g < (unsigned)ctx->cuCount * 4 ? g : (unsigned)ctx->cuCount * 4;
     if (wide) {
-        if ((rc = ensure(&ctx->idxA, &ctx->idxCap, nn, "hipMalloc(indices)"))) return rc;
-        if ((rc = ensure(&ctx->idxB, &ctx->idxBCap, nn, "hipMalloc(indices tmp)"))) return rc;
+        if ((rc = ensure(ctx, &ctx->idxA, &ctx->idxCap, nn, "hipMalloc(indices)"))) return rc;
+        if ((rc = ensure(ctx, &ctx->idxB, &ctx->idxBCap, nn, "hipMalloc(indices tmp)"))) return rc;
         k_morton64<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, ctx->recB, ctx->idxA, ctx->flags,
                                                         ctx->ctrl);
         // keys in recB (tmp: recA), records written to recA afterwards
@@ -563,7 +563,7 @@ static int refit_impl(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* d_sorted_r
     }
     int rc = set_device(ctx);
     if (rc) return rc;
-    if ((rc = ensure(&ctx->flags, &ctx->flagsCap, (size_t)n, "hipMalloc(flags)"))) return rc;
+    if ((rc = ensure(ctx, &ctx->flags, &ctx->flagsCap, (size_t)n, "hipMalloc(flags)"))) return rc;
     unsigned g = grid1((size_t)n);
     k_climb_reset<<<g, kBuildBlock, 0, s>>>(ctx->flags, n - 1, ctx->ctrl);
     if ((rc = launch_climb(ctx, tris, n, d_sorted_records, d_deltas, (Node*)d_nodes, (uint32_t*)d_root, s)))

@@ -70,16 +70,17 @@ extern "C" int mlbvh_ctx_create(int device, mlbvh_ctx** out)
     ctx->device = device;
     ctx->cuCount = prop.multiProcessorCount;
     int rc = set_device(ctx);
+    if (!rc) rc = noalloc_setup(device);  // the allocation-free build's scratch, once per device
     size_t one = 0, cap = 0;
-    if (!rc) rc = ensure(&ctx->hist, &one, (size_t)8 * 256, "hipMalloc(hist)");
+    if (!rc) rc = ensure(ctx, &ctx->hist, &one, (size_t)8 * 256, "hipMalloc(hist)");
     cap = 0;
-    if (!rc) rc = ensure(&ctx->ctrl, &cap, (size_t)kCtrlWords, "hipMalloc(ctrl)");
+    if (!rc) rc = ensure(ctx, &ctx->ctrl, &cap, (size_t)kCtrlWords, "hipMalloc(ctrl)");
     if (!rc) {
         hipError_t e = hipMemset(ctx->ctrl, 0, kCtrlWords * sizeof(uint32_t));
         if (e != hipSuccess) rc = hip_status(e, "hipMemset(ctrl)");
     }
     cap = 0;
-    if (!rc) rc = ensure(&ctx->sceneBox, &cap, (size_t)256, "hipMalloc(sceneBox)");
+    if (!rc) rc = ensure(ctx, &ctx->sceneBox, &cap, (size_t)256, "hipMalloc(sceneBox)");
     if (rc) {
         mlbvh_ctx_destroy(ctx);
         return rc;

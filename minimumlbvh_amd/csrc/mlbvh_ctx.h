@@ -42,6 +42,11 @@ struct mlbvh_ctx {
     size_t meshBoxCap = 0;
     int traceBlocksPerCU = 8;  // persistent trace grid = cuCount * this
     int traceChunk = 64;       // rays a wave takes per work-queue pull (multiple of 64)
+
+    // Set on a context view whose buffers are carved from fixed device
+    // scratch (the allocation-free build): ensure() then never frees or
+    // allocates and reports a buffer that is too small as an error.
+    bool fixedScratch = false;
 };
 
 namespace mlbvh {
@@ -87,9 +92,14 @@ __device__ inline void sort_state_init(uint32_t* ctrl, int nPass, uint32_t* hist
 
 // Grow a device buffer to hold at least `need` elements (contents dropped).
 template <class T>
-int ensure(T** p, size_t* cap, size_t need, const char* what)
+int ensure(const mlbvh_ctx* ctx, T** p, size_t* cap, size_t need, const char* what)
 {
     if (need <= *cap && *p) return MLBVH_OK;
+    if (ctx->fixedScratch) {
+        set_error("%s: %zu elements needed, the fixed scratch holds %zu (no allocation in this entry point)", what,
+                  need, *cap);
+        return MLBVH_E_INVALID_ARG;
+    }
     if (*p) {
         hipError_t e = hipFree(*p);
         *p = nullptr;
@@ -130,6 +140,10 @@ int ensure_status(mlbvh_ctx* ctx, uint32_t tiles);
 int ensure_partials(mlbvh_ctx* ctx, int blocks, int nPass);
 void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream, int firstHist = 0,
                         const uint32_t* guard = nullptr);
+
+// Resolve and zero the allocation-free build's device scratch (noalloc.hip);
+// once per device, with that device current.
+int noalloc_setup(int device);
 
 // One plain build (the body of mlbvh_build_ex).
 int build_one(mlbvh_ctx* ctx, const void* d_tris, int n, uint32_t opt, void* d_nodes, void* d_root,

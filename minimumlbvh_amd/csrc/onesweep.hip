@@ -556,7 +556,7 @@ int ensure_status(mlbvh_ctx* ctx, uint32_t tiles)
 {
     size_t need = (size_t)tiles * 256;
     if (need <= ctx->statusWords && ctx->status) return MLBVH_OK;
-    int rc = ensure(&ctx->status, &ctx->statusWords, need, "hipMalloc(look-back status)");
+    int rc = ensure(ctx, &ctx->status, &ctx->statusWords, need, "hipMalloc(look-back status)");
     if (rc) return rc;
     hipError_t e = hipMemset(ctx->status, 0, ctx->statusWords * sizeof(uint64_t));
     if (e != hipSuccess) return hip_status(e, "hipMemset(status)");
@@ -565,7 +565,7 @@ int ensure_status(mlbvh_ctx* ctx, uint32_t tiles)
 
 int ensure_partials(mlbvh_ctx* ctx, int blocks, int nPass)
 {
-    return ensure(&ctx->partials, &ctx->partialsCap, (size_t)blocks * nPass * 256, "hipMalloc(histogram partials)");
+    return ensure(ctx, &ctx->partials, &ctx->partialsCap, (size_t)blocks * nPass * 256, "hipMalloc(histogram partials)");
 }
 
 template <class K, bool kVals, int B, int I>
