@@ -74,8 +74,26 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_calib(verbose)
     build_facade_demo(verbose)
     return LIB
+
+
+CALIB_SRC = os.path.join(CSRC, "calib", "probes.hip")
+CALIB_LIB = os.path.join(LIBDIR, "libmlbvh_calib.so")
+
+
+def build_calib(verbose: bool = False) -> str:
+    """The roofline calibration probes (not part of libmlbvh's ABI)."""
+    tmp = CALIB_LIB + ".tmp"
+    cmd = [hipcc(), *HIPFLAGS, "-shared", CALIB_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"calibration probes build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CALIB_LIB)
+    if verbose:
+        print(f"built {CALIB_LIB}")
+    return CALIB_LIB
 
 
 FACADE_SRC = os.path.join(ROOT, "tests", "facade", "facade_demo.hip")

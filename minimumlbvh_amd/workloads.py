@@ -9,6 +9,7 @@ profiler can wrap exactly one of them:
     python -m minimumlbvh_amd.workloads walks    [--reps K]   # escape-link / indexed trace, indexed / 63-bit build
     python -m minimumlbvh_amd.workloads pt       [--reps K]   # 21M-tri sphere field: build + 1080p 8 spp PT
     python -m minimumlbvh_amd.workloads copy     [--reps K]   # 1 GiB device copy (counter calibration)
+    python -m minimumlbvh_amd.workloads gather   [--reps K]   # random 64-B node gathers (traversal roof)
 
 Scenes:
   * ao / build: procedural icosphere of subdivision level 9 (5,242,880
@@ -166,6 +167,64 @@ def events_ms(fn, reps: int, warmup: int = 1, stream=None):
     e.record(stream)
     torch.cuda.synchronize()
     return s.elapsed_time(e) / reps
+
+
+_calib = None
+
+
+def calib_lib():
+    """The roofline probes of csrc/calib/probes.hip (libmlbvh_calib.so)."""
+    global _calib
+    if _calib is None:
+        import ctypes
+        import os
+
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libmlbvh_calib.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run `python -m minimumlbvh_amd._build` first")
+        L = ctypes.CDLL(path)
+        L.mlbvh_calib_gather.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.mlbvh_calib_gather.restype = ctypes.c_int
+        _calib = L
+    return _calib
+
+
+# Gather probes: (name, table bytes, dependent, blocks per CU).  2 MiB fits
+# in every XCD's 4 MiB L2; 256 MiB is the size of the MI355X's Infinity
+# Cache (MALL); 4 GiB exceeds both (HBM).  The dependent probe at 5 blocks of
+# 256 lanes per CU is the AO kernel's residency (20 waves per CU).
+GATHER_PROBES = [("l2_independent", 2 << 20, 0, 8), ("l2_dependent_ao_occupancy", 2 << 20, 1, 5),
+                 ("mall_independent", 256 << 20, 0, 8), ("hbm_independent", 4 << 30, 0, 8),
+                 ("l2_quad_cooperative", 2 << 20, 2, 8), ("mall_quad_cooperative", 256 << 20, 2, 8)]
+
+
+def run_gather(reps: int, out: dict, iters: int = 256, probes=None):
+    """Random 64-B node gathers (four 16-B loads each, like the traversal's
+    node fetch) from tables of several sizes: GB/s of 64-B node fetches."""
+    import ctypes
+
+    L = calib_lib()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    stream = torch.cuda.current_stream()
+    for name, nbytes, dep, bpc in probes or GATHER_PROBES:
+        nodes = nbytes // 64
+        table = torch.randint(0, 2**31 - 1, (nbytes // 4,), dtype=torch.int32, device="cuda:0")
+        grid = cus * bpc
+        sink = torch.empty(grid * 256, dtype=torch.int32, device="cuda:0")
+
+        def run():
+            rc = L.mlbvh_calib_gather(ctypes.c_void_p(table.data_ptr()), nodes, iters, dep,
+                                      ctypes.c_void_p(sink.data_ptr()), grid, ctypes.c_void_p(stream.cuda_stream))
+            if rc != 0:
+                raise RuntimeError(f"mlbvh_calib_gather failed ({rc})")
+
+        ms = events_ms(run, reps, warmup=2, stream=stream)
+        fetched = grid * 256 * iters * 64
+        out[name] = {"table_bytes": nbytes, "dependent": bool(dep), "lanes": grid * 256, "fetches_per_lane": iters,
+                     "ms": round(ms, 4), "gbs": round(fetched / (ms / 1e3) / 1e9, 1)}
+        del table, sink
+    return out
 
 
 def run_batched(reps: int, out: dict, graph: bool = True):
@@ -327,8 +386,9 @@ def run_diff(steps: int, out: dict):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("workload", choices=["ao", "build", "sort", "batched", "diff", "walks", "pt", "copy"])
+    ap.add_argument("workload", choices=["ao", "build", "sort", "batched", "diff", "walks", "pt", "copy", "gather"])
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--probe", default=None, help="gather: run only this probe (names in GATHER_PROBES)")
     args = ap.parse_args(argv)
     res = {"workload": args.workload}
     if args.workload in ("ao", "build"):
@@ -357,6 +417,9 @@ def main(argv=None):
         run_walks(args.reps, res)
     elif args.workload == "pt":
         run_pt(args.reps, res)
+    elif args.workload == "gather":
+        pick = [p for p in GATHER_PROBES if args.probe in (None, p[0])]
+        run_gather(args.reps, res, probes=pick)
     else:
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
         b = torch.empty_like(a)

@@ -377,7 +377,10 @@ int orc_build(const orc_tri* tris, int n, uint32_t opt, orc_node* nodes, uint32_
         orc_morton64_sorted(tris, n, &scene, rec, deltas);
     } else {
         orc_morton_records(tris, n, &scene, opt, rec);
-        orc_sort_records(rec, n);
+        if (opt & ORC_OPT_COMPARISON_SORT)
+            orc_sort_records_comparison(rec, n);
+        else
+            orc_sort_records(rec, n);
         orc_deltas(rec, n, deltas);
     }
     if (opt & ORC_OPT_CPU_PARALLEL) {

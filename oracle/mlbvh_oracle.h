@@ -53,8 +53,14 @@ typedef struct {                                          /* main_gpu.cu:10-15 *
 #define ORC_OPT_USE_NORMAL 1u
 #define ORC_OPT_CPU_PARALLEL 2u
 #define ORC_OPT_MORTON64 4u /* no reference counterpart: full 63-bit keys */
+/* sort the records with the reference CPU builder's std::sort on the key
+   alone (minimum_lbvh.h:554; unstable) instead of the stable radix sort:
+   the timed CPU baseline in the reference's own structure */
+#define ORC_OPT_COMPARISON_SORT 8u
 #define ORC_QUERY_CLOSEST 0
 #define ORC_QUERY_ANY 1
+
+void orc_sort_records_comparison(uint64_t* records, int n); /* sort_comparison.cpp */
 
 /* bit utilities, minimum_lbvh.h:99-205 */
 int orc_clz(uint32_t x);

@@ -145,6 +145,29 @@ def test_build_signed_zero_coordinates(orc):
         assert np.array_equal(b[0], nodes) and b[1] == root
 
 
+def test_comparison_sort_build_reference_structure(orc):
+    """The CPU-baseline build (reference structure: std::sort on the key
+    alone, minimum_lbvh.h:554) gives a valid LBVH over the same sorted keys;
+    with all keys distinct the order cannot differ from the stable sort, so
+    the tree is the radix build's, bit for bit.  With ties, only the order of
+    equal-key leaves may differ."""
+    t = scenes.random_triangles(65536, 0)
+    a = orc.build(t, opt=0)
+    b = orc.build(t, opt=8)
+    keys = a[2] & 0xFFFFFFFF
+    assert np.unique(keys).size == keys.size
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1] and np.array_equal(a[2], b[2])
+    t = scenes.random_triangles(20000, 4)
+    t[::3] = t[5]  # many equal keys
+    a = orc.build(t, opt=0)
+    b = orc.build(t, opt=8, parallel=False)
+    assert np.array_equal(a[2] & 0xFFFFFFFF, b[2] & 0xFFFFFFFF)
+    assert np.array_equal(np.sort(a[2]), np.sort(b[2]))
+    assert np.array_equal(a[3], b[3])
+    C.structural_checks(b[0], b[1], t.shape[0])
+    assert orc.validate(b[1], b[0], b[3])
+
+
 def test_build_serial_equals_parallel(orc):
     tris = scenes.random_triangles(65536, 0)
     a = orc.build(tris, parallel=True)
