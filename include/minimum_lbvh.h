@@ -4,14 +4,17 @@
 // GPU path: same namespace, types (identical layouts), host+device inline
 // functions and the BVHGPUBuilder class, compiled offline with hipcc instead
 // of through Orochi/orortc.  BVHGPUBuilder runs on libmlbvh.so (include/mlbvh.h).
-// The CPU builder BVHCPUBuilder and the Embree builder of the reference are
-// not part of this GPU library.
+// BVHCPUBuilder builds on the host with the same build_lbvh (leaf-parallel
+// under OpenMP when compiled with -fopenmp); the reference's Embree builder is
+// not provided.  The functions restate the reference API functions (cited
+// file:line) with the reference's signatures.
 //
 // Arithmetic: every inline function evaluates in the reference's operation
 // order; compile with -ffp-contract=off to keep results bit-identical to the
 // reference's host arithmetic (HIP's fp32 divide/sqrt are correctly rounded by
-// default).  AABB unions use the reference's host min/max (a<b?a:b), slab
-// tests the hardware min/max (equal for the NaN-free values of finite rays).
+// default).  Point-into-box extension uses the reference's host min/max
+// (a<b?a:b); box-into-box merges keep -0 below +0 (see umin); slab tests use
+// the hardware min/max (equal for the NaN-free values of finite rays).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,6 +22,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "mlbvh.h"
 #include "tinyhiponesweep.h"
@@ -122,10 +128,6 @@ MINIMUM_LBVH_DEVICE inline T ss_min(T x, T y)
 {
     return (y < x) ? y : x;
 }
-MINIMUM_LBVH_DEVICE inline float remap(float value, float inputMin, float inputMax, float outputMin, float outputMax)
-{
-    return (value - inputMin) * ((outputMax - outputMin) / (inputMax - inputMin)) + outputMin;
-}
 MINIMUM_LBVH_DEVICE inline float compMin(float3 v) { return fminf(fminf(v.x, v.y), v.z); }
 MINIMUM_LBVH_DEVICE inline float compMax(float3 v) { return fmaxf(fmaxf(v.x, v.y), v.z); }
 
@@ -190,20 +192,12 @@ MINIMUM_LBVH_DEVICE inline uint64_t encodeMortonCode(uint32_t x, uint32_t y, uin
 {
     return splat3(x) | splat3(y) << 1 | splat3(z) << 2;
 }
-MINIMUM_LBVH_DEVICE inline uint32_t encode32Morton2D(uint16_t x, uint16_t y)
-{
-    uint64_t res = x | (uint64_t(y) << 32);
-    res = (res | (res << 8)) & 0x00ff00ff00ff00ffull;
-    res = (res | (res << 4)) & 0x0f0f0f0f0f0f0f0full;
-    res = (res | (res << 2)) & 0x3333333333333333ull;
-    res = (res | (res << 1)) & 0x5555555555555555ull;
-    return uint32_t(res | (res >> 31));
-}
 
 // minimum_lbvh.h:207-282 (layouts are the ABI)
 struct Triangle {
     float3 vs[3];
 };
+// minimum_lbvh.h:212-215
 MINIMUM_LBVH_DEVICE inline float3 normalOf(const Triangle& tri)
 {
     return normalize(cross(sub(tri.vs[1], tri.vs[0]), sub(tri.vs[2], tri.vs[0])));
@@ -517,6 +511,68 @@ MINIMUM_LBVH_DEVICE inline void intersect_escape_link(Hit* hit, const InternalNo
         curr_node = next_node;
     }
 }
+
+// BVHCPUBuilder, minimum_lbvh.h:509-669 (without the Embree builder): the
+// host build over build_lbvh.  Records are ordered by (morton, index), the
+// stable order the GPU builder produces; the reference's std::sort on the
+// key alone leaves equal keys in an unspecified order.  With
+// BUILD_OPTION_CPU_PARALLEL the leaves climb in parallel when the caller is
+// compiled with OpenMP, else one after another (the same tree either way).
+// n < 2 aborts (the reference writes out of bounds for n == 1).
+class BVHCPUBuilder {
+public:
+    void build(const Triangle* triangles, int nTriangles, uint32_t buildOption)
+    {
+        MINIMUM_LBVH_ASSERT(2 <= nTriangles);
+        m_internals.assign((size_t)nTriangles - 1, InternalNode());
+        for (InternalNode& node : m_internals) node.context = 0xFFFFFFFF;
+        m_deltas.assign((size_t)nTriangles - 1, 0);
+
+        AABB sceneAABB;
+        sceneAABB.setEmpty();
+        for (int i = 0; i < nTriangles; i++)
+            for (int j = 0; j < 3; ++j) sceneAABB.extend(triangles[i].vs[j]);
+
+        std::vector<IndexedMorton> mortons((size_t)nTriangles);
+        for (int i = 0; i < nTriangles; i++) {
+            const Triangle& tri = triangles[i];
+            float3 center = div(add(add(tri.vs[0], tri.vs[1]), tri.vs[2]), 3.0f);
+            uint32_t pMorton = (uint32_t)(sceneAABB.encodeMortonCode(center) >> 31);
+            if (buildOption & BUILD_OPTION_USE_NORMAL) {
+                AABB unit;
+                unit.lower = v3(-1.0f);
+                unit.upper = v3(1.0f);
+                uint32_t nMorton = (uint32_t)(unit.encodeMortonCode(normalOf(tri)) >> 31);
+                mortons[i].morton = (pMorton & 0xFFFFFFC0) | (nMorton >> 26);
+            } else {
+                mortons[i].morton = pMorton;
+            }
+            mortons[i].index = (uint32_t)i;
+        }
+        std::sort(mortons.begin(), mortons.end(), [](IndexedMorton a, IndexedMorton b) {
+            return a.morton != b.morton ? a.morton < b.morton : a.index < b.index;
+        });
+        for (size_t i = 0; i < m_deltas.size(); i++) m_deltas[i] = (uint8_t)delta(mortons[i].morton, mortons[i + 1].morton);
+
+        const int n = nTriangles;
+        if (buildOption & BUILD_OPTION_CPU_PARALLEL) {
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < n; i++)
+                build_lbvh(&m_rootNode, m_internals.data(), triangles, n, m_deltas.data(), (uint32_t)i,
+                           mortons[i].index);
+        } else {
+            for (int i = 0; i < n; i++)
+                build_lbvh(&m_rootNode, m_internals.data(), triangles, n, m_deltas.data(), (uint32_t)i,
+                           mortons[i].index);
+        }
+    }
+    bool empty() const { return m_internals.empty(); }
+    void validate() const { validate_lbvh(m_rootNode, m_internals.data(), m_deltas.data(), 255); }
+
+    NodeIndex m_rootNode;
+    std::vector<InternalNode> m_internals;
+    std::vector<uint8_t> m_deltas;
+};
 
 // DeviceStopwatch, minimum_lbvh.h:693-724 (hipEvent pair).
 class DeviceStopwatch {

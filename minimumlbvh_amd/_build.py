@@ -76,6 +76,7 @@ def build(verbose: bool = False) -> str:
         print(f"built {LIB}")
     build_calib(verbose)
     build_facade_demo(verbose)
+    build_facade_cpu(verbose)
     return LIB
 
 
@@ -94,6 +95,24 @@ def build_calib(verbose: bool = False) -> str:
     if verbose:
         print(f"built {CALIB_LIB}")
     return CALIB_LIB
+
+
+FACADE_CPU_SRC = os.path.join(ROOT, "tests", "facade", "facade_cpu.cpp")
+FACADE_CPU_BIN = os.path.join(ROOT, "tests", "facade", "_build", "facade_cpu")
+
+
+def build_facade_cpu(verbose: bool = False) -> str:
+    """The host-side facade test program (BVHCPUBuilder and the inline
+    traversals on the CPU); needs no GPU to run."""
+    os.makedirs(os.path.dirname(FACADE_CPU_BIN), exist_ok=True)
+    cmd = [hipcc(), *[f for f in HIPFLAGS if f != "-fvisibility=hidden"], "-fopenmp", FACADE_CPU_SRC, "-o",
+           FACADE_CPU_BIN, f"-L{LIBDIR}", "-lmlbvh", "-Wl,-rpath,$ORIGIN/../../../minimumlbvh_amd/_lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade CPU program build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {FACADE_CPU_BIN}")
+    return FACADE_CPU_BIN
 
 
 FACADE_SRC = os.path.join(ROOT, "tests", "facade", "facade_demo.hip")

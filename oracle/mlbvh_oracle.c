@@ -245,6 +245,17 @@ uint64_t orc_aabb_encode_morton(const orc_aabb* box, orc_v3 p)
     return orc_encode_morton((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
 }
 
+void orc_points_morton(const float* pts, int n, float* boxOut, uint64_t* codes) /* scene box of the points, then
+                                                                                  AABB::encodeMortonCode of each */
+{
+    orc_aabb box;
+    aabb_empty(&box);
+    for (int i = 0; i < n; i++) aabb_extend_p(&box, V(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+    for (int i = 0; i < n; i++) codes[i] = orc_aabb_encode_morton(&box, V(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+    boxOut[0] = box.lower.x, boxOut[1] = box.lower.y, boxOut[2] = box.lower.z;
+    boxOut[3] = box.upper.x, boxOut[4] = box.upper.y, boxOut[5] = box.upper.z;
+}
+
 /* records: low 32 bits = morton key, high 32 bits = triangle index, the
    little-endian view of IndexedMorton{morton, index}.  minimum_lbvh.h:534-553 */
 void orc_morton_records(const orc_tri* tris, int n, const orc_aabb* scene, uint32_t opt, uint64_t* records)

@@ -61,6 +61,8 @@ typedef struct {                                          /* main_gpu.cu:10-15 *
 #define ORC_QUERY_ANY 1
 
 void orc_sort_records_comparison(uint64_t* records, int n); /* sort_comparison.cpp */
+/* the box of n points (AABB::extend), then AABB::encodeMortonCode of each point, minimum_lbvh.h:246-251 */
+void orc_points_morton(const float* pts, int n, float* boxOut, uint64_t* codes);
 
 /* bit utilities, minimum_lbvh.h:99-205 */
 int orc_clz(uint32_t x);

@@ -79,6 +79,7 @@ def lib():
             "orc_exp2_fin": ([c_float], c_float),
             "orc_pow_pos": ([c_float, c_float], c_float),
             "orc_sincos_batch": ([P, c_int, P, P], None),
+            "orc_points_morton": ([P, c_int, P, P], None),
             "orc_pow_batch": ([P, c_int, c_float, P], None),
             "orc_sobol_2d": ([POINTER(c_float), POINTER(c_float), c_uint32, c_uint32, c_uint32, c_uint32], None),
             "orc_raygen_lookat": ([P, V3, V3, V3, c_float, c_int, c_int], None),
@@ -231,6 +232,15 @@ def sincos(x):
     s, c = c_float(), c_float()
     lib().orc_sincos(x, ctypes.byref(s), ctypes.byref(c))
     return np.float32(s.value), np.float32(c.value)
+
+
+def points_morton(pts):
+    """(box (6,), 63-bit codes) of AABB::encodeMortonCode over the points' box."""
+    pts = _f32(pts).reshape(-1, 3)
+    box = np.zeros(6, np.float32)
+    codes = np.zeros(pts.shape[0], np.uint64)
+    lib().orc_points_morton(_p(pts), pts.shape[0], _p(box), _p(codes))
+    return box, codes
 
 
 def sincos_batch(x):
