@@ -80,7 +80,8 @@ MINIMUM_LBVH_DEVICE inline float umin(float a, float b)
     uint32_t u = ua | ub;
     float t;
     memcpy(&t, &u, 4);
-    return a < b ? a : (b < a ? b : t);
+    const float m = a < b ? a : b;
+    return a == b ? t : m;
 }
 MINIMUM_LBVH_DEVICE inline float umax(float a, float b)
 {
@@ -90,7 +91,8 @@ MINIMUM_LBVH_DEVICE inline float umax(float a, float b)
     uint32_t u = ua & ub;
     float t;
     memcpy(&t, &u, 4);
-    return a > b ? a : (b > a ? b : t);
+    const float m = a > b ? a : b;
+    return a == b ? t : m;
 }
 // float -> int truncation with the reference host's (x86 cvttss2si) rule
 MINIMUM_LBVH_DEVICE inline int f2i(float f)

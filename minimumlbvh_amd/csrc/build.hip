@@ -206,18 +206,6 @@ __global__ void __launch_bounds__(kBuildBlock)
     }
 }
 
-// delta(a, b) = 32 - clz(a ^ b) of neighbouring sorted keys
-// (minimum_lbvh.cu:94-102).
-__global__ void __launch_bounds__(kBuildBlock)
-    k_deltas(const uint64_t* __restrict__ rec, int n, uint8_t* __restrict__ deltas)
-{
-    int i = blockIdx.x * kBuildBlock + threadIdx.x;
-    if (i < n - 1) {
-        uint32_t a = (uint32_t)rec[i], b = (uint32_t)rec[i + 1];
-        deltas[i] = (uint8_t)(32 - __clz(a ^ b));
-    }
-}
-
 // Agglomerative climb (build_lbvh, minimum_lbvh.h:284-361) in two phases.
 //
 // Internal node p (the split between sorted leaves p and p+1) covers the
@@ -246,11 +234,15 @@ __global__ void __launch_bounds__(kBuildBlock)
 #endif
 constexpr int kLocalLeaves = MLBVH_CLIMB_LEAVES;
 
-template <class TS>
+// kFuseDeltas: the deltas come from the block's own sorted records
+// (delta(a, b) = 32 - clz(a ^ b) of neighbouring keys, minimum_lbvh.cu:94-102)
+// and the block writes d[b0 .. b1] to `deltasOut` for the upper phase and the
+// caller; otherwise they are read from `deltas` (63-bit keys, refit).
+template <class TS, bool kFuseDeltas>
 __global__ void __launch_bounds__(kLocalLeaves)
     k_climb_local(TS tris, uint32_t n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
-                  Node* __restrict__ nodes, uint32_t* __restrict__ frontier, uint32_t* __restrict__ ctrl,
-                  uint32_t* __restrict__ root)
+                  uint8_t* __restrict__ deltasOut, Node* __restrict__ nodes, uint32_t* __restrict__ frontier,
+                  uint32_t* __restrict__ ctrl, uint32_t* __restrict__ root)
 {
     constexpr int kWaves = kLocalLeaves / 64;
     // Local delta j is d[b0 - 1 + j], j in [0, m] (m = leaves in the block).
@@ -273,14 +265,32 @@ __global__ void __launch_bounds__(kLocalLeaves)
     const uint32_t b1 = b0 + m - 1;
     const uint32_t nInternals = n - 1;
 
+    const uint32_t pos = b0 + (uint32_t)t;
+    const uint64_t myRec = (uint32_t)t < m ? rec[pos] : 0ull;
     // deltas that exist and lie in [b0-1, b1]; others read as 0 (they never
     // decide a comparison: the b0 == 0 / b1 == n-1 cases are tested apart)
-    auto local_delta = [&](uint32_t j) -> uint32_t {
-        uint32_t g = b0 + j;  // global delta index + 1
-        return (j <= m && g >= 1 && g - 1 < nInternals) ? (uint32_t)deltas[g - 1] : 0u;
-    };
-    uint32_t e0 = local_delta((uint32_t)t);      // prefix element j = t
-    uint32_t e1 = local_delta((uint32_t)t + 1);  // suffix element j = t + 1
+    uint32_t e0, e1;  // local deltas j = t (prefix element) and j = t + 1 (suffix element)
+    if (kFuseDeltas) {
+        __shared__ uint32_t s_k[kLocalLeaves + 2];  // s_k[j] = key of sorted position b0 - 1 + j
+        if ((uint32_t)t < m) s_k[t + 1] = (uint32_t)myRec;
+        if (t == 0 && b0 > 0) s_k[0] = (uint32_t)rec[b0 - 1];
+        if ((uint32_t)t == m - 1 && b0 + m < n) s_k[m + 1] = (uint32_t)rec[b0 + m];
+        __syncthreads();
+        auto fused_delta = [&](uint32_t j) -> uint32_t {
+            const uint32_t g = b0 + j;  // global delta index + 1
+            return (j <= m && g >= 1 && g - 1 < nInternals) ? (uint32_t)(32 - __clz(s_k[j] ^ s_k[j + 1])) : 0u;
+        };
+        e0 = fused_delta((uint32_t)t);
+        e1 = fused_delta((uint32_t)t + 1);
+        if ((uint32_t)t < m && pos < nInternals) deltasOut[pos] = (uint8_t)e1;  // d[b0 + t]
+    } else {
+        auto local_delta = [&](uint32_t j) -> uint32_t {
+            uint32_t g = b0 + j;  // global delta index + 1
+            return (j <= m && g >= 1 && g - 1 < nInternals) ? (uint32_t)deltas[g - 1] : 0u;
+        };
+        e0 = local_delta((uint32_t)t);
+        e1 = local_delta((uint32_t)t + 1);
+    }
     s_d[t] = (uint8_t)e0;
     if (t == kLocalLeaves - 1) s_d[kLocalLeaves] = (uint8_t)e1;
     s_flag[t] = 0xFFFFFFFFu;
@@ -313,11 +323,10 @@ __global__ void __launch_bounds__(kLocalLeaves)
     }
     __syncthreads();
 
-    const uint32_t pos = b0 + (uint32_t)t;
     bool rootHere = false, emit = false;
     uint32_t node = 0, lo = pos, hi = pos;
     if ((uint32_t)t < m) {
-        const uint32_t triIndex = (uint32_t)(rec[pos] >> 32);
+        const uint32_t triIndex = (uint32_t)(myRec >> 32);
         Box box = tri_box(tris[triIndex]);
         node = triIndex | kLeafBit;
         while (true) {
@@ -434,15 +443,21 @@ __global__ void __launch_bounds__(kBuildBlock) k_climb_reset(uint32_t* flags, in
     if (i == 0) ctrl[kCtrlFrontier] = 0;
 }
 
+// fuseDeltas: compute the deltas from `rec` in the local phase and write
+// them to `deltas` (the 32-bit build); else read them from `deltas`.
 template <class TS>
-static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, const uint8_t* deltas,
+static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uint8_t* deltas, bool fuseDeltas,
                         Node* nodes, uint32_t* root, hipStream_t stream)
 {
     int rc = ensure(ctx, &ctx->frontier, &ctx->frontierCap, (size_t)n * 3, "hipMalloc(climb frontier)");
     if (rc) return rc;
     unsigned blocks = (unsigned)(((size_t)n + kLocalLeaves - 1) / kLocalLeaves);
-    k_climb_local<<<blocks, kLocalLeaves, 0, stream>>>(tris, (uint32_t)n, rec, deltas, nodes, ctx->frontier,
-                                                        ctx->ctrl, root);
+    if (fuseDeltas)
+        k_climb_local<TS, true><<<blocks, kLocalLeaves, 0, stream>>>(tris, (uint32_t)n, rec, nullptr, deltas, nodes,
+                                                                     ctx->frontier, ctx->ctrl, root);
+    else
+        k_climb_local<TS, false><<<blocks, kLocalLeaves, 0, stream>>>(tris, (uint32_t)n, rec, deltas, nullptr, nodes,
+                                                                      ctx->frontier, ctx->ctrl, root);
     // one lane per expected entry (about n/64 for surface meshes), at most 8
     // blocks per CU; lanes stride over a larger list
     size_t want = ((size_t)n / 64 + kBuildBlock - 1) / kBuildBlock + 1;
@@ -509,9 +524,9 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         launch_hist_reduce(ctx, (int)gm, 4, stream);
         rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts);
         if (rc) return rc;
-        k_deltas<<<g, kBuildBlock, 0, stream>>>(ctx->recA, n, ctx->deltas);
     }
-    if ((rc = launch_climb(ctx, tris, n, ctx->recA, ctx->deltas, nodes, root, stream))) return rc;
+    // 32-bit keys: the local climb computes the deltas from the records
+    if ((rc = launch_climb(ctx, tris, n, ctx->recA, ctx->deltas, !wide, nodes, root, stream))) return rc;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_status(e, "build kernel launch");
     if (d_sorted) {
@@ -566,7 +581,8 @@ static int refit_impl(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* d_sorted_r
     if ((rc = ensure(ctx, &ctx->flags, &ctx->flagsCap, (size_t)n, "hipMalloc(flags)"))) return rc;
     unsigned g = grid1((size_t)n);
     k_climb_reset<<<g, kBuildBlock, 0, s>>>(ctx->flags, n - 1, ctx->ctrl);
-    if ((rc = launch_climb(ctx, tris, n, d_sorted_records, d_deltas, (Node*)d_nodes, (uint32_t*)d_root, s)))
+    if ((rc = launch_climb(ctx, tris, n, d_sorted_records, const_cast<uint8_t*>(d_deltas), false, (Node*)d_nodes,
+                           (uint32_t*)d_root, s)))
         return rc;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_status(e, "refit kernel launch");

@@ -79,13 +79,18 @@ __device__ inline f3 hmax3(f3 a, f3 b) { return mk(hmax(a.x, b.x), hmax(a.y, b.y
 // parallel CPU builds are order-dependent there).  Ties keep -0 for a lower
 // bound and +0 for an upper one (bitwise OR / AND of equal values): every
 // other input gives the host rule's bits, and the merge is symmetric.
+// (Written as two selects: the nested form compiled to branches.)
 __device__ inline float umin(float a, float b)
 {
-    return a < b ? a : (b < a ? b : __uint_as_float(__float_as_uint(a) | __float_as_uint(b)));
+    const float m = a < b ? a : b;
+    const float tie = __uint_as_float(__float_as_uint(a) | __float_as_uint(b));
+    return a == b ? tie : m;
 }
 __device__ inline float umax(float a, float b)
 {
-    return a > b ? a : (b > a ? b : __uint_as_float(__float_as_uint(a) & __float_as_uint(b)));
+    const float m = a > b ? a : b;
+    const float tie = __uint_as_float(__float_as_uint(a) & __float_as_uint(b));
+    return a == b ? tie : m;
 }
 __device__ inline f3 umin3(f3 a, f3 b) { return mk(umin(a.x, b.x), umin(a.y, b.y), umin(a.z, b.z)); }
 __device__ inline f3 umax3(f3 a, f3 b) { return mk(umax(a.x, b.x), umax(a.y, b.y), umax(a.z, b.z)); }
