@@ -210,6 +210,7 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ uint32_t s_gbase[256];
     __shared__ uint32_t s_tmp[4];
     __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_hist[256];  // this tile's digit counts
 
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -219,6 +220,7 @@ __global__ void __launch_bounds__(kBlock)
 
     if (t == 0) s_tile = atomicAdd(ticket, 1u);
     for (int i = t; i < kWaves * 256; i += kBlock) (&s_wcnt[0][0])[i] = 0;
+    if (t < 256) s_hist[t] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint64_t tileBase = (uint64_t)tile * kTile;
@@ -237,6 +239,21 @@ __global__ void __launch_bounds__(kBlock)
             key[k] = 0;
             if (kVals) val[k] = 0;
         }
+    }
+
+    // The tile's aggregate goes out as soon as its keys are in: counted with
+    // LDS atomics before the ranking, so that successors looking back find it
+    // one ranking earlier.  Tile 0 publishes its inclusive prefix directly.
+    if (!preScanned) {
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            uint64_t idx = waveBase + (uint64_t)k * 64 + lane;
+            if (idx < n) atomicAdd(&s_hist[digit_of(key[k], shift)], 1u);
+        }
+        __syncthreads();
+        if (t < 256)
+            __hip_atomic_store(status + (uint64_t)tile * 256 + t, lb_pack(epoch, tile == 0 ? kFlagP : kFlagA, s_hist[t]),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 
     // Wave64 multisplit: lanes whose 8-bit digits agree are found with eight
@@ -282,10 +299,7 @@ __global__ void __launch_bounds__(kBlock)
         uint32_t excl = 0;
         if (preScanned) {  // k_tile_scan has published every tile's inclusive prefix
             if (tile != 0) excl = (uint32_t)status[(uint64_t)t * gridDim.x + tile - 1];
-        } else if (tile == 0) {
-            __hip_atomic_store(my, lb_pack(epoch, kFlagP, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(my, lb_pack(epoch, kFlagA, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (tile != 0) {  // the aggregate (tot) went out before the ranking
             int64_t j = (int64_t)tile - 1;
             while (true) {
                 const uint64_t* src = status + (uint64_t)j * 256 + t;
