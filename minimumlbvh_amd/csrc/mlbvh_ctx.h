@@ -90,7 +90,24 @@ __device__ inline void sort_state_init(uint32_t* ctrl, int nPass, uint32_t* hist
     if (threadIdx.x == 0) ctrl[kCtrlSortFallback] = 0;
 }
 
+// A failed scratch allocation: under stream capture the message says what
+// to do instead (a graph cannot record hipMalloc / hipFree).
+inline int alloc_status(hipError_t e, const char* what)
+{
+    if (e == hipErrorStreamCaptureUnsupported || e == hipErrorStreamCaptureInvalidated ||
+        e == hipErrorStreamCaptureImplicit || e == hipErrorCapturedEvent) {
+        set_error("%s: the call needs more scratch than this context holds, and a stream is being captured (%s); "
+                  "run the same call once outside capture at the largest size first, or use "
+                  "mlbvh_build_ex_noalloc",
+                  what, hipGetErrorString(e));
+        return MLBVH_E_INVALID_ARG;
+    }
+    return hip_status(e, what);
+}
+
 // Grow a device buffer to hold at least `need` elements (contents dropped).
+// Calls at a size the context has already served allocate nothing, so they
+// may be captured in a graph.
 template <class T>
 int ensure(const mlbvh_ctx* ctx, T** p, size_t* cap, size_t need, const char* what)
 {
@@ -104,13 +121,13 @@ int ensure(const mlbvh_ctx* ctx, T** p, size_t* cap, size_t need, const char* wh
         hipError_t e = hipFree(*p);
         *p = nullptr;
         *cap = 0;
-        if (e != hipSuccess) return hip_status(e, what);
+        if (e != hipSuccess) return alloc_status(e, what);
     }
     size_t n = need < 1 ? 1 : need;
     hipError_t e = hipMalloc((void**)p, n * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
-        return hip_status(e, what);
+        return alloc_status(e, what);
     }
     *cap = n;
     return MLBVH_OK;

@@ -148,12 +148,13 @@ def test_batched_build_trace_graph_replays(gpu):
         assert np.array_equal(bb.roots.cpu().numpy().view(np.uint32), roots), rep
 
 
-def test_batched_full_configuration(gpu):
+def test_batched_full_configuration(gpu, orc):
     """The batching configuration at full size (4096 meshes, about 1.3e8
     triangles and 4.8e7 rays, generated on the device): one segmented build +
     persistent trace gives the same nodes, roots and hits, bit for bit, as the
-    plain per-mesh build + trace loop (each of which the other tests check
-    against the oracle)."""
+    plain per-mesh build + trace loop, and, for the six largest meshes (up to
+    262,144 triangles) and six others drawn at random, as the oracle's build
+    and trace of that mesh alone."""
     from minimumlbvh_amd import workloads
 
     sc = workloads.BatchedScene()
@@ -169,3 +170,20 @@ def test_batched_full_configuration(gpu):
     assert torch.equal(sc.bvh.nodes, nodes)
     assert torch.equal(loop_hits, hits)
     assert (hits[:, 7] != -1).float().mean().item() > 0.05
+    # against the oracle, mesh by mesh, at the sizes only the full batch has
+    sizes = np.diff(sc.toff.astype(np.int64))
+    pick = list(np.argsort(sizes)[-6:]) + list(np.random.default_rng(4).choice(sc.n_meshes, 6, replace=False))
+    roots_np = roots.cpu().numpy().view(np.uint32)
+    for m in pick:
+        t0, t1 = int(sc.toff[m]), int(sc.toff[m + 1])
+        r0, r1 = int(sc.roff[m]), int(sc.roff[m + 1])
+        mt = sc.tris[t0:t1].cpu().numpy()
+        on, oroot, _, _ = orc.build(mt)
+        print(f"batched full size vs oracle: mesh {m}, {t1 - t0} triangles, {r1 - r0} rays", flush=True)
+        assert roots_np[m] == oroot, m
+        assert np.array_equal(nodes[t0 - m: t1 - m - 1].cpu().numpy().view(np.uint32), on), m
+        mr = sc.rays[r0:r1].cpu().numpy().copy()
+        mr[:, 7] = 0
+        oh, _ = orc.trace(on, mt, oroot, mr)
+        assert np.array_equal(hits[r0:r1].cpu().numpy().view(np.uint32), oh.view(np.uint32).reshape(-1, 8)), m
+    assert sizes[pick[5]] > 250_000
