@@ -43,65 +43,70 @@ class V3(ctypes.Structure):
 _lib = None
 
 
+def load(path: str) -> ctypes.CDLL:
+    """The oracle library at `path` with its function signatures set."""
+    L = ctypes.CDLL(path)
+    P = c_void_p
+    sig = {
+        "orc_clz": ([c_uint32], c_int), "orc_clz64": ([c_uint64], c_int), "orc_ffs": ([c_uint32], c_int),
+        "orc_ffs64": ([c_uint64], c_int), "orc_delta32": ([c_uint32, c_uint32], c_int),
+        "orc_splat3": ([c_uint32], c_uint64), "orc_compact3": ([c_uint64], c_uint32),
+        "orc_encode_morton": ([c_uint32, c_uint32, c_uint32], c_uint64),
+        "orc_morton_batch": ([P, P, c_int, c_int], None),
+        "orc_decode_batch": ([P, P, c_int], None),
+        "orc_f2i_trunc": ([c_float], c_int),
+        "orc_scene_aabb": ([P, c_int, P], None),
+        "orc_morton_records": ([P, c_int, P, c_uint32, P], None),
+        "orc_sort_records": ([P, c_int], None),
+        "orc_deltas": ([P, c_int, P], None),
+        "orc_build_leaf": ([P, P, P, c_int, P, c_uint32, c_uint32], None),
+        "orc_build": ([P, c_int, c_uint32, P, P, P, P, c_int], c_int),
+        "orc_validate": ([c_uint32, P, P, c_int], c_int),
+        "orc_radix_sort_u32": ([P, P, c_uint32, c_int, c_int], c_int),
+        "orc_radix_sort_u64": ([P, P, c_uint32, c_int, c_int], c_int),
+        "orc_trace": ([P, P, c_uint32, P, c_int, c_int, c_int, P, P, c_int], None),
+        "orc_brute_force": ([P, c_int, P, c_int, P], None),
+        "orc_hash_pcg": ([c_uint32], c_uint32),
+        "orc_pcg32": ([c_uint64, c_uint64, c_int, P], None),
+        "orc_pcg32_uniformf": ([c_uint64, c_uint64, c_int, P], None),
+        "orc_hemisphere_cos": ([c_float, c_float, P], None),
+        "orc_onb": ([P, P], None),
+        "orc_cosine_dir": ([P, c_float, c_float, P], None),
+        "orc_sincos": ([c_float, POINTER(c_float), POINTER(c_float)], None),
+        "orc_log2_pos": ([c_float], c_float),
+        "orc_exp2_fin": ([c_float], c_float),
+        "orc_pow_pos": ([c_float, c_float], c_float),
+        "orc_sincos_batch": ([P, c_int, P, P], None),
+        "orc_points_morton": ([P, c_int, P, P], None),
+        "orc_pow_batch": ([P, c_int, c_float, P], None),
+        "orc_sobol_2d": ([POINTER(c_float), POINTER(c_float), c_uint32, c_uint32, c_uint32, c_uint32], None),
+        "orc_raygen_lookat": ([P, V3, V3, V3, c_float, c_int, c_int], None),
+        "orc_raygen_shoot": ([P, P, P, c_float, c_float], None),
+        "orc_pack_rgba": ([c_float, c_float, c_float, c_float], c_uint32),
+        "orc_render_normal": ([P, c_int, c_int, P, c_uint32, P, P, c_int], None),
+        "orc_render_ao": ([P, P, c_int, c_int, P, c_uint32, P, P, c_int, c_int, c_float, c_uint32, c_int,
+                           c_int, P, c_int, c_int], None),
+        "orc_render_pt": ([P, c_int, c_int, P, c_uint32, P, P, P, c_int, c_int, c_int, c_int], None),
+        "orc_any_hit_stack_batch": ([P, P, c_uint32, P, c_int, c_int, P, P, c_int], None),
+        "orc_closest_hit_stack_batch": ([P, P, c_uint32, P, c_int, c_int, c_int, P, P, c_int], None),
+        "orc_pack": ([P, P, c_int], None),
+        "orc_primary_eval64": ([P, P, P, c_int, c_int, P, P, P], None),
+        "orc_primary_grad64": ([P, P, P, c_int, c_int, P, P, P, P], None),
+        "orc_now": ([], c_double),
+        "orc_max_threads": ([], c_int),
+    }
+    for name, (a, r) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = a
+        f.restype = r
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         build_lib()
-        L = ctypes.CDLL(LIB)
-        P = c_void_p
-        sig = {
-            "orc_clz": ([c_uint32], c_int), "orc_clz64": ([c_uint64], c_int), "orc_ffs": ([c_uint32], c_int),
-            "orc_ffs64": ([c_uint64], c_int), "orc_delta32": ([c_uint32, c_uint32], c_int),
-            "orc_splat3": ([c_uint32], c_uint64), "orc_compact3": ([c_uint64], c_uint32),
-            "orc_encode_morton": ([c_uint32, c_uint32, c_uint32], c_uint64),
-            "orc_morton_batch": ([P, P, c_int, c_int], None),
-            "orc_decode_batch": ([P, P, c_int], None),
-            "orc_f2i_trunc": ([c_float], c_int),
-            "orc_scene_aabb": ([P, c_int, P], None),
-            "orc_morton_records": ([P, c_int, P, c_uint32, P], None),
-            "orc_sort_records": ([P, c_int], None),
-            "orc_deltas": ([P, c_int, P], None),
-            "orc_build_leaf": ([P, P, P, c_int, P, c_uint32, c_uint32], None),
-            "orc_build": ([P, c_int, c_uint32, P, P, P, P, c_int], c_int),
-            "orc_validate": ([c_uint32, P, P, c_int], c_int),
-            "orc_radix_sort_u32": ([P, P, c_uint32, c_int, c_int], c_int),
-            "orc_radix_sort_u64": ([P, P, c_uint32, c_int, c_int], c_int),
-            "orc_trace": ([P, P, c_uint32, P, c_int, c_int, c_int, P, P, c_int], None),
-            "orc_brute_force": ([P, c_int, P, c_int, P], None),
-            "orc_hash_pcg": ([c_uint32], c_uint32),
-            "orc_pcg32": ([c_uint64, c_uint64, c_int, P], None),
-            "orc_pcg32_uniformf": ([c_uint64, c_uint64, c_int, P], None),
-            "orc_hemisphere_cos": ([c_float, c_float, P], None),
-            "orc_onb": ([P, P], None),
-            "orc_cosine_dir": ([P, c_float, c_float, P], None),
-            "orc_sincos": ([c_float, POINTER(c_float), POINTER(c_float)], None),
-            "orc_log2_pos": ([c_float], c_float),
-            "orc_exp2_fin": ([c_float], c_float),
-            "orc_pow_pos": ([c_float, c_float], c_float),
-            "orc_sincos_batch": ([P, c_int, P, P], None),
-            "orc_points_morton": ([P, c_int, P, P], None),
-            "orc_pow_batch": ([P, c_int, c_float, P], None),
-            "orc_sobol_2d": ([POINTER(c_float), POINTER(c_float), c_uint32, c_uint32, c_uint32, c_uint32], None),
-            "orc_raygen_lookat": ([P, V3, V3, V3, c_float, c_int, c_int], None),
-            "orc_raygen_shoot": ([P, P, P, c_float, c_float], None),
-            "orc_pack_rgba": ([c_float, c_float, c_float, c_float], c_uint32),
-            "orc_render_normal": ([P, c_int, c_int, P, c_uint32, P, P, c_int], None),
-            "orc_render_ao": ([P, P, c_int, c_int, P, c_uint32, P, P, c_int, c_int, c_float, c_uint32, c_int,
-                               c_int, P, c_int, c_int], None),
-            "orc_render_pt": ([P, c_int, c_int, P, c_uint32, P, P, P, c_int, c_int, c_int, c_int], None),
-            "orc_any_hit_stack_batch": ([P, P, c_uint32, P, c_int, c_int, P, P, c_int], None),
-            "orc_closest_hit_stack_batch": ([P, P, c_uint32, P, c_int, c_int, c_int, P, P, c_int], None),
-            "orc_pack": ([P, P, c_int], None),
-            "orc_primary_eval64": ([P, P, P, c_int, c_int, P, P, P], None),
-            "orc_primary_grad64": ([P, P, P, c_int, c_int, P, P, P, P], None),
-            "orc_now": ([], c_double),
-            "orc_max_threads": ([], c_int),
-        }
-        for name, (a, r) in sig.items():
-            f = getattr(L, name)
-            f.argtypes = a
-            f.restype = r
-        _lib = L
+        _lib = load(LIB)
     return _lib
 
 

@@ -239,15 +239,19 @@ def _render_ao(orc, tris, cam, w, h, n_sample, radius):
 
 
 @pytest.mark.parametrize("n_sample", [4, 16])
-def test_ao_closed_box_is_fully_occluded(orc, n_sample):
+@pytest.mark.parametrize("winding", ["as_built", "reversed"])
+def test_ao_closed_box_is_fully_occluded(orc, n_sample, winding):
     """Inside the closed cube [-1, 1]^3 every primary ray hits and, with a
     radius above the cube's diagonal, every AO ray hits a wall: ao = N * 1/N
-    = 1 exactly and the pixel is black.  This fails if the normal were not
-    turned toward the ray (the AO origin would sit 0.001 outside the box and
-    every ray would leave it)."""
+    = 1 exactly and the pixel is black, whichever way the walls are wound.
+    With the walls' normals pointing out of the box this fails if the normal
+    were not turned toward the ray (the AO origin would sit 0.001 outside the
+    box and every ray would leave it)."""
     from minimumlbvh_amd import scenes
 
     tris, _ = scenes.furnace_box()
+    if winding == "reversed":
+        tris = np.ascontiguousarray(tris[:, ::-1])
     px, ao = _render_ao(orc, tris, scenes.CAMERA_FURNACE, 40, 32, n_sample, 4.0)
     assert np.all(ao == 1.0) and np.all(px == 0)
 
