@@ -153,14 +153,12 @@ def main():
         if i >= args.warmup:
             sort_ms.append(s.elapsed_time(e))
     sort_avg = sum(sort_ms) / len(sort_ms)
-    # Two byte models.  The 8-pass LSD model, (2P+1)*8*N with P = 8, is the one
-    # the 60 % target is stated in (<= 0.951 ms at N = 2^25).  The bytes this
-    # sort actually moves on its large-input route: one read for the three top
-    # digits' histograms, three digit passes (read + write) and the segment
-    # finish (read + write) = 9 * 8 * N.
+    # The fraction is taken over the bytes this sort actually moves on its
+    # large-input route: one read for the three top digits' histograms, three
+    # digit passes (read + write) and the segment finish (read + write) =
+    # 9 * 8 * N.  An 8-pass LSD sort would move (2P+1)*8*N with P = 8; at 60 %
+    # of peak that is 0.951 ms at N = 2^25, reported as a time bound only.
     P = 8
-    sort_bytes = (2 * P + 1) * 8 * N
-    sort_gbs = sort_bytes / (sort_avg / 1e3) / 1e9
     run_bytes = (1 + 2 * 3 + 2) * 8 * N
     run_gbs = run_bytes / (sort_avg / 1e3) / 1e9
     del keys0, keys, tmp
@@ -255,13 +253,14 @@ def main():
             "route": "3 top-digit onesweep passes (the first reads tile prefixes scanned from the histogram read, "
                      "no look-back) + segment finish (full 8-pass fallback only for long runs of "
                      "equal 24-bit prefix)",
-            "target_ms": round((2 * P + 1) * 8 * N / (0.6 * HBM_PEAK_GBS * 1e9) * 1e3, 4),
-            "lsd8_algorithmic_bytes": sort_bytes,
-            "lsd8_achieved_gbs": round(sort_gbs, 1),
-            "lsd8_frac": round(sort_gbs / HBM_PEAK_GBS, 4),
-            "run_algorithmic_bytes": run_bytes,
-            "run_achieved_gbs": round(run_gbs, 1),
-            "run_frac": round(run_gbs / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": run_bytes,
+            "achieved_gbs": round(run_gbs, 1),
+            "frac": round(run_gbs / HBM_PEAK_GBS, 4),
+            "bytes_model": "bytes the route moves: 2 x 8 B per key for each of the 3 passes and the finish, "
+                           "plus the 8-B histogram read",
+            "lsd8_equivalent_ms": round((2 * P + 1) * 8 * N / (0.6 * HBM_PEAK_GBS * 1e9) * 1e3, 4),
+            "lsd8_equivalent_note": "time an 8-pass LSD sort would need at 60 % of 8 TB/s; the route sorts in "
+                                    "fewer passes, so this is a time bound, not a bandwidth fraction",
             "tile": "x".join(str(v) for v in autotune.SORT_CONFIGS[best_cfg]),
         },
         "batched": batched or None,
