@@ -217,13 +217,25 @@ __global__ void __launch_bounds__(kBuildBlock)
 // which both of p's children evaluate alike from a prefix and a suffix
 // maximum of the block's deltas.
 //
-// Phase 1 (k_climb_local): a block of kLocalLeaves threads climbs its leaves
-// through such block-local nodes, handing the sibling bounds over in LDS.  A
-// thread whose next parent is not block-local keeps (node, lo, hi) for the
-// frontier list and stops.  The block-local nodes are assembled in LDS
-// (children, boxes, parent, context) and written at the end as whole 64-byte
-// nodes with coalesced 16-byte stores; a block-local node whose parent is not
-// block-local gets its parent word from phase 2.
+// Phase 1 (k_climb_local): a block of kLocalLeaves threads builds its
+// block-local nodes without climbing, each node on its own.  For internal
+// node p, Lp and Rp come from two nearest-greater searches (binary lifting
+// over a sparse table of the block's deltas in LDS).  Its parent is Lp - 1
+// if d[Lp - 1] < d[Rp], else Rp (the climb's direction rule), and every
+// node and leaf writes its index into its parent's child slot.  Its two
+// child boxes are the unions of the leaf boxes over [Lp, p] and [p + 1, Rp],
+// read from a segment tree of the block's leaf boxes in LDS.  The box merge
+// (umin / umax: min and max under a total order that puts -0 below +0) is
+// associative and commutative, so these unions equal the climb's pairwise
+// merges bit for bit.  The tree holds each coordinate as its order-
+// preserving integer key (f2ord), on which umin / umax are integer min / max
+// for every non-NaN value: a box merge is six integer instructions.  A leaf or block-local node whose parent is not
+// block-local goes to the frontier list with its range: the block's own
+// stretch of the list, entries b0 .. b0 + m - 1 (a block has at most one
+// entry per leaf), closed by an end mark when shorter.  Each block-local
+// node is written whole (64 bytes) by its thread; one whose parent is not
+// block-local gets its parent word again from phase 2.  The block's barriers
+// order LDS only, so its global stores are never waited for.
 //
 // Phase 2 (k_climb_upper): the frontier entries (about 1% of the nodes for
 // surface meshes) climb on through the device-scope arrival words
@@ -233,6 +245,27 @@ __global__ void __launch_bounds__(kBuildBlock)
 #define MLBVH_CLIMB_LEAVES 1024
 #endif
 constexpr int kLocalLeaves = MLBVH_CLIMB_LEAVES;
+constexpr uint32_t kDeltaInf = 255;  // the missing d[-1] and d[n-1]: above every delta
+constexpr int log2_exact(int v) { return v <= 1 ? 0 : 1 + log2_exact(v / 2); }
+constexpr int kLiftLevels = log2_exact(kLocalLeaves) + 1;  // windows of 1 .. kLocalLeaves deltas
+constexpr uint32_t kFrontierEnd = 0xFFFFFFFFu;  // no node or leaf index (n < 2^31)
+
+// A box as order-preserving keys of its six coordinates.
+struct KeyBox {
+    uint32_t lo[3], hi[3];
+};
+__device__ inline KeyBox key_box(const Box& b)
+{
+    return {{f2ord(b.lo.x), f2ord(b.lo.y), f2ord(b.lo.z)}, {f2ord(b.hi.x), f2ord(b.hi.y), f2ord(b.hi.z)}};
+}
+__device__ inline void key_merge(KeyBox& a, const KeyBox& b)
+{
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        a.lo[k] = a.lo[k] < b.lo[k] ? a.lo[k] : b.lo[k];
+        a.hi[k] = a.hi[k] > b.hi[k] ? a.hi[k] : b.hi[k];
+    }
+}
 
 // kFuseDeltas: the deltas come from the block's own sorted records
 // (delta(a, b) = 32 - clz(a ^ b) of neighbouring keys, minimum_lbvh.cu:94-102)
@@ -242,195 +275,230 @@ template <class TS, bool kFuseDeltas>
 __global__ void __launch_bounds__(kLocalLeaves)
     k_climb_local(TS tris, uint32_t n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
                   uint8_t* __restrict__ deltasOut, Node* __restrict__ nodes, uint32_t* __restrict__ frontier,
-                  uint32_t* __restrict__ ctrl, uint32_t* __restrict__ root)
+                  uint32_t* __restrict__ root)
 {
+    static_assert(kLocalLeaves == 1 << (kLiftLevels - 1) && kLocalLeaves >= 256 && kLocalLeaves <= 1024,
+                  "a power-of-two block of 256 .. 1024 leaves");
     constexpr int kWaves = kLocalLeaves / 64;
-    // Local delta j is d[b0 - 1 + j], j in [0, m] (m = leaves in the block).
-    __shared__ uint8_t s_d[kLocalLeaves + 1];
-    __shared__ uint8_t s_pm[kLocalLeaves];      // s_pm[j] = max s_d[0 .. j]
-    __shared__ uint8_t s_sm[kLocalLeaves + 1];  // s_sm[j] = max s_d[j .. kLocalLeaves]
-    __shared__ uint8_t s_loc[kLocalLeaves];     // internal node b0 + j lies inside the block (the predicate above)
-    __shared__ uint32_t s_wmax[2][kWaves];
-    __shared__ uint32_t s_flag[kLocalLeaves];   // arrival word of local node b0 + slot
-    __shared__ Box s_box[kLocalLeaves][2];      // the node's child boxes
+    constexpr int kRow = kLocalLeaves + 16;
+    // Local delta j is d[b0 - 1 + j], j in [0, m]; s_st[k][j] = max of the
+    // local deltas j .. j + 2^k - 1.
+    __shared__ uint8_t s_st[kLiftLevels][kRow];
+    // Segment tree of the leaf boxes: leaf j at kLocalLeaves + j, node i the
+    // union of nodes 2i and 2i + 1.
+    __shared__ KeyBox s_tree[2 * kLocalLeaves];
     __shared__ uint32_t s_child[kLocalLeaves][2];
-    __shared__ uint32_t s_parent[kLocalLeaves];
-    __shared__ uint32_t s_context[kLocalLeaves];
+    __shared__ uint8_t s_loc[kLocalLeaves];  // internal node b0 + j lies inside the block
     __shared__ uint32_t s_wcount[kWaves];
-    __shared__ uint32_t s_base;
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t b0 = blockIdx.x * kLocalLeaves;
     const uint32_t m = n - b0 < (uint32_t)kLocalLeaves ? n - b0 : (uint32_t)kLocalLeaves;
-    const uint32_t b1 = b0 + m - 1;
     const uint32_t nInternals = n - 1;
-
     const uint32_t pos = b0 + (uint32_t)t;
-    const uint64_t myRec = (uint32_t)t < m ? rec[pos] : 0ull;
-    // deltas that exist and lie in [b0-1, b1]; others read as 0 (they never
-    // decide a comparison: the b0 == 0 / b1 == n-1 cases are tested apart)
-    uint32_t e0, e1;  // local deltas j = t (prefix element) and j = t + 1 (suffix element)
+    const bool isLeaf = (uint32_t)t < m;
+    const bool isNode = (uint32_t)t + 1 < m;  // internal node pos: the split between leaves pos and pos + 1
+
+    const uint64_t myRec = isLeaf ? rec[pos] : 0ull;
+    const uint32_t triIndex = (uint32_t)(myRec >> 32);
+    Box leafBox;
+    if (isLeaf) {
+        leafBox = tri_box(tris[triIndex]);
+    } else {
+        leafBox.lo = mk(kFltMax, kFltMax, kFltMax);
+        leafBox.hi = mk(-kFltMax, -kFltMax, -kFltMax);
+    }
+    s_tree[kLocalLeaves + t] = key_box(leafBox);
+
     if (kFuseDeltas) {
         __shared__ uint32_t s_k[kLocalLeaves + 2];  // s_k[j] = key of sorted position b0 - 1 + j
-        if ((uint32_t)t < m) s_k[t + 1] = (uint32_t)myRec;
+        if (isLeaf) s_k[t + 1] = (uint32_t)myRec;
         if (t == 0 && b0 > 0) s_k[0] = (uint32_t)rec[b0 - 1];
         if ((uint32_t)t == m - 1 && b0 + m < n) s_k[m + 1] = (uint32_t)rec[b0 + m];
-        __syncthreads();
-        auto fused_delta = [&](uint32_t j) -> uint32_t {
+        lds_barrier();
+        auto delta_at = [&](uint32_t j) -> uint32_t {
             const uint32_t g = b0 + j;  // global delta index + 1
-            return (j <= m && g >= 1 && g - 1 < nInternals) ? (uint32_t)(32 - __clz(s_k[j] ^ s_k[j + 1])) : 0u;
+            return (g == 0 || g - 1 >= nInternals) ? kDeltaInf : (uint32_t)(32 - __clz(s_k[j] ^ s_k[j + 1]));
         };
-        e0 = fused_delta((uint32_t)t);
-        e1 = fused_delta((uint32_t)t + 1);
-        if ((uint32_t)t < m && pos < nInternals) deltasOut[pos] = (uint8_t)e1;  // d[b0 + t]
+        s_st[0][t] = (uint8_t)delta_at((uint32_t)t);
+        if (t == kLocalLeaves - 1) s_st[0][kLocalLeaves] = (uint8_t)delta_at(kLocalLeaves);
     } else {
-        auto local_delta = [&](uint32_t j) -> uint32_t {
-            uint32_t g = b0 + j;  // global delta index + 1
-            return (j <= m && g >= 1 && g - 1 < nInternals) ? (uint32_t)deltas[g - 1] : 0u;
+        auto delta_at = [&](uint32_t j) -> uint32_t {
+            const uint32_t g = b0 + j;
+            return (g == 0 || g - 1 >= nInternals) ? kDeltaInf : (uint32_t)deltas[g - 1];
         };
-        e0 = local_delta((uint32_t)t);
-        e1 = local_delta((uint32_t)t + 1);
+        s_st[0][t] = (uint8_t)delta_at((uint32_t)t);
+        if (t == kLocalLeaves - 1) s_st[0][kLocalLeaves] = (uint8_t)delta_at(kLocalLeaves);
     }
-    s_d[t] = (uint8_t)e0;
-    if (t == kLocalLeaves - 1) s_d[kLocalLeaves] = (uint8_t)e1;
-    s_flag[t] = 0xFFFFFFFFu;
-    s_parent[t] = kInvalidNode;
-    uint32_t pm = e0, sm = e1;
+    lds_barrier();
+    // Two levels per step: level k from k - 1, level k + 1 from the four
+    // windows of level k - 1 (the box merge is exact, so the order does not
+    // matter).
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t up = __shfl_up(pm, o, 64);
-        uint32_t dn = __shfl_down(sm, o, 64);
-        if (lane >= o) pm = pm > up ? pm : up;
-        if (lane + o < 64) sm = sm > dn ? sm : dn;
+    for (int k = 1; k < kLiftLevels; k += 2) {
+        const int h = 1 << (k - 1);
+        if (t + 2 * h - 1 <= kLocalLeaves) {
+            const uint32_t a = s_st[k - 1][t], b = s_st[k - 1][t + h];
+            s_st[k][t] = (uint8_t)(a > b ? a : b);
+        }
+        if (k + 1 < kLiftLevels && t + 4 * h - 1 <= kLocalLeaves) {
+            uint32_t a = s_st[k - 1][t], b = s_st[k - 1][t + h], c = s_st[k - 1][t + 2 * h], d = s_st[k - 1][t + 3 * h];
+            a = a > b ? a : b;
+            c = c > d ? c : d;
+            s_st[k + 1][t] = (uint8_t)(a > c ? a : c);
+        }
+        const int first = kLocalLeaves >> k;  // level k of the tree: nodes first .. 2 first - 1
+        if (t < first) {
+            KeyBox u = s_tree[2 * (first + t)];
+            key_merge(u, s_tree[2 * (first + t) + 1]);
+            s_tree[first + t] = u;
+        } else if (t >= kLocalLeaves / 2 && t - kLocalLeaves / 2 < first / 2) {  // level k + 1 from level k - 1
+            const int i = first / 2 + (t - kLocalLeaves / 2);
+            KeyBox u = s_tree[4 * i], v = s_tree[4 * i + 2];
+            key_merge(u, s_tree[4 * i + 1]);
+            key_merge(v, s_tree[4 * i + 3]);
+            key_merge(u, v);
+            s_tree[i] = u;
+        }
+        lds_barrier();
     }
-    if (lane == 63) s_wmax[0][wave] = pm;
-    if (lane == 0) s_wmax[1][wave] = sm;
-    __syncthreads();
+
+    // Node pos has local delta jp = t + 1.  jL: the last j < jp with
+    // d_j >= d_jp, so Lp = b0 + jL; jR: the first j > jp with d_j > d_jp, so
+    // Rp = b0 - 1 + jR.  The node is block-local iff both lie in [0, m].
+    int jL = -1, jR = (int)m + 1;
+    bool local = false;
+    if (isNode) {
+        const uint32_t x = s_st[0][t + 1];
+        jL = t;
+        jR = t + 2;
 #pragma unroll
-    for (int w = 0; w < kWaves; w++) {  // all reads at once, no per-wave trip count
-        const uint32_t a = s_wmax[0][w], b = s_wmax[1][w];
-        if (w < wave) pm = pm > a ? pm : a;
-        if (w > wave) sm = sm > b ? sm : b;
-    }
-    s_pm[t] = (uint8_t)pm;
-    s_sm[t + 1] = (uint8_t)sm;
-    __syncthreads();
-    // the block-local test of each internal node of the block, once, so that a
-    // climb step reads it together with its two deltas
-    if ((uint32_t)t + 1 < m) {
-        const uint32_t dp = s_d[t + 1];
-        s_loc[t] = (uint8_t)((b0 == 0 || (uint32_t)s_pm[t] >= dp) && (b1 == nInternals || (uint32_t)s_sm[t + 2] > dp));
-    }
-    __syncthreads();
-
-    bool rootHere = false, emit = false;
-    uint32_t node = 0, lo = pos, hi = pos;
-    if ((uint32_t)t < m) {
-        const uint32_t triIndex = (uint32_t)(myRec >> 32);
-        Box box = tri_box(tris[triIndex]);
-        node = triIndex | kLeafBit;
-        while (true) {
-            if (hi - lo >= nInternals) {
-                rootHere = true;
-                break;
-            }
-            // four independent LDS reads (indices stay inside the block: the
-            // range [lo, hi] only grows through block-local parents)
-            const uint32_t vL = s_d[lo - b0], vR = s_d[hi - b0 + 1];
-            const uint32_t lL = s_loc[lo > b0 ? lo - 1 - b0 : 0], lR = s_loc[hi - b0];
-            const uint32_t dL = 0 < lo ? vL : 0xFFFFFFFFu;
-            const uint32_t dR = hi < nInternals ? vR : 0xFFFFFFFFu;
-            const bool locL = lo > b0 && lL != 0;  // parent lo - 1
-            const bool locR = hi < b1 && lR != 0;  // parent hi
-            int goLeft = dL < dR ? 1 : 0;
-            uint32_t parent = goLeft ? lo - 1 : hi;
-            const bool local = goLeft ? locL : locR;
-            if (!local) {
-                emit = true;
-                break;
-            }
-            const uint32_t slot = parent - b0;
-            if (!(node & kLeafBit)) s_parent[node - b0] = parent;
-            s_child[slot][goLeft] = node;
-            s_box[slot][goLeft] = box;
-            uint32_t idx = __hip_atomic_exchange(&s_flag[slot], goLeft ? hi : lo, __ATOMIC_ACQ_REL,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (idx == 0xFFFFFFFFu) break;
-            lo = lo < idx ? lo : idx;
-            hi = hi < idx ? idx : hi;
-            s_context[slot] = hi < nInternals ? hi : 0xFFFFFFFFu;
-            node = parent;
-            Box other = s_box[slot][goLeft ^ 1];
-            box.lo = umin3(box.lo, other.lo);
-            box.hi = umax3(box.hi, other.hi);
+        for (int k = kLiftLevels - 1; k >= 0; k--) {
+            const int w = 1 << k;
+            if (jL - w + 1 >= 0 && s_st[k][jL - w + 1] < x) jL -= w;
+            if (jR + w - 1 <= (int)m && s_st[k][jR] <= x) jR += w;
         }
+        local = jL >= 0 && jR <= (int)m;
+        s_loc[t] = (uint8_t)local;
     }
-    if (rootHere) *root = node;
+    lds_barrier();
 
-    // frontier entries: a block-wide compaction, one global atomic per block
-    const uint64_t emitMask = __ballot(emit);
-    if (lane == 0) s_wcount[wave] = (uint32_t)__popcll(emitMask);
-    __syncthreads();
-    if (t == 0) {
-        uint32_t total = 0;
-        for (int w = 0; w < kWaves; w++) {
-            uint32_t c = s_wcount[w];
-            s_wcount[w] = total;
-            total += c;
-        }
-        s_base = total ? atomicAdd(&ctrl[kCtrlFrontier], total) : 0u;
+    // Parents, child slots and frontier entries.
+    bool emitLeaf = false, emitNode = false, isRoot = false;
+    uint32_t parent = kInvalidNode;
+    const uint32_t Lp = b0 + (uint32_t)jL, Rp = b0 - 1 + (uint32_t)jR;
+    if (isLeaf) {
+        const uint32_t dL = s_st[0][t], dR = s_st[0][t + 1];
+        const int goLeft = dL < dR ? 1 : 0;  // parent pos - 1 (slot t - 1) or pos (slot t)
+        const bool pLocal = goLeft ? (t > 0 && s_loc[t - 1] != 0) : (isNode && s_loc[t] != 0);
+        if (pLocal)
+            s_child[goLeft ? t - 1 : t][goLeft] = triIndex | kLeafBit;
+        else
+            emitLeaf = true;
     }
-    __syncthreads();
-    if (emit) {
-        const uint32_t k = s_base + s_wcount[wave] + (uint32_t)__popcll(emitMask & ((1ull << lane) - 1ull));
-        frontier[3 * (size_t)k] = node;
-        frontier[3 * (size_t)k + 1] = lo;
-        frontier[3 * (size_t)k + 2] = hi;
-    }
-
-    // whole block-local nodes (both arrivals seen: the arrival word holds the
-    // second one's bound), four 16-byte stores each
-    const uint32_t nSlots = m - 1;  // internal nodes b0 .. b1-1
-    for (uint32_t q = (uint32_t)t; q < 4 * nSlots; q += kLocalLeaves) {
-        const uint32_t slot = q >> 2, part = q & 3;
-        if (s_flag[slot] == 0xFFFFFFFFu) continue;
-        const Box& l = s_box[slot][0];
-        const Box& r = s_box[slot][1];
-        uint4 v;
-        if (part == 0) {
-            v = make_uint4(s_child[slot][0], s_child[slot][1], __float_as_uint(l.lo.x), __float_as_uint(l.lo.y));
-        } else if (part == 1) {
-            v = make_uint4(__float_as_uint(l.lo.z), __float_as_uint(l.hi.x), __float_as_uint(l.hi.y),
-                           __float_as_uint(l.hi.z));
-        } else if (part == 2) {
-            v = make_uint4(__float_as_uint(r.lo.x), __float_as_uint(r.lo.y), __float_as_uint(r.lo.z),
-                           __float_as_uint(r.hi.x));
+    if (local) {
+        isRoot = Lp == 0 && Rp == nInternals;
+        if (isRoot) {
+            *root = pos;
         } else {
-            v = make_uint4(__float_as_uint(r.hi.y), __float_as_uint(r.hi.z), s_parent[slot], s_context[slot]);
+            const uint32_t dL = s_st[0][jL], dR = s_st[0][jR];
+            const int goLeft = dL < dR ? 1 : 0;
+            parent = goLeft ? Lp - 1 : Rp;
+            const uint32_t slot = parent - b0;  // wraps to a huge value for the parent b0 - 1
+            const bool pLocal = slot < m - 1u && s_loc[slot] != 0;
+            if (pLocal)
+                s_child[slot][goLeft] = pos;
+            else
+                emitNode = true;
         }
-        reinterpret_cast<uint4*>(nodes + b0 + slot)[part] = v;
     }
+
+    // frontier entries: compacted into the block's stretch of the list
+    const uint64_t leafMask = __ballot(emitLeaf), nodeMask = __ballot(emitNode);
+    if (lane == 0) s_wcount[wave] = (uint32_t)(__popcll(leafMask) + __popcll(nodeMask));
+    lds_barrier();  // also publishes the child slots
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        const uint32_t c = s_wcount[w];
+        before += w < wave ? c : 0u;
+        total += c;
+    }
+    if (emitLeaf || emitNode) {
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint32_t k = b0 + before + (uint32_t)(__popcll(leafMask & below) + __popcll(nodeMask & below));
+        if (emitLeaf) {
+            frontier[3 * (size_t)k] = triIndex | kLeafBit;
+            frontier[3 * (size_t)k + 1] = pos;
+            frontier[3 * (size_t)k + 2] = pos;
+            k++;
+        }
+        if (emitNode) {
+            frontier[3 * (size_t)k] = pos;
+            frontier[3 * (size_t)k + 1] = Lp;
+            frontier[3 * (size_t)k + 2] = Rp;
+        }
+    }
+    if (t == 0 && total < m) frontier[3 * (size_t)(b0 + total)] = kFrontierEnd;
+
+    // The block-local node, whole: child boxes over [Lp, pos] and [pos + 1, Rp].
+    if (local) {
+        auto range_box = [&](uint32_t a, uint32_t b) -> Box {  // local leaves a .. b
+            KeyBox acc = {{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, {0u, 0u, 0u}};
+            uint32_t l = a + kLocalLeaves, r = b + kLocalLeaves + 1;
+            while (l < r) {
+                if (l & 1) key_merge(acc, s_tree[l++]);
+                if (r & 1) key_merge(acc, s_tree[--r]);
+                l >>= 1;
+                r >>= 1;
+            }
+            Box out;
+            out.lo = mk(ord2f(acc.lo[0]), ord2f(acc.lo[1]), ord2f(acc.lo[2]));
+            out.hi = mk(ord2f(acc.hi[0]), ord2f(acc.hi[1]), ord2f(acc.hi[2]));
+            return out;
+        };
+        const Box l = range_box(Lp - b0, (uint32_t)t), r = range_box((uint32_t)t + 1, Rp - b0);
+        const uint32_t context = Rp < nInternals ? Rp : 0xFFFFFFFFu;
+        uint4* dst = reinterpret_cast<uint4*>(nodes + pos);
+        dst[0] = make_uint4(s_child[t][0], s_child[t][1], __float_as_uint(l.lo.x), __float_as_uint(l.lo.y));
+        dst[1] = make_uint4(__float_as_uint(l.lo.z), __float_as_uint(l.hi.x), __float_as_uint(l.hi.y),
+                            __float_as_uint(l.hi.z));
+        dst[2] = make_uint4(__float_as_uint(r.lo.x), __float_as_uint(r.lo.y), __float_as_uint(r.lo.z),
+                            __float_as_uint(r.hi.x));
+        dst[3] = make_uint4(__float_as_uint(r.hi.y), __float_as_uint(r.hi.z), parent, context);
+    }
+    // d[b0 + t], stored last: nothing in the block waits for a global store
+    if (kFuseDeltas && isLeaf && pos < nInternals) deltasOut[pos] = s_st[0][t + 1];
 }
 
+// One wave per block of phase 1, over that block's stretch of the frontier
+// list up to its end mark.
 template <class TS>
-__global__ void __launch_bounds__(kBuildBlock)
-    k_climb_upper(TS tris, uint32_t n, const uint8_t* __restrict__ deltas, Node* nodes,
-                  uint32_t* flags, const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ ctrl,
-                  uint32_t* root)
+__global__ void __launch_bounds__(64)
+    k_climb_upper(TS tris, uint32_t n, const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags,
+                  const uint32_t* __restrict__ frontier, uint32_t* root)
 {
-    const uint32_t count = ctrl[kCtrlFrontier];
-    for (uint32_t i = blockIdx.x * kBuildBlock + threadIdx.x; i < count; i += gridDim.x * kBuildBlock) {
-        const uint32_t node = frontier[3 * (size_t)i], lo = frontier[3 * (size_t)i + 1],
-                       hi = frontier[3 * (size_t)i + 2];
-        Box box;
-        if (node & kLeafBit) {
-            box = tri_box(tris[node & ~kLeafBit]);
-        } else {
-            const Node& nd = nodes[node];
-            box.lo = umin3(nd.box[0].lo, nd.box[1].lo);
-            box.hi = umax3(nd.box[0].hi, nd.box[1].hi);
+    const uint32_t b0 = blockIdx.x * kLocalLeaves;
+    const uint32_t m = n - b0 < (uint32_t)kLocalLeaves ? n - b0 : (uint32_t)kLocalLeaves;
+    for (uint32_t base = 0; base < m; base += 64) {
+        const uint32_t k = base + threadIdx.x;
+        const size_t e = 3 * (size_t)(b0 + k);
+        const uint32_t node = k < m ? frontier[e] : kFrontierEnd;
+        const uint64_t endMask = __ballot(node == kFrontierEnd);
+        const uint32_t valid = endMask ? (uint32_t)__ffsll((unsigned long long)endMask) - 1u : 64u;
+        if (threadIdx.x < valid) {
+            const uint32_t lo = frontier[e + 1], hi = frontier[e + 2];
+            Box box;
+            if (node & kLeafBit) {
+                box = tri_box(tris[node & ~kLeafBit]);
+            } else {
+                const Node& nd = nodes[node];
+                box.lo = umin3(nd.box[0].lo, nd.box[1].lo);
+                box.hi = umax3(nd.box[0].hi, nd.box[1].hi);
+            }
+            climb_from(n, lo, hi, node, box, deltas, nodes, flags, root);
         }
-        climb_from(n, lo, hi, node, box, deltas, nodes, flags, root);
+        if (endMask) break;
     }
 }
 
@@ -454,17 +522,11 @@ static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uin
     unsigned blocks = (unsigned)(((size_t)n + kLocalLeaves - 1) / kLocalLeaves);
     if (fuseDeltas)
         k_climb_local<TS, true><<<blocks, kLocalLeaves, 0, stream>>>(tris, (uint32_t)n, rec, nullptr, deltas, nodes,
-                                                                     ctx->frontier, ctx->ctrl, root);
+                                                                     ctx->frontier, root);
     else
         k_climb_local<TS, false><<<blocks, kLocalLeaves, 0, stream>>>(tris, (uint32_t)n, rec, deltas, nullptr, nodes,
-                                                                      ctx->frontier, ctx->ctrl, root);
-    // one lane per expected entry (about n/64 for surface meshes), at most 8
-    // blocks per CU; lanes stride over a larger list
-    size_t want = ((size_t)n / 64 + kBuildBlock - 1) / kBuildBlock + 1;
-    unsigned cap = (unsigned)ctx->cuCount * 8;
-    unsigned upper = want < cap ? (unsigned)want : cap;
-    k_climb_upper<<<upper, kBuildBlock, 0, stream>>>(tris, (uint32_t)n, deltas, nodes, ctx->flags, ctx->frontier,
-                                                     ctx->ctrl, root);
+                                                                      ctx->frontier, root);
+    k_climb_upper<<<blocks, 64, 0, stream>>>(tris, (uint32_t)n, deltas, nodes, ctx->flags, ctx->frontier, root);
     return MLBVH_OK;
 }
 

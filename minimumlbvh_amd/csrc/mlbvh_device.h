@@ -129,6 +129,15 @@ __device__ inline uint64_t morton_in_box(f3 lo, f3 hi, f3 p)
     return splat3(cx) | splat3(cy) << 1 | splat3(cz) << 2;
 }
 
+// Block barrier for LDS data only: unlike __syncthreads it does not wait for
+// the wave's outstanding global loads and stores.
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Order-preserving float <-> uint map for atomic min/max of floats.
 __device__ inline uint32_t f2ord(float f)
 {
