@@ -20,7 +20,8 @@ def gpu_build(gpu, tris, opt=0, keep=False):
     return nodes, root, b, t
 
 
-@pytest.mark.parametrize("n", [2, 3, 4, 100, 257, 4096, 65536, 65537, 300_001])
+@pytest.mark.parametrize("n", [2, 3, 4, 100, 257, 1023, 1024, 1025, 2047, 2048, 2049, 3073, 4096, 65536, 65537,
+                               300_001])
 def test_build_bit_exact(gpu, orc, n):
     print(f"build n={n}", flush=True)
     tris = scenes.random_triangles(n, 1000 + n)
@@ -46,7 +47,7 @@ def signed_zero_triangles(n, seed):
     return t
 
 
-@pytest.mark.parametrize("case", ["all_equal", "flat_axis", "duplicates", "tiny_extent", "signed_zero"])
+@pytest.mark.parametrize("case", ["all_equal", "flat_axis", "duplicates", "tiny_extent", "signed_zero", "long_runs"])
 def test_build_degenerate(gpu, orc, case):
     """signed_zero: merges that meet -0 and +0 keep -0 below and +0 above,
     whichever child's thread arrives second (both climb phases: 300,001
@@ -58,6 +59,8 @@ def test_build_degenerate(gpu, orc, case):
     elif case == "flat_axis":
         tris = scenes.random_triangles(20000, 2)
         tris[:, :, 1] = 0.5
+    elif case == "long_runs":  # runs of 1,200 equal keys, each across a 1024-leaf block boundary
+        tris = np.repeat(scenes.random_triangles(5, 6), 1200, axis=0)
     elif case == "duplicates":
         tris = scenes.random_triangles(20000, 3)
         tris[::2] = tris[7]
