@@ -8,13 +8,17 @@
 //  * the Morton kernel also builds the four 8-bit digit histograms the sort
 //    needs and initialises the climb's arrival words, so the sort starts with
 //    its first reorder pass and no memset of the node array is needed;
-//  * the climb runs in two phases: nodes whose leaves all lie in one block of
-//    1024 sorted leaves are merged in LDS, the rest (about 1% of the nodes of
-//    a surface mesh) through device-scope arrival words, one u32 per internal
-//    node; `context` is written once, after the merge, with the reference's
-//    final value;
+//  * no climb inside blocks: nodes whose leaves lie in one block of 256
+//    sorted leaves are built each on its own from the deltas (ranges by
+//    nearest-greater searches, boxes by range unions in LDS), then the same
+//    over blocks of 1024 of the resulting subtrees; the rest (about 0.05% of
+//    the nodes of a surface mesh) climbs through device-scope arrival words,
+//    one u32 per internal node; `context` is written once, with the
+//    reference's final value;
 //  * one thread per leaf for all n leaves (the reference launches
 //    ceil((n-1)/256) blocks and drops the last leaf when (n-1) % 256 == 0).
+#include <algorithm>
+
 #include "mlbvh_ctx.h"
 
 namespace mlbvh {
@@ -206,48 +210,56 @@ __global__ void __launch_bounds__(kBuildBlock)
     }
 }
 
-// Agglomerative climb (build_lbvh, minimum_lbvh.h:284-361) in two phases.
+// Agglomerative climb (build_lbvh, minimum_lbvh.h:284-361), built level by
+// level without climbing where the nodes fit a block.
 //
 // Internal node p (the split between sorted leaves p and p+1) covers the
 // leaves [Lp, Rp] with Lp = 1 + (last q < p with d[q] >= d[p], else -1) and
 // Rp = (first q > p with d[q] > d[p], else n-1): the climb merges across the
-// smaller boundary delta first and goes right on ties.  So p lies entirely
-// inside a block's leaves [b0, b1] iff
-//   (b0 == 0 or max d[b0-1 .. p-1] >= d[p]) and (b1 == n-1 or max d[p+1 .. b1] > d[p]),
-// which both of p's children evaluate alike from a prefix and a suffix
-// maximum of the block's deltas.
+// smaller boundary delta first and goes right on ties.  Its parent is Lp - 1
+// if d[Lp - 1] < d[Rp], else Rp (a missing d[-1] or d[n-1] counts as above
+// every delta).
 //
-// Phase 1 (k_climb_local): a block of kLocalLeaves threads builds its
-// block-local nodes without climbing, each node on its own.  For internal
-// node p, Lp and Rp come from two nearest-greater searches (binary lifting
-// over a sparse table of the block's deltas in LDS).  Its parent is Lp - 1
-// if d[Lp - 1] < d[Rp], else Rp (the climb's direction rule), and every
-// node and leaf writes its index into its parent's child slot.  Its two
-// child boxes are the unions of the leaf boxes over [Lp, p] and [p + 1, Rp],
-// read from a segment tree of the block's leaf boxes in LDS.  The box merge
-// (umin / umax: min and max under a total order that puts -0 below +0) is
-// associative and commutative, so these unions equal the climb's pairwise
-// merges bit for bit.  The tree holds each coordinate as its order-
-// preserving integer key (f2ord), on which umin / umax are integer min / max
-// for every non-NaN value: a box merge is six integer instructions.  A leaf or block-local node whose parent is not
-// block-local goes to the frontier list with its range: the block's own
-// stretch of the list, entries b0 .. b0 + m - 1 (a block has at most one
-// entry per leaf), closed by an end mark when shorter.  Each block-local
-// node is written whole (64 bytes) by its thread; one whose parent is not
-// block-local gets its parent word again from phase 2.  The block's barriers
-// order LDS only, so its global stores are never waited for.
-//
-// Phase 2 (k_climb_upper): the frontier entries (about 1% of the nodes for
-// surface meshes) climb on through the device-scope arrival words
-// (climb_from).  The result is the same tree as one thread per leaf climbing
-// through device memory all the way.
-#ifndef MLBVH_CLIMB_LEAVES
-#define MLBVH_CLIMB_LEAVES 1024
+// k_local_nodes builds, for a block of consecutive elements of a sequence,
+// every node whose range lies inside the block, each node on its own:
+//   - the block's deltas go into a sparse table of range maxima in LDS, and
+//     the elements' boxes into a segment tree in LDS;
+//   - node p finds the two ends of its range by binary lifting over the
+//     sparse table; it is block-local iff both lie in the block;
+//   - every element and block-local node writes its index into its parent's
+//     child slot (when that parent is block-local), else it becomes a
+//     frontier entry (node, first leaf, last leaf);
+//   - the child boxes are unions over the element ranges [Lp, p] and
+//     [p + 1, Rp] from the segment tree.  The box merge (umin / umax: min
+//     and max under a total order that puts -0 below +0) is associative and
+//     commutative, so these unions equal the climb's pairwise merges bit for
+//     bit.  The tree holds each coordinate as its order-preserving integer
+//     key (f2ord), on which umin / umax are integer min / max for every
+//     non-NaN value;
+//   - each block-local node is written whole (64 bytes) by its thread, and
+//     the block's frontier entries go to its own stretch of the frontier
+//     list (entries b0 .. b0 + m - 1: at most one per element) in element
+//     order, closed by an end mark when shorter, with their count.
+// Level 1 runs it over the sorted leaves in blocks of kLeaves1.  Its frontier
+// entries, in leaf order over all blocks, are the maximal level-1 subtrees:
+// a sequence of "super-leaves" that tiles the leaves.  Between super-leaves
+// j and j + 1 lies internal node hi_j with delta d[hi_j], and the nodes above
+// level 1 are exactly the tree these splits form under the same rule, so
+// level 2 runs the same kernel over the super-leaves in blocks of kLeaves2.
+// What crosses a level-2 block (a few thousand nodes for surface meshes)
+// climbs on through the device-scope arrival words (k_climb_upper,
+// climb_from).  The result is the tree of one thread per leaf climbing
+// through device memory all the way.  A node whose parent is built by a
+// later level gets its parent word from its own level (the rule above gives
+// it) and again, identically, from climb_from.  The blocks' barriers order
+// LDS only, so their global stores are never waited for.
+#ifndef MLBVH_LEVEL1_LEAVES
+#define MLBVH_LEVEL1_LEAVES 256
 #endif
-constexpr int kLocalLeaves = MLBVH_CLIMB_LEAVES;
+constexpr int kLeaves1 = MLBVH_LEVEL1_LEAVES;
+constexpr int kLeaves2 = 1024;
 constexpr uint32_t kDeltaInf = 255;  // the missing d[-1] and d[n-1]: above every delta
 constexpr int log2_exact(int v) { return v <= 1 ? 0 : 1 + log2_exact(v / 2); }
-constexpr int kLiftLevels = log2_exact(kLocalLeaves) + 1;  // windows of 1 .. kLocalLeaves deltas
 constexpr uint32_t kFrontierEnd = 0xFFFFFFFFu;  // no node or leaf index (n < 2^31)
 
 // A box as order-preserving keys of its six coordinates.
@@ -267,238 +279,358 @@ __device__ inline void key_merge(KeyBox& a, const KeyBox& b)
     }
 }
 
-// kFuseDeltas: the deltas come from the block's own sorted records
-// (delta(a, b) = 32 - clz(a ^ b) of neighbouring keys, minimum_lbvh.cu:94-102)
-// and the block writes d[b0 .. b1] to `deltasOut` for the upper phase and the
-// caller; otherwise they are read from `deltas` (63-bit keys, refit).
-template <class TS, bool kFuseDeltas>
-__global__ void __launch_bounds__(kLocalLeaves)
-    k_climb_local(TS tris, uint32_t n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
-                  uint8_t* __restrict__ deltasOut, Node* __restrict__ nodes, uint32_t* __restrict__ frontier,
-                  uint32_t* __restrict__ root)
+// Sequences k_local_nodes runs over.
+enum LocalMode {
+    kLeavesFusedDeltas = 0,  // sorted leaves; deltas from the records' keys (32 - clz(a ^ b),
+                             // minimum_lbvh.cu:94-102), written to deltasOut
+    kLeavesGivenDeltas = 1,  // sorted leaves; deltas read from `deltas` (63-bit keys, refit)
+    kSuperLeaves = 2,        // the super-leaves (node, lo, hi) of `elems`, ctrl[kCtrlSuperCount] of them
+};
+
+template <class TS, int kLeaves, int kMode>
+__global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kLeaves <= 256 ? 8 : 1)))
+    k_local_nodes(TS tris, uint32_t n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
+                  uint8_t* __restrict__ deltasOut, const uint32_t* __restrict__ elems,
+                  const uint32_t* __restrict__ ctrl, Node* __restrict__ nodes, uint32_t* __restrict__ frontier,
+                  uint32_t* __restrict__ blockCounts, uint32_t* __restrict__ root)
 {
-    static_assert(kLocalLeaves == 1 << (kLiftLevels - 1) && kLocalLeaves >= 256 && kLocalLeaves <= 1024,
-                  "a power-of-two block of 256 .. 1024 leaves");
-    constexpr int kWaves = kLocalLeaves / 64;
-    constexpr int kRow = kLocalLeaves + 16;
-    // Local delta j is d[b0 - 1 + j], j in [0, m]; s_st[k][j] = max of the
-    // local deltas j .. j + 2^k - 1.
-    __shared__ uint8_t s_st[kLiftLevels][kRow];
-    // Segment tree of the leaf boxes: leaf j at kLocalLeaves + j, node i the
+    constexpr int kLift = log2_exact(kLeaves) + 1;  // windows of 1 .. kLeaves deltas
+    static_assert(kLeaves == 1 << (kLift - 1) && kLeaves >= 256 && kLeaves <= 1024,
+                  "a power-of-two block of 256 .. 1024 elements");
+    constexpr int kWaves = kLeaves / 64;
+    constexpr int kRow = kLeaves + 16;
+    // Local delta j belongs to the split between elements b0 - 1 + j and
+    // b0 + j, j in [0, m]; s_st[k][j] = max of the local deltas j .. j + 2^k - 1.
+    __shared__ uint8_t s_st[kLift][kRow];
+    // Segment tree of the element boxes: element j at kLeaves + j, node i the
     // union of nodes 2i and 2i + 1.
-    __shared__ KeyBox s_tree[2 * kLocalLeaves];
-    __shared__ uint32_t s_child[kLocalLeaves][2];
-    __shared__ uint8_t s_loc[kLocalLeaves];  // internal node b0 + j lies inside the block
+    __shared__ KeyBox s_tree[2 * kLeaves];
+    __shared__ uint32_t s_child[kLeaves][2];
+    __shared__ uint8_t s_loc[kLeaves];   // split b0 + j lies inside the block
+    __shared__ uint8_t s_mark[kLeaves];  // a frontier entry starts at element b0 + j
+    // kLeavesFusedDeltas: s_aux[j] = key of sorted position b0 - 1 + j;
+    // kSuperLeaves: s_aux[j] = first leaf of element b0 + j, s_aux[m] = last leaf of the block + 1
+    __shared__ uint32_t s_aux[kLeaves + 2];
     __shared__ uint32_t s_wcount[kWaves];
+    // The frontier entry starting at element j: its node and last leaf (its
+    // first leaf is element j's), in the sparse table's LDS once that is dead.
+    uint2* s_ent = reinterpret_cast<uint2*>(&s_st[0][0]);
+    static_assert(sizeof(uint2) * kLeaves <= sizeof(s_st), "the entries fit the sparse table's LDS");
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const uint32_t b0 = blockIdx.x * kLocalLeaves;
-    const uint32_t m = n - b0 < (uint32_t)kLocalLeaves ? n - b0 : (uint32_t)kLocalLeaves;
+    const uint32_t count = kMode == kSuperLeaves ? ctrl[kCtrlSuperCount] : n;
     const uint32_t nInternals = n - 1;
-    const uint32_t pos = b0 + (uint32_t)t;
-    const bool isLeaf = (uint32_t)t < m;
-    const bool isNode = (uint32_t)t + 1 < m;  // internal node pos: the split between leaves pos and pos + 1
+    // Level 1 has one block per kLeaves leaves; level 2 loops over its blocks
+    // (their number is known on the device only).
+    for (uint32_t blk = blockIdx.x; (size_t)blk * kLeaves < count; blk += gridDim.x) {
+        const uint32_t b0 = blk * kLeaves;
+        const uint32_t m = count - b0 < (uint32_t)kLeaves ? count - b0 : (uint32_t)kLeaves;
+        const uint32_t s = b0 + (uint32_t)t;
+        const bool isElem = (uint32_t)t < m;
+        const bool isSplit = (uint32_t)t + 1 < m;  // split s: between elements s and s + 1
 
-    const uint64_t myRec = isLeaf ? rec[pos] : 0ull;
-    const uint32_t triIndex = (uint32_t)(myRec >> 32);
-    Box leafBox;
-    if (isLeaf) {
-        leafBox = tri_box(tris[triIndex]);
-    } else {
-        leafBox.lo = mk(kFltMax, kFltMax, kFltMax);
-        leafBox.hi = mk(-kFltMax, -kFltMax, -kFltMax);
-    }
-    s_tree[kLocalLeaves + t] = key_box(leafBox);
-
-    if (kFuseDeltas) {
-        __shared__ uint32_t s_k[kLocalLeaves + 2];  // s_k[j] = key of sorted position b0 - 1 + j
-        if (isLeaf) s_k[t + 1] = (uint32_t)myRec;
-        if (t == 0 && b0 > 0) s_k[0] = (uint32_t)rec[b0 - 1];
-        if ((uint32_t)t == m - 1 && b0 + m < n) s_k[m + 1] = (uint32_t)rec[b0 + m];
-        lds_barrier();
-        auto delta_at = [&](uint32_t j) -> uint32_t {
-            const uint32_t g = b0 + j;  // global delta index + 1
-            return (g == 0 || g - 1 >= nInternals) ? kDeltaInf : (uint32_t)(32 - __clz(s_k[j] ^ s_k[j + 1]));
-        };
-        s_st[0][t] = (uint8_t)delta_at((uint32_t)t);
-        if (t == kLocalLeaves - 1) s_st[0][kLocalLeaves] = (uint8_t)delta_at(kLocalLeaves);
-    } else {
-        auto delta_at = [&](uint32_t j) -> uint32_t {
-            const uint32_t g = b0 + j;
-            return (g == 0 || g - 1 >= nInternals) ? kDeltaInf : (uint32_t)deltas[g - 1];
-        };
-        s_st[0][t] = (uint8_t)delta_at((uint32_t)t);
-        if (t == kLocalLeaves - 1) s_st[0][kLocalLeaves] = (uint8_t)delta_at(kLocalLeaves);
-    }
-    lds_barrier();
-    // Two levels per step: level k from k - 1, level k + 1 from the four
-    // windows of level k - 1 (the box merge is exact, so the order does not
-    // matter).
-#pragma unroll
-    for (int k = 1; k < kLiftLevels; k += 2) {
-        const int h = 1 << (k - 1);
-        if (t + 2 * h - 1 <= kLocalLeaves) {
-            const uint32_t a = s_st[k - 1][t], b = s_st[k - 1][t + h];
-            s_st[k][t] = (uint8_t)(a > b ? a : b);
-        }
-        if (k + 1 < kLiftLevels && t + 4 * h - 1 <= kLocalLeaves) {
-            uint32_t a = s_st[k - 1][t], b = s_st[k - 1][t + h], c = s_st[k - 1][t + 2 * h], d = s_st[k - 1][t + 3 * h];
-            a = a > b ? a : b;
-            c = c > d ? c : d;
-            s_st[k + 1][t] = (uint8_t)(a > c ? a : c);
-        }
-        const int first = kLocalLeaves >> k;  // level k of the tree: nodes first .. 2 first - 1
-        if (t < first) {
-            KeyBox u = s_tree[2 * (first + t)];
-            key_merge(u, s_tree[2 * (first + t) + 1]);
-            s_tree[first + t] = u;
-        } else if (t >= kLocalLeaves / 2 && t - kLocalLeaves / 2 < first / 2) {  // level k + 1 from level k - 1
-            const int i = first / 2 + (t - kLocalLeaves / 2);
-            KeyBox u = s_tree[4 * i], v = s_tree[4 * i + 2];
-            key_merge(u, s_tree[4 * i + 1]);
-            key_merge(v, s_tree[4 * i + 3]);
-            key_merge(u, v);
-            s_tree[i] = u;
-        }
-        lds_barrier();
-    }
-
-    // Node pos has local delta jp = t + 1.  jL: the last j < jp with
-    // d_j >= d_jp, so Lp = b0 + jL; jR: the first j > jp with d_j > d_jp, so
-    // Rp = b0 - 1 + jR.  The node is block-local iff both lie in [0, m].
-    int jL = -1, jR = (int)m + 1;
-    bool local = false;
-    if (isNode) {
-        const uint32_t x = s_st[0][t + 1];
-        jL = t;
-        jR = t + 2;
-#pragma unroll
-        for (int k = kLiftLevels - 1; k >= 0; k--) {
-            const int w = 1 << k;
-            if (jL - w + 1 >= 0 && s_st[k][jL - w + 1] < x) jL -= w;
-            if (jR + w - 1 <= (int)m && s_st[k][jR] <= x) jR += w;
-        }
-        local = jL >= 0 && jR <= (int)m;
-        s_loc[t] = (uint8_t)local;
-    }
-    lds_barrier();
-
-    // Parents, child slots and frontier entries.
-    bool emitLeaf = false, emitNode = false, isRoot = false;
-    uint32_t parent = kInvalidNode;
-    const uint32_t Lp = b0 + (uint32_t)jL, Rp = b0 - 1 + (uint32_t)jR;
-    if (isLeaf) {
-        const uint32_t dL = s_st[0][t], dR = s_st[0][t + 1];
-        const int goLeft = dL < dR ? 1 : 0;  // parent pos - 1 (slot t - 1) or pos (slot t)
-        const bool pLocal = goLeft ? (t > 0 && s_loc[t - 1] != 0) : (isNode && s_loc[t] != 0);
-        if (pLocal)
-            s_child[goLeft ? t - 1 : t][goLeft] = triIndex | kLeafBit;
-        else
-            emitLeaf = true;
-    }
-    if (local) {
-        isRoot = Lp == 0 && Rp == nInternals;
-        if (isRoot) {
-            *root = pos;
-        } else {
-            const uint32_t dL = s_st[0][jL], dR = s_st[0][jR];
-            const int goLeft = dL < dR ? 1 : 0;
-            parent = goLeft ? Lp - 1 : Rp;
-            const uint32_t slot = parent - b0;  // wraps to a huge value for the parent b0 - 1
-            const bool pLocal = slot < m - 1u && s_loc[slot] != 0;
-            if (pLocal)
-                s_child[slot][goLeft] = pos;
-            else
-                emitNode = true;
-        }
-    }
-
-    // frontier entries: compacted into the block's stretch of the list
-    const uint64_t leafMask = __ballot(emitLeaf), nodeMask = __ballot(emitNode);
-    if (lane == 0) s_wcount[wave] = (uint32_t)(__popcll(leafMask) + __popcll(nodeMask));
-    lds_barrier();  // also publishes the child slots
-    uint32_t before = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; w++) {
-        const uint32_t c = s_wcount[w];
-        before += w < wave ? c : 0u;
-        total += c;
-    }
-    if (emitLeaf || emitNode) {
-        const uint64_t below = (1ull << lane) - 1ull;
-        uint32_t k = b0 + before + (uint32_t)(__popcll(leafMask & below) + __popcll(nodeMask & below));
-        if (emitLeaf) {
-            frontier[3 * (size_t)k] = triIndex | kLeafBit;
-            frontier[3 * (size_t)k + 1] = pos;
-            frontier[3 * (size_t)k + 2] = pos;
-            k++;
-        }
-        if (emitNode) {
-            frontier[3 * (size_t)k] = pos;
-            frontier[3 * (size_t)k + 1] = Lp;
-            frontier[3 * (size_t)k + 2] = Rp;
-        }
-    }
-    if (t == 0 && total < m) frontier[3 * (size_t)(b0 + total)] = kFrontierEnd;
-
-    // The block-local node, whole: child boxes over [Lp, pos] and [pos + 1, Rp].
-    if (local) {
-        auto range_box = [&](uint32_t a, uint32_t b) -> Box {  // local leaves a .. b
-            KeyBox acc = {{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, {0u, 0u, 0u}};
-            uint32_t l = a + kLocalLeaves, r = b + kLocalLeaves + 1;
-            while (l < r) {
-                if (l & 1) key_merge(acc, s_tree[l++]);
-                if (r & 1) key_merge(acc, s_tree[--r]);
-                l >>= 1;
-                r >>= 1;
+        // Element t: node, first and last leaf, box.
+        uint32_t elemNode = 0, elemLo = s, elemHi = s;
+        KeyBox kb = {{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, {0u, 0u, 0u}};
+        uint64_t myRec = 0;
+        if (kMode != kSuperLeaves) {
+            myRec = isElem ? rec[s] : 0ull;
+            elemNode = (uint32_t)(myRec >> 32) | kLeafBit;
+            if (isElem) kb = key_box(tri_box(tris[(uint32_t)(myRec >> 32)]));
+        } else if (isElem) {
+            elemNode = elems[3 * (size_t)s];
+            elemLo = elems[3 * (size_t)s + 1];
+            elemHi = elems[3 * (size_t)s + 2];
+            if (elemNode & kLeafBit) {
+                kb = key_box(tri_box(tris[elemNode & ~kLeafBit]));
+            } else {
+                const Node& nd = nodes[elemNode];
+                kb = key_box(nd.box[0]);
+                key_merge(kb, key_box(nd.box[1]));
             }
-            Box out;
-            out.lo = mk(ord2f(acc.lo[0]), ord2f(acc.lo[1]), ord2f(acc.lo[2]));
-            out.hi = mk(ord2f(acc.hi[0]), ord2f(acc.hi[1]), ord2f(acc.hi[2]));
-            return out;
-        };
-        const Box l = range_box(Lp - b0, (uint32_t)t), r = range_box((uint32_t)t + 1, Rp - b0);
-        const uint32_t context = Rp < nInternals ? Rp : 0xFFFFFFFFu;
-        uint4* dst = reinterpret_cast<uint4*>(nodes + pos);
-        dst[0] = make_uint4(s_child[t][0], s_child[t][1], __float_as_uint(l.lo.x), __float_as_uint(l.lo.y));
-        dst[1] = make_uint4(__float_as_uint(l.lo.z), __float_as_uint(l.hi.x), __float_as_uint(l.hi.y),
-                            __float_as_uint(l.hi.z));
-        dst[2] = make_uint4(__float_as_uint(r.lo.x), __float_as_uint(r.lo.y), __float_as_uint(r.lo.z),
-                            __float_as_uint(r.hi.x));
-        dst[3] = make_uint4(__float_as_uint(r.hi.y), __float_as_uint(r.hi.z), parent, context);
+        }
+        s_tree[kLeaves + t] = kb;
+        s_mark[t] = 0;
+
+        // Local deltas.  Split b0 - 1 + t exists for t < m unless b0 + t == 0
+        // (for t == m it is the sequence's last split's successor, or lies past
+        // the block); its node is leaf position b0 - 1 + t (level 1) or the
+        // last leaf of element b0 - 1 + t = elemLo - 1 (level 2).
+        const bool jValid = isElem && s > 0;
+        uint32_t dj = kDeltaInf, dLast = kDeltaInf;  // local deltas t and (thread kLeaves - 1) kLeaves
+        const bool lastValid = t == kLeaves - 1 && m == (uint32_t)kLeaves && (size_t)b0 + kLeaves < count;
+        if (kMode == kLeavesFusedDeltas) {
+            if (isElem) s_aux[t + 1] = (uint32_t)myRec;
+            if (t == 0 && b0 > 0) s_aux[0] = (uint32_t)rec[b0 - 1];
+            if ((uint32_t)t == m - 1 && b0 + m < count) s_aux[m + 1] = (uint32_t)rec[b0 + m];
+            lds_barrier();
+            if (jValid) dj = 32 - __clz(s_aux[t] ^ s_aux[t + 1]);
+            if (lastValid) dLast = 32 - __clz(s_aux[kLeaves] ^ s_aux[kLeaves + 1]);
+        } else if (kMode == kLeavesGivenDeltas) {
+            if (jValid) dj = deltas[s - 1];
+            if (lastValid) dLast = deltas[s];
+        } else {
+            if (isElem) s_aux[t] = elemLo;
+            if ((uint32_t)t == m - 1) s_aux[m] = elemHi + 1;
+            if (jValid) dj = deltas[elemLo - 1];
+            if (lastValid) dLast = deltas[elemHi];
+        }
+        s_st[0][t] = (uint8_t)dj;
+        if (t == kLeaves - 1) s_st[0][kLeaves] = (uint8_t)dLast;
+        lds_barrier();
+        // Two levels per step: level k from k - 1, level k + 1 from the four
+        // windows of level k - 1 (the merge is exact, so grouping does not matter).
+#pragma unroll
+        for (int k = 1; k < kLift; k += 2) {
+            const int h = 1 << (k - 1);
+            if (t + 2 * h - 1 <= kLeaves) {
+                const uint32_t a = s_st[k - 1][t], b = s_st[k - 1][t + h];
+                s_st[k][t] = (uint8_t)(a > b ? a : b);
+            }
+            if (k + 1 < kLift && t + 4 * h - 1 <= kLeaves) {
+                uint32_t a = s_st[k - 1][t], b = s_st[k - 1][t + h], c = s_st[k - 1][t + 2 * h],
+                         d = s_st[k - 1][t + 3 * h];
+                a = a > b ? a : b;
+                c = c > d ? c : d;
+                s_st[k + 1][t] = (uint8_t)(a > c ? a : c);
+            }
+            const int first = kLeaves >> k;  // level k of the tree: nodes first .. 2 first - 1
+            if (t < first) {
+                KeyBox u = s_tree[2 * (first + t)];
+                key_merge(u, s_tree[2 * (first + t) + 1]);
+                s_tree[first + t] = u;
+            } else if (t >= kLeaves / 2 && t - kLeaves / 2 < first / 2) {  // level k + 1 from level k - 1
+                const int i = first / 2 + (t - kLeaves / 2);
+                KeyBox u = s_tree[4 * i], v = s_tree[4 * i + 2];
+                key_merge(u, s_tree[4 * i + 1]);
+                key_merge(v, s_tree[4 * i + 3]);
+                key_merge(u, v);
+                s_tree[i] = u;
+            }
+            lds_barrier();
+        }
+
+        // Split s has local delta jp = t + 1.  jL: the last j < jp with
+        // d_j >= d_jp; jR: the first j > jp with d_j > d_jp.  Its range is
+        // elements b0 + jL .. b0 - 1 + jR; it is block-local iff both lie in [0, m].
+        int jL = -1, jR = (int)m + 1;
+        bool local = false;
+        if (isSplit) {
+            const uint32_t x = s_st[0][t + 1];
+            jL = t;
+            jR = t + 2;
+#pragma unroll
+            for (int k = kLift - 1; k >= 0; k--) {
+                const int w = 1 << k;
+                if (jL - w + 1 >= 0 && s_st[k][jL - w + 1] < x) jL -= w;
+                if (jR + w - 1 <= (int)m && s_st[k][jR] <= x) jR += w;
+            }
+            local = jL >= 0 && jR <= (int)m;
+            s_loc[t] = (uint8_t)local;
+        }
+        lds_barrier();
+
+        // Parents, child slots and frontier entries.
+        bool emitElem = false, emitNode = false, isRoot = false;
+        uint32_t parent = kInvalidNode, gNode = 0, gLo = 0, gHi = 0;
+        const uint32_t dNext = s_st[0][t + 1];  // d[b0 + t] (level 1), kept for deltasOut
+        if (isElem) {
+            const uint32_t dL = s_st[0][t], dR = dNext;
+            const int goLeft = dL < dR ? 1 : 0;  // parent: split s - 1 (slot t - 1) or split s (slot t)
+            const bool pLocal = goLeft ? (t > 0 && s_loc[t - 1] != 0) : (isSplit && s_loc[t] != 0);
+            if (pLocal)
+                s_child[goLeft ? t - 1 : t][goLeft] = elemNode;
+            else
+                emitElem = true;
+        }
+        if (local) {
+            // the split's node: leaf position s (level 1) or the last leaf of element s (level 2)
+            gNode = kMode == kSuperLeaves ? elemHi : s;
+            gLo = kMode == kSuperLeaves ? s_aux[jL] : b0 + (uint32_t)jL;
+            gHi = kMode == kSuperLeaves ? s_aux[jR] - 1u : b0 - 1u + (uint32_t)jR;
+            isRoot = gLo == 0 && gHi == nInternals;
+            if (isRoot) {
+                *root = gNode;
+            } else {
+                const int goLeft = s_st[0][jL] < s_st[0][jR] ? 1 : 0;
+                parent = goLeft ? gLo - 1 : gHi;  // the last leaf left of the range, or the range's last leaf
+                const uint32_t slot = goLeft ? (uint32_t)jL - 1u : (uint32_t)jR - 1u;  // wraps for jL == 0
+                const bool pLocal = slot < m - 1u && s_loc[slot] != 0;
+                if (pLocal)
+                    s_child[slot][goLeft] = gNode;
+                else
+                    emitNode = true;
+            }
+        }
+        if (emitElem) s_mark[t] = 1;
+        if (emitNode) s_mark[jL] = 1;
+        lds_barrier();  // publishes the child slots and the marks; the sparse table is dead from here
+
+        // Frontier entries in element order: the entry starting at element j
+        // is stored at j, then ranked by the marks before it.
+        if (emitElem) s_ent[t] = make_uint2(elemNode, elemHi);
+        if (emitNode) s_ent[jL] = make_uint2(gNode, gHi);
+        const bool marked = isElem && s_mark[t] != 0;
+        const uint64_t markMask = __ballot(marked);
+        if (lane == 0) s_wcount[wave] = (uint32_t)__popcll(markMask);
+        lds_barrier();
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            const uint32_t c = s_wcount[w];
+            before += w < wave ? c : 0u;
+            total += c;
+        }
+        if (marked) {
+            const uint32_t k = b0 + before + (uint32_t)__popcll(markMask & ((1ull << lane) - 1ull));
+            const uint2 e = s_ent[t];
+            frontier[3 * (size_t)k] = e.x;
+            frontier[3 * (size_t)k + 1] = elemLo;  // the entry's first leaf is element t's
+            frontier[3 * (size_t)k + 2] = e.y;
+        }
+        if (t == 0) {
+            if (total < m) frontier[3 * (size_t)(b0 + total)] = kFrontierEnd;
+            if (blockCounts) blockCounts[blk] = total;
+        }
+
+        // The block-local node, whole: child boxes over elements [jL, t] and [t + 1, jR - 1].
+        if (local) {
+            auto range_box = [&](uint32_t a, uint32_t b) -> Box {  // local elements a .. b
+                KeyBox acc = {{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, {0u, 0u, 0u}};
+                uint32_t l = a + kLeaves, r = b + kLeaves + 1;
+                while (l < r) {
+                    if (l & 1) key_merge(acc, s_tree[l++]);
+                    if (r & 1) key_merge(acc, s_tree[--r]);
+                    l >>= 1;
+                    r >>= 1;
+                }
+                Box out;
+                out.lo = mk(ord2f(acc.lo[0]), ord2f(acc.lo[1]), ord2f(acc.lo[2]));
+                out.hi = mk(ord2f(acc.hi[0]), ord2f(acc.hi[1]), ord2f(acc.hi[2]));
+                return out;
+            };
+            const Box l = range_box((uint32_t)jL, (uint32_t)t), r = range_box((uint32_t)t + 1, (uint32_t)jR - 1u);
+            const uint32_t context = gHi < nInternals ? gHi : 0xFFFFFFFFu;
+            uint4* dst = reinterpret_cast<uint4*>(nodes + gNode);
+            dst[0] = make_uint4(s_child[t][0], s_child[t][1], __float_as_uint(l.lo.x), __float_as_uint(l.lo.y));
+            dst[1] = make_uint4(__float_as_uint(l.lo.z), __float_as_uint(l.hi.x), __float_as_uint(l.hi.y),
+                                __float_as_uint(l.hi.z));
+            dst[2] = make_uint4(__float_as_uint(r.lo.x), __float_as_uint(r.lo.y), __float_as_uint(r.lo.z),
+                                __float_as_uint(r.hi.x));
+            dst[3] = make_uint4(__float_as_uint(r.hi.y), __float_as_uint(r.hi.z), parent, context);
+        }
+        // d[b0 + t], stored last: nothing in the block waits for a global store
+        if (kMode == kLeavesFusedDeltas && isElem && s < nInternals) deltasOut[s] = (uint8_t)dNext;
+        if (kMode != kSuperLeaves) break;  // one block per workgroup: no loop code
+        lds_barrier();                     // the next block of the loop reuses the LDS
     }
-    // d[b0 + t], stored last: nothing in the block waits for a global store
-    if (kFuseDeltas && isLeaf && pos < nInternals) deltasOut[pos] = s_st[0][t + 1];
 }
 
-// One wave per block of phase 1, over that block's stretch of the frontier
-// list up to its end mark.
+// Exclusive scan of the level-1 blocks' frontier counts into offsets, and
+// their total (the number of super-leaves) into ctrl[kCtrlSuperCount].  One
+// block, in tiles of kScanTile counts: coalesced loads into LDS, each thread
+// scans a contiguous run of kScanTile / 1024 there, coalesced stores.
+constexpr uint32_t kScanTile = 16384;
+__global__ void __launch_bounds__(1024)
+    k_frontier_scan(const uint32_t* __restrict__ counts, uint32_t nBlocks, uint32_t* __restrict__ offsets,
+                    uint32_t* __restrict__ ctrl)
+{
+    constexpr int kWaves = 16;
+    constexpr uint32_t kPer = kScanTile / 1024;
+    __shared__ uint32_t s_c[kScanTile];
+    __shared__ uint32_t s_w[kWaves];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nBlocks; base += kScanTile) {
+        const uint32_t m = nBlocks - base < kScanTile ? nBlocks - base : kScanTile;
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; q++) {
+            const uint32_t i = q * 1024 + (uint32_t)t;
+            s_c[i] = i < m ? counts[base + i] : 0u;
+        }
+        __syncthreads();
+        uint32_t v[kPer], sum = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; q++) {
+            v[q] = s_c[(uint32_t)t * kPer + q];
+            sum += v[q];
+        }
+        uint32_t x = sum;  // inclusive scan over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_w[wave] = x;
+        __syncthreads();
+        uint32_t run = carry + x - sum, total = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            const uint32_t c = s_w[w];
+            run += w < wave ? c : 0u;
+            total += c;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; q++) {
+            s_c[(uint32_t)t * kPer + q] = run;
+            run += v[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; q++) {
+            const uint32_t i = q * 1024 + (uint32_t)t;
+            if (i < m) offsets[base + i] = s_c[i];
+        }
+        carry += total;
+        __syncthreads();
+    }
+    if (t == 0) ctrl[kCtrlSuperCount] = carry;
+}
+
+// One wave per level-1 block: its frontier stretch, already in leaf order, to
+// its place in the super-leaf list.
+__global__ void __launch_bounds__(64)
+    k_frontier_compact(const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ counts,
+                       const uint32_t* __restrict__ offsets, uint32_t* __restrict__ superList, uint32_t leavesPerBlock)
+{
+    const uint32_t b = blockIdx.x;
+    const size_t words = 3 * (size_t)counts[b];
+    const size_t src = 3 * (size_t)b * leavesPerBlock, dst = 3 * (size_t)offsets[b];
+    for (size_t k = threadIdx.x; k < words; k += 64) superList[dst + k] = frontier[src + k];
+}
+
+// One wave per level-2 block (looping over them), over that block's stretch
+// of the frontier list up to its end mark: each entry climbs through the
+// device-scope arrival words.
 template <class TS>
 __global__ void __launch_bounds__(64)
     k_climb_upper(TS tris, uint32_t n, const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags,
-                  const uint32_t* __restrict__ frontier, uint32_t* root)
+                  const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ ctrl, uint32_t* root)
 {
-    const uint32_t b0 = blockIdx.x * kLocalLeaves;
-    const uint32_t m = n - b0 < (uint32_t)kLocalLeaves ? n - b0 : (uint32_t)kLocalLeaves;
-    for (uint32_t base = 0; base < m; base += 64) {
-        const uint32_t k = base + threadIdx.x;
-        const size_t e = 3 * (size_t)(b0 + k);
-        const uint32_t node = k < m ? frontier[e] : kFrontierEnd;
-        const uint64_t endMask = __ballot(node == kFrontierEnd);
-        const uint32_t valid = endMask ? (uint32_t)__ffsll((unsigned long long)endMask) - 1u : 64u;
-        if (threadIdx.x < valid) {
-            const uint32_t lo = frontier[e + 1], hi = frontier[e + 2];
-            Box box;
-            if (node & kLeafBit) {
-                box = tri_box(tris[node & ~kLeafBit]);
-            } else {
-                const Node& nd = nodes[node];
-                box.lo = umin3(nd.box[0].lo, nd.box[1].lo);
-                box.hi = umax3(nd.box[0].hi, nd.box[1].hi);
+    const uint32_t count = ctrl[kCtrlSuperCount];
+    for (uint32_t blk = blockIdx.x; (size_t)blk * kLeaves2 < count; blk += gridDim.x) {
+        const uint32_t b0 = blk * kLeaves2;
+        const uint32_t m = count - b0 < (uint32_t)kLeaves2 ? count - b0 : (uint32_t)kLeaves2;
+        for (uint32_t base = 0; base < m; base += 64) {
+            const uint32_t k = base + threadIdx.x;
+            const size_t e = 3 * (size_t)(b0 + k);
+            const uint32_t node = k < m ? frontier[e] : kFrontierEnd;
+            const uint64_t endMask = __ballot(node == kFrontierEnd);
+            const uint32_t valid = endMask ? (uint32_t)__ffsll((unsigned long long)endMask) - 1u : 64u;
+            if (threadIdx.x < valid) {
+                const uint32_t lo = frontier[e + 1], hi = frontier[e + 2];
+                Box box;
+                if (node & kLeafBit) {
+                    box = tri_box(tris[node & ~kLeafBit]);
+                } else {
+                    const Node& nd = nodes[node];
+                    box.lo = umin3(nd.box[0].lo, nd.box[1].lo);
+                    box.hi = umax3(nd.box[0].hi, nd.box[1].hi);
+                }
+                climb_from(n, lo, hi, node, box, deltas, nodes, flags, root);
             }
-            climb_from(n, lo, hi, node, box, deltas, nodes, flags, root);
+            if (endMask) break;
         }
-        if (endMask) break;
     }
 }
 
@@ -511,22 +643,36 @@ __global__ void __launch_bounds__(kBuildBlock) k_climb_reset(uint32_t* flags, in
     if (i == 0) ctrl[kCtrlFrontier] = 0;
 }
 
-// fuseDeltas: compute the deltas from `rec` in the local phase and write
-// them to `deltas` (the 32-bit build); else read them from `deltas`.
+// fuseDeltas: compute the deltas from `rec` in level 1 and write them to
+// `deltas` (the 32-bit build); else read them from `deltas`.
 template <class TS>
 static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uint8_t* deltas, bool fuseDeltas,
                         Node* nodes, uint32_t* root, hipStream_t stream)
 {
+    const unsigned blocks1 = (unsigned)(((size_t)n + kLeaves1 - 1) / kLeaves1);
     int rc = ensure(ctx, &ctx->frontier, &ctx->frontierCap, (size_t)n * 3, "hipMalloc(climb frontier)");
+    if (!rc) rc = ensure(ctx, &ctx->superList, &ctx->superCap, (size_t)n * 3, "hipMalloc(super-leaf list)");
+    if (!rc) rc = ensure(ctx, &ctx->blockCounts, &ctx->blockCountsCap, 2 * (size_t)blocks1, "hipMalloc(block counts)");
     if (rc) return rc;
-    unsigned blocks = (unsigned)(((size_t)n + kLocalLeaves - 1) / kLocalLeaves);
+    uint32_t* counts = ctx->blockCounts;
+    uint32_t* offsets = ctx->blockCounts + blocks1;
     if (fuseDeltas)
-        k_climb_local<TS, true><<<blocks, kLocalLeaves, 0, stream>>>(tris, (uint32_t)n, rec, nullptr, deltas, nodes,
-                                                                     ctx->frontier, root);
+        k_local_nodes<TS, kLeaves1, kLeavesFusedDeltas><<<blocks1, kLeaves1, 0, stream>>>(
+            tris, (uint32_t)n, rec, nullptr, deltas, nullptr, nullptr, nodes, ctx->frontier, counts, root);
     else
-        k_climb_local<TS, false><<<blocks, kLocalLeaves, 0, stream>>>(tris, (uint32_t)n, rec, deltas, nullptr, nodes,
-                                                                      ctx->frontier, root);
-    k_climb_upper<<<blocks, 64, 0, stream>>>(tris, (uint32_t)n, deltas, nodes, ctx->flags, ctx->frontier, root);
+        k_local_nodes<TS, kLeaves1, kLeavesGivenDeltas><<<blocks1, kLeaves1, 0, stream>>>(
+            tris, (uint32_t)n, rec, deltas, nullptr, nullptr, nullptr, nodes, ctx->frontier, counts, root);
+    k_frontier_scan<<<1, 1024, 0, stream>>>(counts, blocks1, offsets, ctx->ctrl);
+    k_frontier_compact<<<blocks1, 64, 0, stream>>>(ctx->frontier, counts, offsets, ctx->superList, kLeaves1);
+    // level 2 and the upper climb loop over however many level-2 blocks the
+    // super-leaves fill (there are at most n super-leaves)
+    const size_t blocks2Max = ((size_t)n + kLeaves2 - 1) / kLeaves2;
+    const unsigned grid2 = (unsigned)std::min<size_t>(blocks2Max, (size_t)ctx->cuCount * 2);
+    k_local_nodes<TS, kLeaves2, kSuperLeaves><<<grid2, kLeaves2, 0, stream>>>(
+        tris, (uint32_t)n, nullptr, deltas, nullptr, ctx->superList, ctx->ctrl, nodes, ctx->frontier, nullptr, root);
+    const unsigned gridUp = (unsigned)std::min<size_t>(blocks2Max, (size_t)ctx->cuCount * 8);
+    k_climb_upper<<<gridUp, 64, 0, stream>>>(tris, (uint32_t)n, deltas, nodes, ctx->flags, ctx->frontier, ctx->ctrl,
+                                             root);
     return MLBVH_OK;
 }
 

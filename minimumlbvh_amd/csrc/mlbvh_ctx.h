@@ -33,8 +33,12 @@ struct mlbvh_ctx {
     size_t deltasCap = 0;
     uint32_t* flags = nullptr;  // per-internal-node arrival word of the climb
     size_t flagsCap = 0;
-    uint32_t* frontier = nullptr;  // (node, lo, hi) where the block-local climb stops, [n][3]
+    uint32_t* frontier = nullptr;  // (node, lo, hi) frontier entries of the block-local levels, [n][3]
     size_t frontierCap = 0;
+    uint32_t* superList = nullptr;  // the level-1 frontier in leaf order: the super-leaves, [n][3]
+    size_t superCap = 0;
+    uint32_t* blockCounts = nullptr;  // level-1 frontier counts and their offsets, [2][blocks]
+    size_t blockCountsCap = 0;
     uint32_t* sceneBox = nullptr;  // copies of 6 order-mapped uint32 (lo xyz, hi xyz), 256 words
 
     // --- batched build / trace ------------------------------------------------
@@ -58,6 +62,7 @@ constexpr int kCtrlEpochs = 32;   // [16]
 constexpr int kCtrlWorkQueue = 48;  // ray counter of the persistent trace
 constexpr int kCtrlFrontier = 49;   // entries of the climb's frontier list
 constexpr int kCtrlSortFallback = 50;  // set when a prefix segment is too long for the LDS sort
+constexpr int kCtrlSuperCount = 51;    // super-leaves of the build's second block-local level
 constexpr int kCtrlWords = 64;
 constexpr int kNumSortCfgs = 5;  // tile shapes of the radix sort (onesweep.hip)
 constexpr uint32_t kEpochLimit = (1u << 30) - 1;  // look-back words keep 30 tag bits

@@ -7,7 +7,8 @@
 // triangles (any sort tile shape, any CU count): the sort's control words and
 // look-back table first, at fixed offsets, because the pass tags they carry
 // stay valid from one call to the next; then the histograms, the records, the
-// deltas, the climb's arrival words and its frontier.  Every kernel of the
+// deltas, the climb's arrival words, its frontier and the level-1 block
+// counts (the super-leaf list reuses the sort's pong buffers).  Every kernel of the
 // build clears or overwrites what it reads before reading it (histograms,
 // tickets, arrival words, frontier count, scene box), so a call does not
 // depend on the contents a previous call left.  All calls on one GPU share
@@ -52,7 +53,14 @@ constexpr size_t kOffIdxB = kOffIdxA + up((size_t)kNoAllocMaxTris * 4);
 constexpr size_t kOffDeltas = kOffIdxB + up((size_t)kNoAllocMaxTris * 4);
 constexpr size_t kOffFlags = kOffDeltas + up((size_t)kNoAllocMaxTris);
 constexpr size_t kOffFrontier = kOffFlags + up((size_t)kNoAllocMaxTris * 4);
-constexpr size_t kScratchBytes = kOffFrontier + up((size_t)kNoAllocMaxTris * 3 * 4);
+constexpr size_t kOffCounts = kOffFrontier + up((size_t)kNoAllocMaxTris * 3 * 4);
+constexpr size_t kCountWords = 2 * ((size_t)kNoAllocMaxTris / 256 + 1);  // level-1 blocks of at least 256 leaves
+constexpr size_t kScratchBytes = kOffCounts + up(kCountWords * 4);
+// The super-leaf list ([n][3] words) shares the sort's pong buffers
+// (records B, indices A and B), which are dead once the sorted records are
+// in records A, before the climb starts.
+constexpr size_t kOffSuper = kOffRecB;
+static_assert(kOffDeltas - kOffSuper >= (size_t)kNoAllocMaxTris * 3 * 4, "the super-leaf list fits the pong buffers");
 // the zeroed prefix: control words, scene box, look-back table
 constexpr size_t kZeroBytes = kOffHist;
 static_assert(kScratchBytes < (48ull << 20), "scratch stays modest: under 48 MiB per GPU");
@@ -141,6 +149,10 @@ extern "C" int mlbvh_build_ex_noalloc(mlbvh_ctx* ctx, const void* d_triangles, i
     view.flagsCap = kNoAllocMaxTris;
     view.frontier = (uint32_t*)(base + kOffFrontier);
     view.frontierCap = (size_t)kNoAllocMaxTris * 3;
+    view.superList = (uint32_t*)(base + kOffSuper);
+    view.superCap = (size_t)kNoAllocMaxTris * 3;
+    view.blockCounts = (uint32_t*)(base + kOffCounts);
+    view.blockCountsCap = kCountWords;
     view.meshBox = nullptr;
     view.meshBoxCap = 0;
     return build_one(&view, d_triangles, n, buildOption, d_nodes, d_root, d_sorted_records, d_deltas,
