@@ -206,9 +206,12 @@ __global__ void __launch_bounds__(kBatchBlock)
     }
 }
 
-// Mesh-local agglomerative climb (build_lbvh, minimum_lbvh.h:284-361) in the
-// two phases of the single-mesh build (k_climb_local / k_climb_upper in
-// build.hip, where the block-local predicate is derived).  A block takes
+// Mesh-local agglomerative climb (build_lbvh, minimum_lbvh.h:284-361) in two
+// phases: a climb with the sibling hand-off in LDS for the nodes inside a
+// block, then the device-scope climb of the single-mesh build (climb_from)
+// for the rest; the block-local predicate is the one derived in build.hip.
+// (The single-mesh build has since replaced its in-block climb by building
+// each node on its own.)  A block takes
 // kBatchLeaves consecutive sorted positions, which may belong to several
 // meshes.  In the block's delta array a split between two meshes (or past
 // either end of the batch) holds 255, above every Morton delta (<= 32), so no
