@@ -210,8 +210,8 @@ __global__ void __launch_bounds__(kBatchBlock)
 // phases: a climb with the sibling hand-off in LDS for the nodes inside a
 // block, then the device-scope climb of the single-mesh build (climb_from)
 // for the rest; the block-local predicate is the one derived in build.hip.
-// (The single-mesh build has since replaced its in-block climb by building
-// each node on its own.)  A block takes
+// (The single-mesh build builds its in-block nodes each on its own instead,
+// k_local_nodes.)  A block takes
 // kBatchLeaves consecutive sorted positions, which may belong to several
 // meshes.  In the block's delta array a split between two meshes (or past
 // either end of the batch) holds 255, above every Morton delta (<= 32), so no
