@@ -64,6 +64,7 @@ constexpr int kCtrlFrontier = 49;   // entries of the climb's frontier list
 constexpr int kCtrlSortFallback = 50;  // set when a prefix segment is too long for the LDS sort
 constexpr int kCtrlSuperCount = 51;    // super-leaves of the build's second block-local level
 constexpr int kCtrlAoTiles = 52;       // next pixel tile of the persistent AO kernel
+constexpr int kCtrlPtTiles = 53;       // next pixel tile of the persistent path-tracing kernel
 constexpr int kCtrlWords = 64;
 constexpr int kNumSortCfgs = 5;  // tile shapes of the radix sort (onesweep.hip)
 constexpr uint32_t kEpochLimit = (1u << 30) - 1;  // look-back words keep 30 tag bits

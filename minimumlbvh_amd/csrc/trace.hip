@@ -772,16 +772,26 @@ static_assert(sizeof(Attrib) == 60, "TriangleAttrib is 60 bytes");
 __global__ void __launch_bounds__(kTraceBlock)
     k_render_pt(float4* acc, int w, int h, mlbvh_raygen cam, const uint32_t* __restrict__ rootPtr,
                 const Node* __restrict__ nodes, const Tri* __restrict__ tris, const Attrib* __restrict__ attribs,
-                int useSobol, int maxSPP, int maxDepth)
+                int useSobol, int maxSPP, int maxDepth, uint32_t* tileCounter)
 {
     __shared__ uint32_t s_stack[kLaneStack][kTraceBlock];
     __shared__ float s_entry[kLaneStack][kTraceBlock];
     uint32_t* const stk = &s_stack[0][threadIdx.x];
     float* const ent = &s_entry[0][threadIdx.x];
-    int xi, yi;
-    tile_pixel(&xi, &yi);
-    if (xi >= w || yi >= h) return;
     const uint32_t root = *rootPtr;
+    const int lane = threadIdx.x & 63;
+    // Persistent waves over 8x8 pixel tiles from the counter, in the order of
+    // the 16x16 groups a block grid would take them.
+    const uint32_t groupsX = (uint32_t)(w + 15) / 16, nTiles = groupsX * ((uint32_t)(h + 15) / 16) * 4;
+    while (true) {
+    uint32_t tile = 0;
+    if (lane == 0) tile = atomicAdd(tileCounter, 1u);
+    tile = __builtin_amdgcn_readfirstlane(tile);
+    if (tile >= nTiles) break;
+    const uint32_t grp = tile >> 2, sub = tile & 3;
+    const int xi = (int)((grp % groupsX) * 16 + (sub & 1) * 8 + (uint32_t)(lane & 7));
+    const int yi = (int)((grp / groupsX) * 16 + (sub >> 1) * 8 + (uint32_t)(lane >> 3));
+    if (xi >= w || yi >= h) continue;
     int pixel = xi + yi * w;
     float4 a = acc[pixel];
     int iteration = (int)a.w;
@@ -843,6 +853,7 @@ __global__ void __launch_bounds__(kTraceBlock)
         a.z += color.z;
         a.w += 1.0f;
         acc[pixel] = a;
+    }
     }
 }
 
@@ -1109,9 +1120,13 @@ extern "C" int mlbvh_render_pt(mlbvh_ctx* ctx, float* d_accumulators, int width,
         set_error("render_pt: invalid arguments");
         return MLBVH_E_INVALID_ARG;
     }
-    k_render_pt<<<image_grid(width, height), kTraceBlock, 0, (hipStream_t)stream>>>(
+    uint32_t* tileCounter = ctx->ctrl + kCtrlPtTiles;
+    k_set_word<<<1, 64, 0, (hipStream_t)stream>>>(tileCounter, 0u);
+    const dim3 g = image_grid(width, height);
+    const unsigned resident = (unsigned)ctx->cuCount * kAoPoolBlocksPerCU, groups = g.x * g.y;
+    k_render_pt<<<resident < groups ? resident : groups, kTraceBlock, 0, (hipStream_t)stream>>>(
         (float4*)d_accumulators, width, height, *camera, (const uint32_t*)d_root, (const Node*)d_nodes,
-        (const Tri*)d_triangles, (const Attrib*)d_attribs, useSobol, maxSPP, maxDepth);
+        (const Tri*)d_triangles, (const Attrib*)d_attribs, useSobol, maxSPP, maxDepth, tileCounter);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? MLBVH_OK : hip_status(e, "render_pt launch");
 }
