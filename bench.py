@@ -184,13 +184,15 @@ def main():
 
     # HBM bytes per k_render_ao_pool launch: FETCH_SIZE x2 + WRITE_SIZE x1 (copy-calibrated) from the
     # rocprofv3 --pmc runs of the same workload, summarised in profiles/
-    ao_traffic, ao_traffic_src = None, None
+    ao_traffic, ao_traffic_src, ao_l2_hit = None, None, None
     prof = os.path.join(ROOT, "profiles", "k_render_ao_pool_l9_1080p.txt")
     if args.level == 9 and os.path.exists(prof):
         for line in open(prof):
             if line.startswith("traffic_bytes_per_launch:"):
                 ao_traffic = int(float(line.split(":", 1)[1]))
                 ao_traffic_src = "profiles/k_render_ao_pool_l9_1080p.txt"
+            elif line.startswith("L2 hit rate:"):
+                ao_l2_hit = float(line.split(":", 1)[1].split()[0])
 
     out = {
         "metric": METRIC,
@@ -233,9 +235,11 @@ def main():
             "hbm_gbs": round(ao_traffic / (ao_ev / 1e3) / 1e9, 1) if ao_traffic else None,
             "hbm_frac": round(ao_traffic / (ao_ev / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if ao_traffic else None,
             "gather": gather,
+            "l2_hit_rate": ao_l2_hit,
             "note": "achieved = algorithmic bytes (64 B x node fetches + 36 B x triangle tests, counts equal to "
-                    "the oracle's walks) / kernel time.  93 % of the kernel's loads hit L2, so the roof is the "
-                    "L2 gather rate of 64-B nodes; HBM moves 1-2 % of its peak (hbm_frac)",
+                    "the oracle's walks) / kernel time.  Most of the kernel's loads hit L2 (l2_hit_rate, from the "
+                    "same profile as traffic), so the roof is the L2 gather rate of 64-B nodes; HBM moves a few "
+                    "percent of its peak (hbm_frac)",
         },
         "build": {
             "triangles": n,
