@@ -6,8 +6,8 @@
 //
 // Differences in structure from the reference (results are identical):
 //  * the Morton kernel also builds the four 8-bit digit histograms the sort
-//    needs and initialises the climb's arrival words, so the sort starts with
-//    its first reorder pass and no memset of the node array is needed;
+//    needs, so the sort starts with its first reorder pass; no memset of the
+//    node array is needed (every node is written whole once);
 //  * no climb inside blocks: nodes whose leaves lie in one block of 256
 //    sorted leaves are built each on its own from the deltas (ranges by
 //    nearest-greater searches, boxes by range unions in LDS), then the same
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(TS tris, int n, uint
 }
 
 // Morton record per triangle (minimum_lbvh.cu:68-92 / minimum_lbvh.h:534-553),
-// the 4 digit histograms of the 32-bit key, and the climb's arrival words.
+// and the 4 digit histograms of the 32-bit key.
 // A few blocks per CU, so that the histograms are merged into global memory
 // by few blocks (global atomics on 1024 shared counters are the cost
 // otherwise); each block takes whole sort tiles (tileKeys triangles,
@@ -118,8 +118,8 @@ constexpr int kMortonUnroll = MLBVH_MORTON_UNROLL;
 template <class TS>
 __global__ void __launch_bounds__(kMortonBlock)
     k_morton(TS tris, int n, const uint32_t* __restrict__ sceneBox, uint32_t opt,
-             uint64_t* __restrict__ rec, uint32_t* __restrict__ partials, uint32_t* __restrict__ flags,
-             uint32_t* __restrict__ ctrl, uint32_t tileKeys, uint32_t nTiles, uint32_t* __restrict__ tileCounts)
+             uint64_t* __restrict__ rec, uint32_t* __restrict__ partials, uint32_t tileKeys, uint32_t nTiles,
+             uint32_t* __restrict__ tileCounts)
 {
     __shared__ uint32_t s_h[4][256];
     __shared__ uint32_t s_t[256];  // this tile's counts of the first digit
@@ -140,8 +140,6 @@ __global__ void __launch_bounds__(kMortonBlock)
             morton = (uint32_t)(morton_in_box(lo, hi, center) >> 31);
         }
         rec[i] = (uint64_t)morton | ((uint64_t)i << 32);
-        if (i < (uint32_t)n - 1) flags[i] = 0xFFFFFFFFu;
-        if (i == 0) ctrl[kCtrlFrontier] = 0;
         atomicAdd(&s_t[morton & 0xFFu], 1u);
 #pragma unroll
         for (int p = 1; p < 4; p++) atomicAdd(&s_h[p][(morton >> (8 * p)) & 0xFFu], 1u);
@@ -174,12 +172,11 @@ __global__ void __launch_bounds__(kMortonBlock)
 
 // The 63-bit key variant (MLBVH_BUILD_OPTION_MORTON64, no reference
 // counterpart): the full Morton code of the centroid as a u64 key beside the
-// triangle index, for a key-value sort over bits [0, 63).  Also opens the
-// climb's arrival words, as k_morton does.
+// triangle index, for a key-value sort over bits [0, 63).
 template <class TS>
 __global__ void __launch_bounds__(kBuildBlock)
     k_morton64(TS tris, int n, const uint32_t* __restrict__ sceneBox, uint64_t* __restrict__ keys,
-               uint32_t* __restrict__ idx, uint32_t* __restrict__ flags, uint32_t* __restrict__ ctrl)
+               uint32_t* __restrict__ idx)
 {
     f3 lo, hi;
     fold_scene_box(sceneBox, &lo, &hi);
@@ -188,8 +185,6 @@ __global__ void __launch_bounds__(kBuildBlock)
         f3 center = ((tri.v[0] + tri.v[1]) + tri.v[2]) / 3.0f;
         keys[i] = morton_in_box(lo, hi, center);
         idx[i] = (uint32_t)i;
-        if (i < n - 1) flags[i] = 0xFFFFFFFFu;
-        if (i == 0) ctrl[kCtrlFrontier] = 0;
     }
 }
 
@@ -292,7 +287,7 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
     k_local_nodes(TS tris, uint32_t n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
                   uint8_t* __restrict__ deltasOut, const uint32_t* __restrict__ elems,
                   const uint32_t* __restrict__ ctrl, Node* __restrict__ nodes, uint32_t* __restrict__ frontier,
-                  uint32_t* __restrict__ blockCounts, uint32_t* __restrict__ root)
+                  uint32_t* __restrict__ blockCounts, uint32_t* __restrict__ flags, uint32_t* __restrict__ root)
 {
     constexpr int kLift = log2_exact(kLeaves) + 1;  // windows of 1 .. kLeaves deltas
     static_assert(kLeaves == 1 << (kLift - 1) && kLeaves >= 256 && kLeaves <= 1024,
@@ -428,6 +423,11 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
             local = jL >= 0 && jR <= (int)m;
             s_loc[t] = (uint8_t)local;
         }
+        // Level 2: every split it does not build (one crossing its block, or
+        // the one after its last element) is built by the upper climb, which
+        // needs its arrival word open; no other node uses one.
+        if (kMode == kSuperLeaves && ((isSplit && !local) || ((uint32_t)t == m - 1 && (size_t)b0 + m < count)))
+            flags[elemHi] = 0xFFFFFFFFu;
         lds_barrier();
 
         // Parents, child slots and frontier entries.
@@ -634,15 +634,6 @@ __global__ void __launch_bounds__(64)
     }
 }
 
-// Zero the frontier count and open every internal node's arrival word (the
-// build does both in k_morton).
-__global__ void __launch_bounds__(kBuildBlock) k_climb_reset(uint32_t* flags, int nInternals, uint32_t* ctrl)
-{
-    int i = blockIdx.x * kBuildBlock + threadIdx.x;
-    if (i < nInternals) flags[i] = 0xFFFFFFFFu;
-    if (i == 0) ctrl[kCtrlFrontier] = 0;
-}
-
 // fuseDeltas: compute the deltas from `rec` in level 1 and write them to
 // `deltas` (the 32-bit build); else read them from `deltas`.
 template <class TS>
@@ -658,10 +649,10 @@ static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uin
     uint32_t* offsets = ctx->blockCounts + blocks1;
     if (fuseDeltas)
         k_local_nodes<TS, kLeaves1, kLeavesFusedDeltas><<<blocks1, kLeaves1, 0, stream>>>(
-            tris, (uint32_t)n, rec, nullptr, deltas, nullptr, nullptr, nodes, ctx->frontier, counts, root);
+            tris, (uint32_t)n, rec, nullptr, deltas, nullptr, nullptr, nodes, ctx->frontier, counts, nullptr, root);
     else
         k_local_nodes<TS, kLeaves1, kLeavesGivenDeltas><<<blocks1, kLeaves1, 0, stream>>>(
-            tris, (uint32_t)n, rec, deltas, nullptr, nullptr, nullptr, nodes, ctx->frontier, counts, root);
+            tris, (uint32_t)n, rec, deltas, nullptr, nullptr, nullptr, nodes, ctx->frontier, counts, nullptr, root);
     k_frontier_scan<<<1, 1024, 0, stream>>>(counts, blocks1, offsets, ctx->ctrl);
     k_frontier_compact<<<blocks1, 64, 0, stream>>>(ctx->frontier, counts, offsets, ctx->superList, kLeaves1);
     // level 2 and the upper climb loop over however many level-2 blocks the
@@ -669,7 +660,8 @@ static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uin
     const size_t blocks2Max = ((size_t)n + kLeaves2 - 1) / kLeaves2;
     const unsigned grid2 = (unsigned)std::min<size_t>(blocks2Max, (size_t)ctx->cuCount * 2);
     k_local_nodes<TS, kLeaves2, kSuperLeaves><<<grid2, kLeaves2, 0, stream>>>(
-        tris, (uint32_t)n, nullptr, deltas, nullptr, ctx->superList, ctx->ctrl, nodes, ctx->frontier, nullptr, root);
+        tris, (uint32_t)n, nullptr, deltas, nullptr, ctx->superList, ctx->ctrl, nodes, ctx->frontier, nullptr,
+        ctx->flags, root);
     const unsigned gridUp = (unsigned)std::min<size_t>(blocks2Max, (size_t)ctx->cuCount * 8);
     k_climb_upper<<<gridUp, 64, 0, stream>>>(tris, (uint32_t)n, deltas, nodes, ctx->flags, ctx->frontier, ctx->ctrl,
                                              root);
@@ -715,8 +707,7 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
     if (wide) {
         if ((rc = ensure(ctx, &ctx->idxA, &ctx->idxCap, nn, "hipMalloc(indices)"))) return rc;
         if ((rc = ensure(ctx, &ctx->idxB, &ctx->idxBCap, nn, "hipMalloc(indices tmp)"))) return rc;
-        k_morton64<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, ctx->recB, ctx->idxA, ctx->flags,
-                                                        ctx->ctrl);
+        k_morton64<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, ctx->recB, ctx->idxA);
         // keys in recB (tmp: recA), records written to recA afterwards
         rc = sort_pairs_u64(ctx, ctx->recB, ctx->idxA, ctx->recA, ctx->idxB, (uint32_t)n, 0, 63, stream);
         if (rc) return rc;
@@ -727,8 +718,8 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         // partials: gm blocks x 4 histograms, then the per-tile counts
         if ((rc = ensure_partials(ctx, (int)(gm * 4 + tiles), 1))) return rc;
         uint32_t* tileCounts = ctx->partials + (size_t)gm * 4 * 256;
-        k_morton<<<gm, kMortonBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials, ctx->flags,
-                                                 ctx->ctrl, (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
+        k_morton<<<gm, kMortonBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials,
+                                                 (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
         launch_hist_reduce(ctx, (int)gm, 4, stream);
         rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts);
         if (rc) return rc;
@@ -787,8 +778,6 @@ static int refit_impl(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* d_sorted_r
     int rc = set_device(ctx);
     if (rc) return rc;
     if ((rc = ensure(ctx, &ctx->flags, &ctx->flagsCap, (size_t)n, "hipMalloc(flags)"))) return rc;
-    unsigned g = grid1((size_t)n);
-    k_climb_reset<<<g, kBuildBlock, 0, s>>>(ctx->flags, n - 1, ctx->ctrl);
     if ((rc = launch_climb(ctx, tris, n, d_sorted_records, const_cast<uint8_t*>(d_deltas), false, (Node*)d_nodes,
                            (uint32_t*)d_root, s)))
         return rc;
