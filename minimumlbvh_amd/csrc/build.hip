@@ -128,6 +128,19 @@ __global__ void __launch_bounds__(kMortonBlock)
     if (t < 256) s_t[t] = 0;
     f3 lo, hi;
     fold_scene_box(sceneBox, &lo, &hi);  // its barrier also orders the clears above
+    // One LDS atomic per wave when the active lanes share the digit (the high
+    // digits of spatially ordered triangles mostly do), else one per lane.
+    auto hist_add = [&](uint32_t* h, uint32_t d) {
+        const uint64_t active = __ballot(1);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        const uint64_t same = __ballot(d == d0);
+        if (same == active) {
+            if ((uint32_t)__lane_id() == (uint32_t)__ffsll((unsigned long long)active) - 1u)
+                atomicAdd(&h[d0], (uint32_t)__popcll(active));
+        } else {
+            atomicAdd(&h[d], 1u);
+        }
+    };
     auto record = [&](const Tri& tri, uint32_t i) {
         f3 center = ((tri.v[0] + tri.v[1]) + tri.v[2]) / 3.0f;
         uint32_t morton;
@@ -140,9 +153,9 @@ __global__ void __launch_bounds__(kMortonBlock)
             morton = (uint32_t)(morton_in_box(lo, hi, center) >> 31);
         }
         rec[i] = (uint64_t)morton | ((uint64_t)i << 32);
-        atomicAdd(&s_t[morton & 0xFFu], 1u);
+        hist_add(s_t, morton & 0xFFu);
 #pragma unroll
-        for (int p = 1; p < 4; p++) atomicAdd(&s_h[p][(morton >> (8 * p)) & 0xFFu], 1u);
+        for (int p = 1; p < 4; p++) hist_add(s_h[p], (morton >> (8 * p)) & 0xFFu);
     };
     for (uint32_t tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
         const uint32_t t0 = tile * tileKeys;
