@@ -295,12 +295,58 @@ enum LocalMode {
     kSuperLeaves = 2,        // the super-leaves (node, lo, hi) of `elems`, ctrl[kCtrlSuperCount] of them
 };
 
+// Single-word decoupled look-back of the build's ordered workgroups (the
+// level-1 frontier compaction, the level-2 blocks).  Word of workgroup j:
+// tag << 32 | prefix flag << 31 | value, value = its own count (flag clear)
+// or the count of all workgroups up to and including it (flag set).  The
+// tag changes with every build, so the words are never cleared (a zeroed
+// word carries tag 0, which no build uses).
+constexpr uint64_t kLbPrefix = 1ull << 31;
+__device__ inline uint64_t lb_word(uint32_t tag, bool prefix, uint32_t value)
+{
+    return ((uint64_t)tag << 32) | (prefix ? kLbPrefix : 0ull) | (uint64_t)(value & 0x7FFFFFFFu);
+}
+
+// Sum of the counts of workgroups 0 .. idx - 1, by the 64 lanes of one wave
+// (the loop is the same in every lane): lane l reads workgroup base - l;
+// the lanes up to the nearest published prefix (lane 0 is the nearest) must
+// all be published, then their values are summed.  Workgroups before the
+// first count as a published prefix of 0.  The caller orders its workgroups
+// by ticket, so every workgroup waited on is running and publishes its count
+// before it waits on anything.
+__device__ inline uint32_t lb_exclusive(const uint64_t* status, uint32_t idx, uint32_t tag, int lane)
+{
+    uint32_t excl = 0;
+    int64_t base = (int64_t)idx - 1;
+    while (base >= 0) {
+        const int64_t j = base - lane;
+        const uint64_t w =
+            j >= 0 ? __hip_atomic_load(status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : lb_word(tag, true, 0u);
+        const bool pub = (uint32_t)(w >> 32) == tag;
+        const uint64_t pubMask = __ballot(pub);
+        const uint64_t preMask = __ballot(pub && (w & kLbPrefix) != 0);
+        const int first = preMask ? __ffsll((unsigned long long)preMask) - 1 : 64;
+        const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+        if ((pubMask & need) == need) {
+            uint32_t v = lane <= first ? (uint32_t)(w & 0x7FFFFFFFu) : 0u;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            excl += v;
+            base = first < 64 ? -1 : base - 64;
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return excl;
+}
+
 template <class TS, int kLeaves, int kMode>
 __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kLeaves <= 256 ? 8 : 1)))
     k_local_nodes(TS tris, uint32_t n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
-                  uint8_t* __restrict__ deltasOut, const uint32_t* __restrict__ elems,
-                  const uint32_t* __restrict__ ctrl, Node* __restrict__ nodes, uint32_t* __restrict__ frontier,
-                  uint32_t* __restrict__ blockCounts, uint32_t* __restrict__ flags, uint32_t* __restrict__ root)
+                  uint8_t* __restrict__ deltasOut, const uint32_t* __restrict__ elems, uint32_t* __restrict__ ctrl,
+                  Node* __restrict__ nodes, uint32_t* __restrict__ frontier, uint32_t* __restrict__ blockCounts,
+                  uint32_t* __restrict__ flags, uint32_t* __restrict__ root, uint64_t* __restrict__ status2,
+                  uint32_t* __restrict__ topList)
 {
     constexpr int kLift = log2_exact(kLeaves) + 1;  // windows of 1 .. kLeaves deltas
     static_assert(kLeaves == 1 << (kLift - 1) && kLeaves >= 256 && kLeaves <= 1024,
@@ -320,17 +366,29 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
     // kSuperLeaves: s_aux[j] = first leaf of element b0 + j, s_aux[m] = last leaf of the block + 1
     __shared__ uint32_t s_aux[kLeaves + 2];
     __shared__ uint32_t s_wcount[kWaves];
+    __shared__ uint32_t s_rank[kMode == kSuperLeaves ? kLeaves : 1];  // level 2: rank of the entry at element j
+    __shared__ uint32_t s_blk, s_excl;  // level 2: this block (by ticket), its entries' offset in the top list
     // The frontier entry starting at element j: its node and last leaf (its
     // first leaf is element j's), in the sparse table's LDS once that is dead.
     uint2* s_ent = reinterpret_cast<uint2*>(&s_st[0][0]);
     static_assert(sizeof(uint2) * kLeaves <= sizeof(s_st), "the entries fit the sparse table's LDS");
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const uint32_t count = kMode == kSuperLeaves ? ctrl[kCtrlSuperCount] : n;
+    constexpr bool kLevel1 = kMode != kSuperLeaves;
+    const uint32_t count = kLevel1 ? n : ctrl[kCtrlSuperCount];
     const uint32_t nInternals = n - 1;
-    // Level 1 has one block per kLeaves leaves; level 2 loops over its blocks
-    // (their number is known on the device only).
-    for (uint32_t blk = blockIdx.x; (size_t)blk * kLeaves < count; blk += gridDim.x) {
+    // Level 1 has one block per kLeaves leaves; level 2 takes its blocks by
+    // ticket (their number is known on the device only, and a block looks
+    // back only at blocks that are running).
+    uint32_t blk = blockIdx.x, tag = 0;
+    if (!kLevel1) {
+        if (t == 0) s_blk = atomicAdd(&ctrl[kCtrlL2Ticket], 1u);
+        tag = ctrl[kCtrlCompactTag] + 1u;
+        lds_barrier();
+        blk = s_blk;
+    }
+    const uint32_t nBlocks = (uint32_t)(((size_t)count + kLeaves - 1) / kLeaves);
+    while (blk < nBlocks) {
         const uint32_t b0 = blk * kLeaves;
         const uint32_t m = count - b0 < (uint32_t)kLeaves ? count - b0 : (uint32_t)kLeaves;
         const uint32_t s = b0 + (uint32_t)t;
@@ -445,7 +503,7 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
 
         // Parents, child slots and frontier entries.
         bool emitElem = false, emitNode = false, isRoot = false;
-        uint32_t parent = kInvalidNode, gNode = 0, gLo = 0, gHi = 0;
+        uint32_t parent = kInvalidNode, gNode = 0, gLo = 0, gHi = 0, dAfter = kDeltaInf;
         const uint32_t dNext = s_st[0][t + 1];  // d[b0 + t] (level 1), kept for deltasOut
         if (isElem) {
             const uint32_t dL = s_st[0][t], dR = dNext;
@@ -465,7 +523,8 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
             if (isRoot) {
                 *root = gNode;
             } else {
-                const int goLeft = s_st[0][jL] < s_st[0][jR] ? 1 : 0;
+                dAfter = s_st[0][jR];
+                const int goLeft = s_st[0][jL] < dAfter ? 1 : 0;
                 parent = goLeft ? gLo - 1 : gHi;  // the last leaf left of the range, or the range's last leaf
                 const uint32_t slot = goLeft ? (uint32_t)jL - 1u : (uint32_t)jR - 1u;  // wraps for jL == 0
                 const bool pLocal = slot < m - 1u && s_loc[slot] != 0;
@@ -494,8 +553,15 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
             before += w < wave ? c : 0u;
             total += c;
         }
+        const uint32_t rank = before + (uint32_t)__popcll(markMask & ((1ull << lane) - 1ull));
+        if (!kLevel1) {  // the count goes out now, for the successors' look-back
+            if (t == 0)
+                __hip_atomic_store(status2 + blk, lb_word(tag, blk == 0, total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (marked) s_rank[t] = rank;
+        }
         if (marked) {
-            const uint32_t k = b0 + before + (uint32_t)__popcll(markMask & ((1ull << lane) - 1ull));
+            const uint32_t k = b0 + rank;
             const uint2 e = s_ent[t];
             frontier[3 * (size_t)k] = e.x;
             frontier[3 * (size_t)k + 1] = elemLo;  // the entry's first leaf is element t's
@@ -503,10 +569,11 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
         }
         if (t == 0) {
             if (total < m) frontier[3 * (size_t)(b0 + total)] = kFrontierEnd;
-            if (blockCounts) blockCounts[blk] = total;
+            blockCounts[blk] = total;
         }
 
         // The block-local node, whole: child boxes over elements [jL, t] and [t + 1, jR - 1].
+        Box entBox;  // level 2: the subtree box of this thread's node entry
         if (local) {
             auto range_box = [&](uint32_t a, uint32_t b) -> Box {  // local elements a .. b
                 KeyBox acc = {{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, {0u, 0u, 0u}};
@@ -523,6 +590,8 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
                 return out;
             };
             const Box l = range_box((uint32_t)jL, (uint32_t)t), r = range_box((uint32_t)t + 1, (uint32_t)jR - 1u);
+            entBox.lo = umin3(l.lo, r.lo);
+            entBox.hi = umax3(l.hi, r.hi);
             const uint32_t context = gHi < nInternals ? gHi : 0xFFFFFFFFu;
             uint4* dst = reinterpret_cast<uint4*>(nodes + gNode);
             dst[0] = make_uint4(s_child[t][0], s_child[t][1], __float_as_uint(l.lo.x), __float_as_uint(l.lo.y));
@@ -534,92 +603,309 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
         }
         // d[b0 + t], stored last: nothing in the block waits for a global store
         if (kMode == kLeavesFusedDeltas && isElem && s < nInternals) deltasOut[s] = (uint8_t)dNext;
-        if (kMode != kSuperLeaves) break;  // one block per workgroup: no loop code
-        lds_barrier();                     // the next block of the loop reuses the LDS
+        if (kLevel1) break;  // one block per workgroup: no loop code
+        // Level 2: this block's entries go to the top list, after those of
+        // the blocks before it (found by the look-back); the last block
+        // writes the list's length, and whether it overflows the top tree.
+        if (wave == 0) {
+            const uint32_t excl = lb_exclusive(status2, blk, tag, lane);
+            if (lane == 0) {
+                s_excl = excl;
+                __hip_atomic_store(status2 + blk, lb_word(tag, true, excl + total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (blk == nBlocks - 1) {
+                    ctrl[kCtrlTopCount] = excl + total;
+                    ctrl[kCtrlTopFallback] = excl + total > kTopMax ? 1u : 0u;
+                }
+            }
+        }
+        lds_barrier();
+        auto put_record = [&](uint32_t k, uint32_t node, uint32_t hi, uint32_t d, const Box& b) {
+            if (k >= kTopMax) return;
+            uint4* dst = reinterpret_cast<uint4*>(topList + (size_t)k * kTopWords);
+            dst[0] = make_uint4(node, hi, d, 0u);
+            dst[1] = make_uint4(__float_as_uint(b.lo.x), __float_as_uint(b.lo.y), __float_as_uint(b.lo.z),
+                                __float_as_uint(b.hi.x));
+            dst[2] = make_uint4(__float_as_uint(b.hi.y), __float_as_uint(b.hi.z), 0u, 0u);
+        };
+        if (emitElem) {
+            Box eb;
+            eb.lo = mk(ord2f(kb.lo[0]), ord2f(kb.lo[1]), ord2f(kb.lo[2]));
+            eb.hi = mk(ord2f(kb.hi[0]), ord2f(kb.hi[1]), ord2f(kb.hi[2]));
+            put_record(s_excl + rank, elemNode, elemHi, dNext, eb);
+        }
+        if (emitNode) put_record(s_excl + s_rank[jL], gNode, gHi, dAfter, entBox);
+        lds_barrier();  // the next block of the loop reuses the LDS
+        if (t == 0) s_blk = atomicAdd(&ctrl[kCtrlL2Ticket], 1u);
+        lds_barrier();
+        blk = s_blk;
     }
 }
 
-// Exclusive scan of the level-1 blocks' frontier counts into offsets, and
-// their total (the number of super-leaves) into ctrl[kCtrlSuperCount].  One
-// block, in tiles of kScanTile counts: coalesced loads into LDS, each thread
-// scans a contiguous run of kScanTile / 1024 there, coalesced stores.
-constexpr uint32_t kScanTile = 16384;
-__global__ void __launch_bounds__(1024)
-    k_frontier_scan(const uint32_t* __restrict__ counts, uint32_t nBlocks, uint32_t* __restrict__ offsets,
-                    uint32_t* __restrict__ ctrl)
+// The level-1 frontier stretches (entries of block b at b * leavesPerBlock,
+// counts[b] of them, each stretch in leaf order) laid out in leaf order as
+// the super-leaf list, and its length in ctrl[kCtrlSuperCount].  A
+// workgroup takes kCompactBlocks level-1 blocks (by ticket, so it waits only
+// on workgroups that are running): it scans their counts, publishes its total
+// at once, finds its offset by summing its predecessors' published totals
+// (or a published prefix) 64 at a time, and copies its entries with all its
+// threads, a few per thread (lb_exclusive).
+constexpr int kCompactBlocks = 256;    // level-1 blocks per workgroup
+constexpr int kCompactThreads = 1024;  // about 9 entries per level-1 block: about 2 per thread
+__global__ void __launch_bounds__(kCompactThreads)
+    k_frontier_compact(const uint32_t* __restrict__ counts, uint32_t nBlocks, const uint32_t* __restrict__ frontier,
+                       uint32_t* __restrict__ superList, uint32_t leavesPerBlock, uint64_t* __restrict__ status,
+                       uint32_t* __restrict__ ctrl)
 {
-    constexpr int kWaves = 16;
-    constexpr uint32_t kPer = kScanTile / 1024;
-    __shared__ uint32_t s_c[kScanTile];
-    __shared__ uint32_t s_w[kWaves];
+    constexpr int kScanWaves = kCompactBlocks / 64;
+    __shared__ uint32_t s_off[kCompactBlocks];
+    __shared__ uint32_t s_w[kScanWaves];
+    __shared__ uint32_t s_tile, s_excl;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nBlocks; base += kScanTile) {
-        const uint32_t m = nBlocks - base < kScanTile ? nBlocks - base : kScanTile;
-#pragma unroll
-        for (uint32_t q = 0; q < kPer; q++) {
-            const uint32_t i = q * 1024 + (uint32_t)t;
-            s_c[i] = i < m ? counts[base + i] : 0u;
-        }
-        __syncthreads();
-        uint32_t v[kPer], sum = 0;
-#pragma unroll
-        for (uint32_t q = 0; q < kPer; q++) {
-            v[q] = s_c[(uint32_t)t * kPer + q];
-            sum += v[q];
-        }
-        uint32_t x = sum;  // inclusive scan over the wave
+    if (t == 0) s_tile = atomicAdd(&ctrl[kCtrlCompactTicket], 1u);
+    const uint32_t tag = ctrl[kCtrlCompactTag] + 1u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (t < kCompactBlocks) {  // scan of the tile's counts, waves 0 .. kScanWaves - 1
+        const uint32_t b = tile * kCompactBlocks + (uint32_t)t;
+        const uint32_t c = b < nBlocks ? counts[b] : 0u;
+        uint32_t x = c;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t y = __shfl_up(x, o, 64);
             if (lane >= o) x += y;
         }
         if (lane == 63) s_w[wave] = x;
-        __syncthreads();
-        uint32_t run = carry + x - sum, total = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; w++) {
-            const uint32_t c = s_w[w];
-            run += w < wave ? c : 0u;
-            total += c;
-        }
-#pragma unroll
-        for (uint32_t q = 0; q < kPer; q++) {
-            s_c[(uint32_t)t * kPer + q] = run;
-            run += v[q];
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t q = 0; q < kPer; q++) {
-            const uint32_t i = q * 1024 + (uint32_t)t;
-            if (i < m) offsets[base + i] = s_c[i];
-        }
-        carry += total;
-        __syncthreads();
+        s_off[t] = x - c;  // within the wave, for now
     }
-    if (t == 0) ctrl[kCtrlSuperCount] = carry;
+    __syncthreads();
+    uint32_t total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanWaves; w++) total += s_w[w];
+    if (t < kCompactBlocks) {
+        uint32_t before = 0;
+#pragma unroll
+        for (int w = 0; w < kScanWaves; w++) before += w < wave ? s_w[w] : 0u;
+        s_off[t] += before;
+    }
+    if (t == 0)
+        __hip_atomic_store(status + tile, lb_word(tag, tile == 0, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (wave == 0) {
+        const uint32_t excl = lb_exclusive(status, tile, tag, lane);
+        if (lane == 0) {
+            s_excl = excl;
+            __hip_atomic_store(status + tile, lb_word(tag, true, excl + total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            if ((size_t)(tile + 1) * kCompactBlocks >= nBlocks) ctrl[kCtrlSuperCount] = excl + total;
+        }
+    }
+    __syncthreads();
+    // Entry e of this tile: the last block k with offset <= e holds it.  Each
+    // round of loads is issued for all of the thread's entries before use.
+    const uint32_t excl = s_excl;
+    constexpr int kUnroll = 4;
+    for (uint32_t e0 = 0; e0 < total; e0 += kUnroll * kCompactThreads) {
+        size_t src[kUnroll];
+        uint32_t v[kUnroll][3];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint32_t e = e0 + (uint32_t)(u * kCompactThreads + t);
+            src[u] = 0;
+            if (e < total) {
+                uint32_t lo = 0, hi = kCompactBlocks - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) / 2;
+                    if (s_off[mid] <= e) lo = mid;
+                    else hi = mid - 1;
+                }
+                src[u] = 3 * ((size_t)(tile * kCompactBlocks + lo) * leavesPerBlock + (e - s_off[lo]));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint32_t e = e0 + (uint32_t)(u * kCompactThreads + t);
+            if (e < total) {
+                v[u][0] = frontier[src[u]];
+                v[u][1] = frontier[src[u] + 1];
+                v[u][2] = frontier[src[u] + 2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint32_t e = e0 + (uint32_t)(u * kCompactThreads + t);
+            if (e < total) {
+                const size_t d = 3 * ((size_t)excl + e);
+                superList[d] = v[u][0];
+                superList[d + 1] = v[u][1];
+                superList[d + 2] = v[u][2];
+            }
+        }
+    }
 }
 
-// One wave per level-1 block: its frontier stretch, already in leaf order, to
-// its place in the super-leaf list.
-__global__ void __launch_bounds__(64)
-    k_frontier_compact(const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ counts,
-                       const uint32_t* __restrict__ offsets, uint32_t* __restrict__ superList, uint32_t leavesPerBlock)
+// The nodes above level 2, in one workgroup.  Level 2's frontier entries, in
+// leaf order in the top list (node, last leaf, delta of the split after it,
+// subtree box), are the elements of a last sequence: entry j covers the
+// leaves [lo_j, hi_j], and the split between entries j and j + 1 is internal
+// node hi_j with delta d[hi_j].  The climb runs over entry indices with its
+// hand-offs in LDS: at each split the first of the two arrivers leaves its
+// subtree box and far end there and stops, the second merges and goes on.
+// Each node's fields are written by the threads that fill them (child and box
+// slots by the arriving child, the context by the second arriver, the parent
+// by the next step up), as in the device-scope climb.  With more than
+// kTopMax entries level 2 has set ctrl[kCtrlTopFallback] instead, and
+// k_climb_upper climbs them through device memory.
+//
+// The kernel also returns the build's tickets to zero and advances its
+// look-back tag, for the next build on this context.
+constexpr int kTopThreads = 1024;
+constexpr int kTopPer = (int)kTopMax / kTopThreads;  // entries per thread
+constexpr uint32_t kTopReady = 0x80000000u;           // the first arriver's deposit is in place
+constexpr uint32_t kTopNoSplit = 0x100u;              // a missing split's delta: above every delta
+__global__ void __launch_bounds__(kTopThreads)
+    k_top_tree(uint32_t n, Node* __restrict__ nodes, const uint32_t* __restrict__ topList, uint32_t* ctrl,
+               uint32_t* root)
 {
-    const uint32_t b = blockIdx.x;
-    const size_t words = 3 * (size_t)counts[b];
-    const size_t src = 3 * (size_t)b * leavesPerBlock, dst = 3 * (size_t)offsets[b];
-    for (size_t k = threadIdx.x; k < words; k += 64) superList[dst + k] = frontier[src + k];
+    __shared__ Box s_box[kTopMax];       // per split: the first arriver's subtree box
+    __shared__ uint32_t s_ctl[kTopMax];  // per split: arrivals (bits 0..7) | far end << 8 | kTopReady
+    // entry j: its last leaf hi_j (the node index of split j) | the delta of split j << 32, read as
+    // above every delta (kTopNoSplit) for the last entry, which has no split after it
+    __shared__ uint64_t s_pk[kTopMax];
+
+    const int t = threadIdx.x;
+    // Entries kTopPer * t .. kTopPer * t + kTopPer - 1 (consecutive entries
+    // are often siblings, so a thread's climbs tend to meet), loaded before
+    // their number is known (the list holds kTopMax records).
+    uint4 rec[kTopPer][3];
+#pragma unroll
+    for (int q = 0; q < kTopPer; q++) {
+        const uint4* src = reinterpret_cast<const uint4*>(topList + (size_t)(kTopPer * t + q) * kTopWords);
+#pragma unroll
+        for (int k = 0; k < 3; k++) rec[q][k] = src[k];
+    }
+    // no super-leaves: level 1 built the root, level 2 ran no block and wrote no count
+    const bool none = ctrl[kCtrlSuperCount] == 0;
+    const uint32_t F = none ? 0u : ctrl[kCtrlTopCount];
+    if (t == 0) {  // the next build starts from ticket 0 with a fresh tag
+        if (none) ctrl[kCtrlTopFallback] = 0u;
+        ctrl[kCtrlCompactTicket] = 0u;
+        ctrl[kCtrlL2Ticket] = 0u;
+        const uint32_t tag = ctrl[kCtrlCompactTag] + 1u;
+        ctrl[kCtrlCompactTag] = tag == 0xFFFFFFFFu ? 0u : tag;  // the next build's words carry tag + 1 != 0
+    }
+    if (F == 0 || F > kTopMax) return;  // the root was built below, or k_climb_upper climbs
+    uint32_t eNode[kTopPer];
+    Box eBox[kTopPer];
+#pragma unroll
+    for (int q = 0; q < kTopPer; q++) {
+        const uint32_t i = (uint32_t)(kTopPer * t + q);
+        eNode[q] = rec[q][0].x;
+        eBox[q].lo = mk(__uint_as_float(rec[q][1].x), __uint_as_float(rec[q][1].y), __uint_as_float(rec[q][1].z));
+        eBox[q].hi = mk(__uint_as_float(rec[q][1].w), __uint_as_float(rec[q][2].x), __uint_as_float(rec[q][2].y));
+        if (i < F) {
+            const uint32_t d = rec[q][0].z == kDeltaInf ? kTopNoSplit : rec[q][0].z;
+            s_pk[i] = (uint64_t)rec[q][0].y | ((uint64_t)d << 32);
+            s_ctl[i] = 0u;
+        }
+    }
+    lds_barrier();
+
+    // The climb is a loop of steps that the whole wave runs together (its
+    // condition is a ballot), and no lane blocks inside a step: a second
+    // arriver whose sibling's deposit is not in place yet checks again in the
+    // next step.  A wait loop inside the step would hang when both arrivers
+    // share a wave, since the compiler may run the waiting lanes' side of the
+    // branch before the depositing lanes' side; the ballot, a convergent
+    // operation, keeps the loop from being rewritten into such a wait.  A lane
+    // whose entry stops (first arriver, or root) takes its next entry.
+    const uint32_t nInternals = n - 1;
+    const uint32_t i0 = (uint32_t)(kTopPer * t);
+    int q = 0;                // the entry being climbed: i0 + q
+    bool active = i0 < F;     // an entry of this lane is climbing
+    bool waiting = false;     // arrived second at split p, the sibling's deposit not yet read
+    bool pendingCtx = false;  // `node` was just completed: its context is written with the next step
+    uint32_t a = i0, b = i0, node = eNode[0], p = 0, pNode = 0;
+    Box box = eBox[0];
+    auto merge = [&](uint32_t w) {  // the sibling's far end (in w) and box: `node` becomes p
+        const uint32_t other = (w & ~kTopReady) >> 8;
+        const Box ob = s_box[p];
+        a = a < other ? a : other;
+        b = b > other ? b : other;
+        box.lo = umin3(box.lo, ob.lo);
+        box.hi = umax3(box.hi, ob.hi);
+        node = pNode;
+        pendingCtx = true;
+        waiting = false;
+    };
+    while (__ballot(active) != 0) {
+        bool progressed = false, done = false;
+        if (active && !waiting) {
+            progressed = true;
+            const uint64_t R = s_pk[b];
+            const uint64_t L = a > 0 ? s_pk[a - 1] : ((uint64_t)kTopNoSplit << 32);
+            const uint32_t myHi = (uint32_t)R;  // the last leaf of [a, b]
+            if (pendingCtx) nodes[node].context = myHi < nInternals ? myHi : 0xFFFFFFFFu;
+            pendingCtx = false;
+            if (a == 0 && b == F - 1) {
+                *root = node;
+                if (!(node & kLeafBit)) nodes[node].parent = kInvalidNode;
+                done = true;
+            } else {
+                const uint32_t goLeft = (uint32_t)(L >> 32) < (uint32_t)(R >> 32) ? 1u : 0u;  // parent: split a - 1 or b
+                p = goLeft ? a - 1 : b;
+                pNode = goLeft ? (uint32_t)L : myHi;
+                Node* pn = nodes + pNode;
+                pn->child[goLeft] = node;
+                pn->box[goLeft] = box;
+                if (!(node & kLeafBit)) nodes[node].parent = pNode;
+                const uint32_t prior =
+                    __hip_atomic_fetch_add(&s_ctl[p], 1u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if ((prior & 0xFFu) == 0) {  // first: deposit the box and the far end, and stop
+                    s_box[p] = box;
+                    const uint32_t far = goLeft ? b : a;
+                    __hip_atomic_fetch_or(&s_ctl[p], kTopReady | (far << 8), __ATOMIC_RELEASE,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                    done = true;
+                } else if (prior & kTopReady) {  // second, and the deposit is in place
+                    merge(prior);
+                } else {
+                    waiting = true;
+                }
+            }
+        }
+        if (active && waiting) {
+            const uint32_t w = __hip_atomic_load(&s_ctl[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (w & kTopReady) {
+                merge(w);
+                progressed = true;
+            }
+        }
+        if (done) {  // the next entry of this lane, if any
+            q++;
+            const uint32_t i = i0 + (uint32_t)q;
+            active = q < kTopPer && i < F;
+            a = b = i;
+#pragma unroll
+            for (int k = 1; k < kTopPer; k++)
+                if (q == k) {
+                    node = eNode[k];
+                    box = eBox[k];
+                }
+        }
+        if (__ballot(progressed) == 0) __builtin_amdgcn_s_sleep(1);
+    }
 }
 
 // One wave per level-2 block (looping over them), over that block's stretch
 // of the frontier list up to its end mark: each entry climbs through the
-// device-scope arrival words.
+// device-scope arrival words.  Runs only when k_top_tree found more entries
+// than it holds.
 template <class TS>
 __global__ void __launch_bounds__(64)
     k_climb_upper(TS tris, uint32_t n, const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags,
                   const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ ctrl, uint32_t* root)
 {
+    if (ctrl[kCtrlTopFallback] == 0) return;
     const uint32_t count = ctrl[kCtrlSuperCount];
     for (uint32_t blk = blockIdx.x; (size_t)blk * kLeaves2 < count; blk += gridDim.x) {
         const uint32_t b0 = blk * kLeaves2;
@@ -657,24 +943,39 @@ static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uin
     int rc = ensure(ctx, &ctx->frontier, &ctx->frontierCap, (size_t)n * 3, "hipMalloc(climb frontier)");
     if (!rc) rc = ensure(ctx, &ctx->superList, &ctx->superCap, (size_t)n * 3, "hipMalloc(super-leaf list)");
     if (!rc) rc = ensure(ctx, &ctx->blockCounts, &ctx->blockCountsCap, 2 * (size_t)blocks1, "hipMalloc(block counts)");
+    const unsigned compactTiles = (blocks1 + kCompactBlocks - 1) / kCompactBlocks;
+    const size_t blocks2Max = ((size_t)n + kLeaves2 - 1) / kLeaves2;
+    if (!rc && (ctx->l1StatusCap < compactTiles + blocks2Max || !ctx->l1Status)) {
+        // a new table starts zeroed: tag 0 is never current
+        rc = ensure(ctx, &ctx->l1Status, &ctx->l1StatusCap, compactTiles + blocks2Max, "hipMalloc(build look-back)");
+        if (!rc) {
+            hipError_t e = hipMemset(ctx->l1Status, 0, ctx->l1StatusCap * sizeof(uint64_t));
+            if (e != hipSuccess) rc = hip_status(e, "hipMemset(build look-back)");
+        }
+    }
+    if (!rc) rc = ensure(ctx, &ctx->topList, &ctx->topListCap, (size_t)kTopMax * kTopWords, "hipMalloc(top list)");
     if (rc) return rc;
-    uint32_t* counts = ctx->blockCounts;
-    uint32_t* offsets = ctx->blockCounts + blocks1;
+    // Every kernel below is launched on every call: k_top_tree returns the
+    // compaction's ticket to zero and advances its tag for the next build.
+    uint32_t* counts1 = ctx->blockCounts;            // level-1 frontier counts
+    uint32_t* counts2 = ctx->blockCounts + blocks1;  // level-2 frontier counts
     if (fuseDeltas)
         k_local_nodes<TS, kLeaves1, kLeavesFusedDeltas><<<blocks1, kLeaves1, 0, stream>>>(
-            tris, (uint32_t)n, rec, nullptr, deltas, nullptr, nullptr, nodes, ctx->frontier, counts, nullptr, root);
+            tris, (uint32_t)n, rec, nullptr, deltas, nullptr, ctx->ctrl, nodes, ctx->frontier, counts1, nullptr,
+            root, nullptr, nullptr);
     else
         k_local_nodes<TS, kLeaves1, kLeavesGivenDeltas><<<blocks1, kLeaves1, 0, stream>>>(
-            tris, (uint32_t)n, rec, deltas, nullptr, nullptr, nullptr, nodes, ctx->frontier, counts, nullptr, root);
-    k_frontier_scan<<<1, 1024, 0, stream>>>(counts, blocks1, offsets, ctx->ctrl);
-    k_frontier_compact<<<blocks1, 64, 0, stream>>>(ctx->frontier, counts, offsets, ctx->superList, kLeaves1);
+            tris, (uint32_t)n, rec, deltas, nullptr, nullptr, ctx->ctrl, nodes, ctx->frontier, counts1, nullptr,
+            root, nullptr, nullptr);
+    k_frontier_compact<<<compactTiles, kCompactThreads, 0, stream>>>(counts1, blocks1, ctx->frontier,
+                                                                    ctx->superList, kLeaves1, ctx->l1Status, ctx->ctrl);
     // level 2 and the upper climb loop over however many level-2 blocks the
     // super-leaves fill (there are at most n super-leaves)
-    const size_t blocks2Max = ((size_t)n + kLeaves2 - 1) / kLeaves2;
     const unsigned grid2 = (unsigned)std::min<size_t>(blocks2Max, (size_t)ctx->cuCount * 2);
     k_local_nodes<TS, kLeaves2, kSuperLeaves><<<grid2, kLeaves2, 0, stream>>>(
-        tris, (uint32_t)n, nullptr, deltas, nullptr, ctx->superList, ctx->ctrl, nodes, ctx->frontier, nullptr,
-        ctx->flags, root);
+        tris, (uint32_t)n, nullptr, deltas, nullptr, ctx->superList, ctx->ctrl, nodes, ctx->frontier, counts2,
+        ctx->flags, root, ctx->l1Status + compactTiles, ctx->topList);
+    k_top_tree<<<1, kTopThreads, 0, stream>>>((uint32_t)n, nodes, ctx->topList, ctx->ctrl, root);
     const unsigned gridUp = (unsigned)std::min<size_t>(blocks2Max, (size_t)ctx->cuCount * 8);
     k_climb_upper<<<gridUp, 64, 0, stream>>>(tris, (uint32_t)n, deltas, nodes, ctx->flags, ctx->frontier, ctx->ctrl,
                                              root);

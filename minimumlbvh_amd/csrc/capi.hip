@@ -95,7 +95,7 @@ extern "C" int mlbvh_ctx_destroy(mlbvh_ctx* ctx)
     hipSetDevice(ctx->device);
     void* bufs[] = {ctx->status, ctx->hist,     ctx->ctrl,     ctx->recA,    ctx->recB,
                     ctx->deltas, ctx->flags,    ctx->sceneBox, ctx->meshBox, ctx->partials, ctx->frontier,
-                    ctx->idxA,   ctx->idxB,     ctx->superList, ctx->blockCounts};
+                    ctx->idxA,   ctx->idxB,     ctx->superList, ctx->blockCounts, ctx->l1Status, ctx->topList};
     int rc = MLBVH_OK;
     for (void* p : bufs) {
         if (!p) continue;

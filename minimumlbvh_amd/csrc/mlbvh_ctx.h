@@ -37,8 +37,12 @@ struct mlbvh_ctx {
     size_t frontierCap = 0;
     uint32_t* superList = nullptr;  // the level-1 frontier in leaf order: the super-leaves, [n][3]
     size_t superCap = 0;
-    uint32_t* blockCounts = nullptr;  // level-1 frontier counts and their offsets, [2][blocks]
+    uint32_t* blockCounts = nullptr;  // level-2 frontier counts, [level-2 blocks]
     size_t blockCountsCap = 0;
+    uint64_t* l1Status = nullptr;  // look-back words: frontier compaction workgroups, then level-2 blocks
+    size_t l1StatusCap = 0;
+    uint32_t* topList = nullptr;   // level-2 frontier records for the top tree, [kTopMax][kTopWords]
+    size_t topListCap = 0;
     uint32_t* sceneBox = nullptr;  // copies of 6 order-mapped uint32 (lo xyz, hi xyz), 256 words
 
     // --- batched build / trace ------------------------------------------------
@@ -65,7 +69,18 @@ constexpr int kCtrlSortFallback = 50;  // set when a prefix segment is too long 
 constexpr int kCtrlSuperCount = 51;    // super-leaves of the build's second block-local level
 constexpr int kCtrlAoTiles = 52;       // next pixel tile of the persistent AO kernel
 constexpr int kCtrlPtTiles = 53;       // next pixel tile of the persistent path-tracing kernel
+constexpr int kCtrlTopFallback = 54;   // set when the nodes above level 2 climb through device memory
+constexpr int kCtrlCompactTicket = 55;  // next workgroup of the level-1 frontier compaction (reset after each build)
+constexpr int kCtrlCompactTag = 56;     // the build's look-back words carry this + 1 (advanced after each build)
+constexpr int kCtrlL2Ticket = 57;       // next block of the build's second block-local level (reset after each build)
+constexpr int kCtrlTopCount = 58;       // entries of the top-tree list (level 2's frontier)
 constexpr int kCtrlWords = 64;
+// The build's top-tree list (mlbvh_ctx::topList): at most kTopMax records of
+// kTopWords words, one per level-2 frontier entry, in leaf order: node, last
+// leaf, delta of the split after it (255: none), 0, then the entry's subtree
+// box (lo xyz, hi xyz) and two zeros.
+constexpr uint32_t kTopMax = 4096;
+constexpr int kTopWords = 12;
 constexpr int kNumSortCfgs = 5;  // tile shapes of the radix sort (onesweep.hip)
 constexpr uint32_t kEpochLimit = (1u << 30) - 1;  // look-back words keep 30 tag bits
 

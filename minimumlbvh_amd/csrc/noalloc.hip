@@ -4,20 +4,23 @@
 // loops, harnesses that forbid hipMalloc).
 //
 // The array, `scratch`, holds a fixed layout sized for kNoAllocMaxTris
-// triangles (any sort tile shape, any CU count): the sort's control words and
-// look-back table first, at fixed offsets, because the pass tags they carry
-// stay valid from one call to the next; then the histograms, the records, the
-// deltas, the climb's arrival words, its frontier and the level-1 block
-// counts (the super-leaf list reuses the sort's pong buffers).  Every kernel of the
-// build clears or overwrites what it reads before reading it (histograms,
-// tickets, arrival words, frontier count, scene box), so a call does not
-// depend on the contents a previous call left.  All calls on one GPU share
+// triangles (any sort tile shape, any CU count): the control words and the
+// two look-back tables (the sort's, and that of the build's frontier
+// compaction) first, at fixed offsets, because the tags they carry stay
+// valid from one call to the next; then the histograms, the records, the
+// deltas, the climb's arrival words, its frontier, the frontier counts and
+// the top-tree list (the super-leaf list reuses the sort's pong buffers).  Every kernel
+// of the build clears or overwrites what it reads before reading it
+// (histograms, sort tickets, arrival words, frontier counts, scene box), and
+// the build's last kernel returns the compaction's ticket to zero and
+// advances its tag, so a call does not depend on other contents a previous
+// call left.  All calls on one GPU share
 // that GPU's array: calls that overlap in time on one GPU race and give
 // undefined results; order them on one stream.
 //
 // Set-up (mlbvh_ctx_create, once per device, with that device current)
 // resolves the array's device address with hipGetSymbolAddress and zeroes
-// its control words and look-back table; host code uses only that address.
+// its control words and look-back tables; host code uses only that address.
 #include <mutex>
 
 #include "mlbvh_ctx.h"
@@ -44,7 +47,9 @@ constexpr size_t kOffCtrl = 0;
 constexpr size_t kOffSceneBox = kOffCtrl + up(kCtrlWords * 4);
 constexpr size_t kOffStatus = kOffSceneBox + up(256 * 4);
 constexpr size_t kStatusWords = kMaxTiles * 256;
-constexpr size_t kOffHist = kOffStatus + up(kStatusWords * 8);
+constexpr size_t kL1Blocks = (size_t)kNoAllocMaxTris / 256 + 1;  // level-1 blocks of at least 256 leaves
+constexpr size_t kOffL1Status = kOffStatus + up(kStatusWords * 8);
+constexpr size_t kOffHist = kOffL1Status + up(kL1Blocks * 8);
 constexpr size_t kOffPartials = kOffHist + up(8 * 256 * 4);
 constexpr size_t kOffRecA = kOffPartials + up(kPartialRows * 256 * 4);
 constexpr size_t kOffRecB = kOffRecA + up((size_t)kNoAllocMaxTris * 8);
@@ -54,14 +59,15 @@ constexpr size_t kOffDeltas = kOffIdxB + up((size_t)kNoAllocMaxTris * 4);
 constexpr size_t kOffFlags = kOffDeltas + up((size_t)kNoAllocMaxTris);
 constexpr size_t kOffFrontier = kOffFlags + up((size_t)kNoAllocMaxTris * 4);
 constexpr size_t kOffCounts = kOffFrontier + up((size_t)kNoAllocMaxTris * 3 * 4);
-constexpr size_t kCountWords = 2 * ((size_t)kNoAllocMaxTris / 256 + 1);  // level-1 blocks of at least 256 leaves
-constexpr size_t kScratchBytes = kOffCounts + up(kCountWords * 4);
+constexpr size_t kCountWords = 2 * kL1Blocks;  // level-1 and level-2 frontier counts
+constexpr size_t kOffTopList = kOffCounts + up(kCountWords * 4);
+constexpr size_t kScratchBytes = kOffTopList + up((size_t)kTopMax * kTopWords * 4);
 // The super-leaf list ([n][3] words) shares the sort's pong buffers
 // (records B, indices A and B), which are dead once the sorted records are
 // in records A, before the climb starts.
 constexpr size_t kOffSuper = kOffRecB;
 static_assert(kOffDeltas - kOffSuper >= (size_t)kNoAllocMaxTris * 3 * 4, "the super-leaf list fits the pong buffers");
-// the zeroed prefix: control words, scene box, look-back table
+// the zeroed prefix: control words, scene box, the look-back tables
 constexpr size_t kZeroBytes = kOffHist;
 static_assert(kScratchBytes < (48ull << 20), "scratch stays modest: under 48 MiB per GPU");
 
@@ -80,8 +86,8 @@ __global__ void k_zero_words(uint64_t* p, size_t words)
 }
 
 // Once per device, with `device` current: resolve the array's address and
-// zero its control words and look-back table (the pass tags start at 0,
-// which no pass uses).
+// zero its control words and look-back tables (the tags start at 0, which
+// no look-back word is written with).
 int noalloc_setup(int device)
 {
     if (device < 0 || device >= kMaxDevices) {
@@ -153,6 +159,10 @@ extern "C" int mlbvh_build_ex_noalloc(mlbvh_ctx* ctx, const void* d_triangles, i
     view.superCap = (size_t)kNoAllocMaxTris * 3;
     view.blockCounts = (uint32_t*)(base + kOffCounts);
     view.blockCountsCap = kCountWords;
+    view.l1Status = (uint64_t*)(base + kOffL1Status);
+    view.l1StatusCap = kL1Blocks;
+    view.topList = (uint32_t*)(base + kOffTopList);
+    view.topListCap = (size_t)kTopMax * kTopWords;
     view.meshBox = nullptr;
     view.meshBoxCap = 0;
     return build_one(&view, d_triangles, n, buildOption, d_nodes, d_root, d_sorted_records, d_deltas,
