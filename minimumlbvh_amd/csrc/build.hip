@@ -24,50 +24,51 @@
 namespace mlbvh {
 
 constexpr int kBuildBlock = 256;
-// The scene box is reduced into kBoxCopies copies of its 6 order-mapped
-// words (block b uses copy b % kBoxCopies), so that the blocks' atomics
-// spread over many L2 lines; the Morton kernels fold the copies.
+// The scene box is reduced into kBoxCopies copies of its 6 coordinates
+// (block b uses copy b % kBoxCopies), so that the blocks' atomics spread over
+// many L2 lines; the Morton kernels fold the copies.  Each slot is a 64-bit
+// word, tag << 32 | key, merged with atomic max: key = f2ord(hi) for the
+// upper bounds and ~f2ord(lo) for the lower ones, tag = the build's tag
+// (kCtrlBuildTag + 1).  A newer tag wins over whatever an earlier build left,
+// so the slots are never cleared (zeroed slots carry tag 0, which no build
+// uses; k_top_tree zeroes them when the tag would wrap).
 #ifndef MLBVH_BOX_COPIES
 #define MLBVH_BOX_COPIES 32
 #endif
 constexpr int kBoxCopies = MLBVH_BOX_COPIES;
 constexpr int kBoxWords = 6 * kBoxCopies;
-static_assert(kBoxWords <= 256, "k_build_init clears the box copies with one block");
+static_assert(kBoxWords <= kSceneBoxWords, "the box copies fit the context's scene-box slots");
 
 // The scene box from the kBoxCopies partial boxes (all threads of the block
 // call it; the result is the same in every thread).
-__device__ inline void fold_scene_box(const uint32_t* sceneBox, f3* lo, f3* hi)
+__device__ inline void fold_scene_box(const uint64_t* sceneBox, f3* lo, f3* hi)
 {
     __shared__ uint32_t s_box[6];
     if (threadIdx.x < 6) {
-        uint32_t r = sceneBox[threadIdx.x];
+        uint64_t r = sceneBox[threadIdx.x];
         for (int c = 1; c < kBoxCopies; c++) {
-            uint32_t v = sceneBox[6 * c + threadIdx.x];
-            r = threadIdx.x < 3 ? (v < r ? v : r) : (v > r ? v : r);
+            const uint64_t v = sceneBox[6 * c + threadIdx.x];
+            r = v > r ? v : r;
         }
-        s_box[threadIdx.x] = r;
+        s_box[threadIdx.x] = threadIdx.x < 3 ? ~(uint32_t)r : (uint32_t)r;
     }
     __syncthreads();
     *lo = mk(ord2f(s_box[0]), ord2f(s_box[1]), ord2f(s_box[2]));
     *hi = mk(ord2f(s_box[3]), ord2f(s_box[4]), ord2f(s_box[5]));
 }
 
-// Scene-box slots plus the sort state of the 4-pass record sort.
-__global__ void k_build_init(uint32_t* sceneBox, uint32_t* ctrl, uint32_t* hist, uint64_t* status,
-                             size_t statusWords)
-{
-    int t = threadIdx.x;
-    if (t < kBoxWords) sceneBox[t] = t % 6 < 3 ? 0xFFFFFFFFu : 0u;  // atomicMin / atomicMax slots (order-mapped)
-    sort_state_init(ctrl, 4, hist, 4, status, statusWords);
-}
-
 // Scene bounds over all vertices: per-thread running min/max, wave and block
-// reduction, then 6 order-mapped integer atomics per block.  TS, here and in
-// the kernels below, is the triangle source: `const Tri*` (a soup) or
+// reduction, then 6 atomics per block on the tagged slots above.  Block 0
+// also sets up the state of the 4-pass record sort.  TS, here and in the
+// kernels below, is the triangle source: `const Tri*` (a soup) or
 // IndexedTris.
 template <class TS>
-__global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(TS tris, int n, uint32_t* sceneBox)
+__global__ void __launch_bounds__(kBuildBlock)
+    k_scene_aabb(TS tris, int n, unsigned long long* sceneBox, uint32_t* ctrl, uint32_t* hist, uint64_t* status,
+                 size_t statusWords)
 {
+    if (blockIdx.x == 0) sort_state_init(ctrl, 4, hist, 4, status, statusWords);
+    const unsigned long long tag = (unsigned long long)(ctrl[kCtrlBuildTag] + 1u) << 32;
     f3 lo = mk(kFltMax, kFltMax, kFltMax), hi = mk(-kFltMax, -kFltMax, -kFltMax);
     for (int i = blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += gridDim.x * kBuildBlock) {
         Tri tri = tris[i];
@@ -94,9 +95,8 @@ __global__ void __launch_bounds__(kBuildBlock) k_scene_aabb(TS tris, int n, uint
         int c = threadIdx.x;
         float r = s_v[0][c];
         for (int w = 1; w < kBuildBlock / 64; w++) r = c < 3 ? fminf(r, s_v[w][c]) : fmaxf(r, s_v[w][c]);
-        uint32_t* box = sceneBox + 6 * (blockIdx.x % kBoxCopies);
-        if (c < 3) atomicMin(&box[c], f2ord(r));
-        else atomicMax(&box[c], f2ord(r));
+        unsigned long long* box = sceneBox + 6 * (blockIdx.x % kBoxCopies);
+        atomicMax(&box[c], tag | (c < 3 ? ~f2ord(r) : f2ord(r)));
     }
 }
 
@@ -117,7 +117,7 @@ constexpr int kMortonBlock = 1024;  // one block per sort tile still fills the C
 constexpr int kMortonUnroll = MLBVH_MORTON_UNROLL;
 template <class TS>
 __global__ void __launch_bounds__(kMortonBlock)
-    k_morton(TS tris, int n, const uint32_t* __restrict__ sceneBox, uint32_t opt,
+    k_morton(TS tris, int n, const uint64_t* __restrict__ sceneBox, uint32_t opt,
              uint64_t* __restrict__ rec, uint32_t* __restrict__ partials, uint32_t tileKeys, uint32_t nTiles,
              uint32_t* __restrict__ tileCounts)
 {
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(kMortonBlock)
 // triangle index, for a key-value sort over bits [0, 63).
 template <class TS>
 __global__ void __launch_bounds__(kBuildBlock)
-    k_morton64(TS tris, int n, const uint32_t* __restrict__ sceneBox, uint64_t* __restrict__ keys,
+    k_morton64(TS tris, int n, const uint64_t* __restrict__ sceneBox, uint64_t* __restrict__ keys,
                uint32_t* __restrict__ idx)
 {
     f3 lo, hi;
@@ -764,7 +764,7 @@ constexpr uint32_t kTopReady = 0x80000000u;           // the first arriver's dep
 constexpr uint32_t kTopNoSplit = 0x100u;              // a missing split's delta: above every delta
 __global__ void __launch_bounds__(kTopThreads)
     k_top_tree(uint32_t n, Node* __restrict__ nodes, const uint32_t* __restrict__ topList, uint32_t* ctrl,
-               uint32_t* root)
+               uint32_t* root, uint64_t* sceneBox)
 {
     __shared__ Box s_box[kTopMax];       // per split: the first arriver's subtree box
     __shared__ uint32_t s_ctl[kTopMax];  // per split: arrivals (bits 0..7) | far end << 8 | kTopReady
@@ -786,12 +786,18 @@ __global__ void __launch_bounds__(kTopThreads)
     // no super-leaves: level 1 built the root, level 2 ran no block and wrote no count
     const bool none = ctrl[kCtrlSuperCount] == 0;
     const uint32_t F = none ? 0u : ctrl[kCtrlTopCount];
-    if (t == 0) {  // the next build starts from ticket 0 with a fresh tag
+    if (t == 0) {  // the next build starts from ticket 0 with fresh tags
         if (none) ctrl[kCtrlTopFallback] = 0u;
         ctrl[kCtrlCompactTicket] = 0u;
         ctrl[kCtrlL2Ticket] = 0u;
         const uint32_t tag = ctrl[kCtrlCompactTag] + 1u;
         ctrl[kCtrlCompactTag] = tag == 0xFFFFFFFFu ? 0u : tag;  // the next build's words carry tag + 1 != 0
+    }
+    {  // the scene box slots' tag likewise; on wrap the slots restart from zero
+        const uint32_t tag = ctrl[kCtrlBuildTag] + 1u;
+        __syncthreads();
+        if (tag == 0xFFFFFFFFu && t < kSceneBoxWords) sceneBox[t] = 0ull;
+        if (t == 0) ctrl[kCtrlBuildTag] = tag == 0xFFFFFFFFu ? 0u : tag;
     }
     if (F == 0 || F > kTopMax) return;  // the root was built below, or k_climb_upper climbs
     uint32_t eNode[kTopPer];
@@ -975,7 +981,7 @@ static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uin
     k_local_nodes<TS, kLeaves2, kSuperLeaves><<<grid2, kLeaves2, 0, stream>>>(
         tris, (uint32_t)n, nullptr, deltas, nullptr, ctx->superList, ctx->ctrl, nodes, ctx->frontier, counts2,
         ctx->flags, root, ctx->l1Status + compactTiles, ctx->topList);
-    k_top_tree<<<1, kTopThreads, 0, stream>>>((uint32_t)n, nodes, ctx->topList, ctx->ctrl, root);
+    k_top_tree<<<1, kTopThreads, 0, stream>>>((uint32_t)n, nodes, ctx->topList, ctx->ctrl, root, ctx->sceneBox);
     const unsigned gridUp = (unsigned)std::min<size_t>(blocks2Max, (size_t)ctx->cuCount * 8);
     k_climb_upper<<<gridUp, 64, 0, stream>>>(tris, (uint32_t)n, deltas, nodes, ctx->flags, ctx->frontier, ctx->ctrl,
                                              root);
@@ -1013,10 +1019,10 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
     uint32_t* root = (uint32_t*)d_root;
 
     if ((rc = ensure_status(ctx, sort_tiles(ctx, (uint32_t)n)))) return rc;
-    k_build_init<<<1, 256, 0, stream>>>(ctx->sceneBox, ctx->ctrl, ctx->hist, ctx->status, ctx->statusWords);
     unsigned g = grid1(nn);
     unsigned gAabb = g < (unsigned)ctx->cuCount * 8 ? g : (unsigned)ctx->cuCount * 8;
-    k_scene_aabb<<<gAabb, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox);
+    k_scene_aabb<<<gAabb, kBuildBlock, 0, stream>>>(tris, n, (unsigned long long*)ctx->sceneBox, ctx->ctrl, ctx->hist,
+                                                    ctx->status, ctx->statusWords);
     unsigned gMorton = g < (unsigned)ctx->cuCount * 4 ? g : (unsigned)ctx->cuCount * 4;
     if (wide) {
         if ((rc = ensure(ctx, &ctx->idxA, &ctx->idxCap, nn, "hipMalloc(indices)"))) return rc;
@@ -1034,8 +1040,7 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         uint32_t* tileCounts = ctx->partials + (size_t)gm * 4 * 256;
         k_morton<<<gm, kMortonBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials,
                                                  (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
-        launch_hist_reduce(ctx, (int)gm, 4, stream);
-        rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts);
+        rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts, (int)gm);
         if (rc) return rc;
     }
     // 32-bit keys: the local climb computes the deltas from the records

@@ -80,7 +80,11 @@ extern "C" int mlbvh_ctx_create(int device, mlbvh_ctx** out)
         if (e != hipSuccess) rc = hip_status(e, "hipMemset(ctrl)");
     }
     cap = 0;
-    if (!rc) rc = ensure(ctx, &ctx->sceneBox, &cap, (size_t)256, "hipMalloc(sceneBox)");
+    if (!rc) rc = ensure(ctx, &ctx->sceneBox, &cap, (size_t)kSceneBoxWords, "hipMalloc(sceneBox)");
+    if (!rc) {  // zeroed slots carry tag 0, which no build uses
+        hipError_t e = hipMemset(ctx->sceneBox, 0, kSceneBoxWords * sizeof(uint64_t));
+        if (e != hipSuccess) rc = hip_status(e, "hipMemset(sceneBox)");
+    }
     if (rc) {
         mlbvh_ctx_destroy(ctx);
         return rc;

@@ -43,7 +43,7 @@ struct mlbvh_ctx {
     size_t l1StatusCap = 0;
     uint32_t* topList = nullptr;   // level-2 frontier records for the top tree, [kTopMax][kTopWords]
     size_t topListCap = 0;
-    uint32_t* sceneBox = nullptr;  // copies of 6 order-mapped uint32 (lo xyz, hi xyz), 256 words
+    uint64_t* sceneBox = nullptr;  // tagged copies of the scene box's 6 coordinates, kSceneBoxWords words
 
     // --- batched build / trace ------------------------------------------------
     uint32_t* meshBox = nullptr;  // [meshes][6] order-mapped bounds
@@ -74,11 +74,13 @@ constexpr int kCtrlCompactTicket = 55;  // next workgroup of the level-1 frontie
 constexpr int kCtrlCompactTag = 56;     // the build's look-back words carry this + 1 (advanced after each build)
 constexpr int kCtrlL2Ticket = 57;       // next block of the build's second block-local level (reset after each build)
 constexpr int kCtrlTopCount = 58;       // entries of the top-tree list (level 2's frontier)
+constexpr int kCtrlBuildTag = 59;       // the scene box slots carry this + 1 (advanced after each build)
 constexpr int kCtrlWords = 64;
 // The build's top-tree list (mlbvh_ctx::topList): at most kTopMax records of
 // kTopWords words, one per level-2 frontier entry, in leaf order: node, last
 // leaf, delta of the split after it (255: none), 0, then the entry's subtree
 // box (lo xyz, hi xyz) and two zeros.
+constexpr int kSceneBoxWords = 256;
 constexpr uint32_t kTopMax = 4096;
 constexpr int kTopWords = 12;
 constexpr int kNumSortCfgs = 5;  // tile shapes of the radix sort (onesweep.hip)
@@ -159,13 +161,15 @@ int set_device(mlbvh_ctx* ctx);
 
 // Radix sort passes shared by mlbvh_sort_* and the builders.  When histReady
 // is true the caller has already run sort_state_init and filled
-// ctx->hist[0..passes) (the builders fuse both into their own kernels) and
-// has called ensure_status(ctx, sort_tiles(n)).
+// ctx->hist[0..passes) or the partials below (the builders fuse both into
+// their own kernels) and has called ensure_status(ctx, sort_tiles(n)).
 // With tileCounts (digit-major per-tile counts of the first digit, over
 // sort_tiles(n) tiles of sort_tile_keys keys) the first pass reads prefixes
-// instead of looking back.
+// instead of looking back.  With histBlocks > 0 the histograms are not
+// reduced yet: ctx->partials holds histBlocks rows of per-pass partials
+// ([block][pass][256]), summed by the sort's first kernel.
 int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, int startBit, int endBit,
-                     bool histReady, hipStream_t stream, const uint32_t* tileCounts = nullptr);
+                     bool histReady, hipStream_t stream, const uint32_t* tileCounts = nullptr, int histBlocks = 0);
 // Key-value sort (u64 keys, u32 values) with its own histograms, as
 // mlbvh_sort_u64.
 int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,

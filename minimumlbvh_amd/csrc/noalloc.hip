@@ -11,10 +11,10 @@
 // deltas, the climb's arrival words, its frontier, the frontier counts and
 // the top-tree list (the super-leaf list reuses the sort's pong buffers).  Every kernel
 // of the build clears or overwrites what it reads before reading it
-// (histograms, sort tickets, arrival words, frontier counts, scene box), and
-// the build's last kernel returns the compaction's ticket to zero and
-// advances its tag, so a call does not depend on other contents a previous
-// call left.  All calls on one GPU share
+// (histograms, sort tickets, arrival words, frontier counts), and the
+// build's last kernel returns the build's tickets to zero and advances the
+// tags of its look-back words and scene-box slots, so a call does not depend
+// on other contents a previous call left.  All calls on one GPU share
 // that GPU's array: calls that overlap in time on one GPU race and give
 // undefined results; order them on one stream.
 //
@@ -45,7 +45,7 @@ constexpr size_t kPartialRows = kMaxHistBlocks * 8 + kMaxTiles;
 // Byte offsets into `scratch`.
 constexpr size_t kOffCtrl = 0;
 constexpr size_t kOffSceneBox = kOffCtrl + up(kCtrlWords * 4);
-constexpr size_t kOffStatus = kOffSceneBox + up(256 * 4);
+constexpr size_t kOffStatus = kOffSceneBox + up(kSceneBoxWords * 8);
 constexpr size_t kStatusWords = kMaxTiles * 256;
 constexpr size_t kL1Blocks = (size_t)kNoAllocMaxTris / 256 + 1;  // level-1 blocks of at least 256 leaves
 constexpr size_t kOffL1Status = kOffStatus + up(kStatusWords * 8);
@@ -135,7 +135,7 @@ extern "C" int mlbvh_build_ex_noalloc(mlbvh_ctx* ctx, const void* d_triangles, i
     mlbvh_ctx view = *ctx;
     view.fixedScratch = true;
     view.ctrl = (uint32_t*)(base + kOffCtrl);
-    view.sceneBox = (uint32_t*)(base + kOffSceneBox);
+    view.sceneBox = (uint64_t*)(base + kOffSceneBox);
     view.status = (uint64_t*)(base + kOffStatus);
     view.statusWords = kStatusWords;
     view.hist = (uint32_t*)(base + kOffHist);
