@@ -340,6 +340,59 @@ __device__ inline uint32_t lb_exclusive(const uint64_t* status, uint32_t idx, ui
     return excl;
 }
 
+// lb_exclusive by all kWaves waves of a block at once (kWaves * 64
+// workgroups per round); every thread calls it, and it returns the same value
+// in every thread.  s_pub / s_pre / s_sum: kWaves words of LDS each.
+template <int kWaves>
+__device__ inline uint32_t lb_exclusive_block(const uint64_t* status, uint32_t idx, uint32_t tag, uint64_t* s_pub,
+                                              uint64_t* s_pre, uint32_t* s_sum)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t excl = 0;
+    int64_t base = (int64_t)idx - 1;
+    while (base >= 0) {  // the same in every thread
+        const int64_t j = base - (int64_t)threadIdx.x;
+        const uint64_t w =
+            j >= 0 ? __hip_atomic_load(status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : lb_word(tag, true, 0u);
+        const bool pub = (uint32_t)(w >> 32) == tag;
+        const uint64_t pubMask = __ballot(pub), preMask = __ballot(pub && (w & kLbPrefix) != 0);
+        if (lane == 0) {
+            s_pub[wave] = pubMask;
+            s_pre[wave] = preMask;
+        }
+        lds_barrier();
+        // the nearest prefix over the block's window, thread t reading base - t
+        int first = kWaves * 64;
+        bool ready = true;
+#pragma unroll
+        for (int k = 0; k < kWaves; k++) {
+            const uint64_t pre = s_pre[k], pubk = s_pub[k];
+            if (first == kWaves * 64) {
+                const int f = pre ? __ffsll((unsigned long long)pre) - 1 : 64;
+                const uint64_t need = f >= 63 ? ~0ull : ((2ull << f) - 1ull);
+                if ((pubk & need) != need) ready = false;
+                if (pre) first = 64 * k + f;
+            }
+        }
+        uint32_t v = ready && (int)threadIdx.x <= first ? (uint32_t)(w & 0x7FFFFFFFu) : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) s_sum[wave] = v;
+        lds_barrier();
+        if (ready) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (int k = 0; k < kWaves; k++) sum += s_sum[k];
+            excl += sum;
+            base = first < kWaves * 64 ? -1 : base - kWaves * 64;
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+        }
+        lds_barrier();  // s_pub / s_pre / s_sum are rewritten by the next round
+    }
+    return excl;
+}
+
 template <class TS, int kLeaves, int kMode>
 __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kLeaves <= 256 ? 8 : 1)))
     k_local_nodes(TS tris, uint32_t n, const uint64_t* __restrict__ rec, const uint8_t* __restrict__ deltas,
@@ -367,7 +420,9 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
     __shared__ uint32_t s_aux[kLeaves + 2];
     __shared__ uint32_t s_wcount[kWaves];
     __shared__ uint32_t s_rank[kMode == kSuperLeaves ? kLeaves : 1];  // level 2: rank of the entry at element j
-    __shared__ uint32_t s_blk, s_excl;  // level 2: this block (by ticket), its entries' offset in the top list
+    __shared__ uint32_t s_blk;  // level 2: this block (by ticket)
+    __shared__ uint64_t s_lbPub[kWaves], s_lbPre[kWaves];  // level 2: the look-back's window, per wave
+    __shared__ uint32_t s_lbSum[kWaves];
     // The frontier entry starting at element j: its node and last leaf (its
     // first leaf is element j's), in the sparse table's LDS once that is dead.
     uint2* s_ent = reinterpret_cast<uint2*>(&s_st[0][0]);
@@ -607,19 +662,15 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
         // Level 2: this block's entries go to the top list, after those of
         // the blocks before it (found by the look-back); the last block
         // writes the list's length, and whether it overflows the top tree.
-        if (wave == 0) {
-            const uint32_t excl = lb_exclusive(status2, blk, tag, lane);
-            if (lane == 0) {
-                s_excl = excl;
-                __hip_atomic_store(status2 + blk, lb_word(tag, true, excl + total), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                if (blk == nBlocks - 1) {
-                    ctrl[kCtrlTopCount] = excl + total;
-                    ctrl[kCtrlTopFallback] = excl + total > kTopMax ? 1u : 0u;
-                }
+        const uint32_t excl = lb_exclusive_block<kWaves>(status2, blk, tag, s_lbPub, s_lbPre, s_lbSum);
+        if (t == 0) {
+            __hip_atomic_store(status2 + blk, lb_word(tag, true, excl + total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            if (blk == nBlocks - 1) {
+                ctrl[kCtrlTopCount] = excl + total;
+                ctrl[kCtrlTopFallback] = excl + total > kTopMax ? 1u : 0u;
             }
         }
-        lds_barrier();
         auto put_record = [&](uint32_t k, uint32_t node, uint32_t hi, uint32_t d, const Box& b) {
             if (k >= kTopMax) return;
             uint4* dst = reinterpret_cast<uint4*>(topList + (size_t)k * kTopWords);
@@ -632,9 +683,9 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
             Box eb;
             eb.lo = mk(ord2f(kb.lo[0]), ord2f(kb.lo[1]), ord2f(kb.lo[2]));
             eb.hi = mk(ord2f(kb.hi[0]), ord2f(kb.hi[1]), ord2f(kb.hi[2]));
-            put_record(s_excl + rank, elemNode, elemHi, dNext, eb);
+            put_record(excl + rank, elemNode, elemHi, dNext, eb);
         }
-        if (emitNode) put_record(s_excl + s_rank[jL], gNode, gHi, dAfter, entBox);
+        if (emitNode) put_record(excl + s_rank[jL], gNode, gHi, dAfter, entBox);
         lds_barrier();  // the next block of the loop reuses the LDS
         if (t == 0) s_blk = atomicAdd(&ctrl[kCtrlL2Ticket], 1u);
         lds_barrier();
@@ -1040,7 +1091,8 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         uint32_t* tileCounts = ctx->partials + (size_t)gm * 4 * 256;
         k_morton<<<gm, kMortonBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials,
                                                  (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
-        rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts, (int)gm);
+        launch_hist_reduce(ctx, (int)gm, 4, stream);
+        rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts);
         if (rc) return rc;
     }
     // 32-bit keys: the local climb computes the deltas from the records

@@ -173,60 +173,23 @@ __global__ void __launch_bounds__(256) k_hist_reduce(const uint32_t* __restrict_
 // counts: status[d * nTiles + tile] = (that pass's tag, prefix flag, count
 // of digit d in tiles 0..tile).  Digit-major, so that each lane writes whole
 // lines; the other passes' look-back words ([tile][digit]) carry other tags,
-// so sharing the table is safe.  One block per digit, which also sums that
-// digit's histogram partials of nPass passes (partials[block][pass][digit],
-// nBlocks rows) into hist[pass][d] (written, not added).
+// so sharing the table is safe.  One block per digit.
 __global__ void __launch_bounds__(256) k_tile_scan(const uint32_t* __restrict__ tileCounts, uint32_t nTiles,
                                                    uint64_t* __restrict__ status, const uint32_t* __restrict__ ctrl,
-                                                   int pass, const uint32_t* __restrict__ partials, int nBlocks,
-                                                   int nPass, uint32_t* __restrict__ hist)
+                                                   int pass)
 {
     __shared__ uint32_t s_tmp[4];
-    __shared__ uint32_t s_sum[4][8];
     const uint32_t d = blockIdx.x;
     const uint32_t epoch = ctrl[kCtrlEpochs + pass];
-    // This thread's stretch of the column (up to kCol counts held in
-    // registers, the rest re-read) and its histogram rows: all loads are
-    // issued before any is used.
-    constexpr uint32_t kCol = 8;
     const uint32_t per = (nTiles + 255) / 256;
     const uint32_t a = threadIdx.x * per < nTiles ? threadIdx.x * per : nTiles;
-    const uint32_t e = a + per < nTiles ? a + per : nTiles;
+    const uint32_t b = a + per < nTiles ? a + per : nTiles;
     const uint32_t* col = tileCounts + (size_t)d * nTiles;
-    uint32_t c[kCol];
-#pragma unroll
-    for (uint32_t k = 0; k < kCol; k++) c[k] = a + k < e ? col[a + k] : 0u;
-    uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nBlocks; b += 256) {
-        const uint32_t* row = partials + (size_t)b * nPass * 256 + d;
-#pragma unroll
-        for (int p = 0; p < 8; p++)
-            if (p < nPass) v[p] += row[p * 256];
-    }
     uint32_t sum = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kCol; k++) sum += c[k];
-    for (uint32_t j = a + kCol; j < e; j++) sum += col[j];
-    if (nPass > 0) {
-#pragma unroll
-        for (int p = 0; p < 8; p++) {
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) v[p] += __shfl_xor(v[p], o, 64);
-            if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6][p] = v[p];
-        }
-    }
-    uint32_t run = scan256(sum, s_tmp);  // its barriers also publish s_sum
-    if ((int)threadIdx.x < nPass)
-        hist[threadIdx.x * 256 + d] =
-            s_sum[0][threadIdx.x] + s_sum[1][threadIdx.x] + s_sum[2][threadIdx.x] + s_sum[3][threadIdx.x];
+    for (uint32_t j = a; j < b; j++) sum += col[j];
+    uint32_t run = scan256(sum, s_tmp);
     uint64_t* out = status + (size_t)d * nTiles;
-#pragma unroll
-    for (uint32_t k = 0; k < kCol; k++)
-        if (a + k < e) {
-            run += c[k];
-            out[a + k] = lb_pack(epoch, kFlagP, run);
-        }
-    for (uint32_t j = a + kCol; j < e; j++) {
+    for (uint32_t j = a; j < b; j++) {
         run += col[j];
         out[j] = lb_pack(epoch, kFlagP, run);
     }
@@ -665,8 +628,8 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
 #if MLBVH_PRESCAN
     k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - 3), 3, ctx->partials, nullptr,
                                              (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
-    k_tile_scan<<<256, 256, 0, stream>>>(tileCounts, tiles, ctx->status, ctx->ctrl, 0, ctx->partials, hg, 3,
-                                         ctx->hist + 256 * (live - 3));
+    launch_hist_reduce(ctx, hg, 3, stream, live - 3);
+    k_tile_scan<<<256, 256, 0, stream>>>(tileCounts, tiles, ctx->status, ctx->ctrl, 0);
 #else
     (void)tileCounts;
     k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - 3), 3, ctx->partials);
@@ -733,8 +696,7 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
 
 template <class K>
 static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, uint32_t n, int startBit,
-                     int endBit, bool histReady, hipStream_t stream, const uint32_t* readyTileCounts = nullptr,
-                     int readyHistBlocks = 0)
+                     int endBit, bool histReady, hipStream_t stream, const uint32_t* readyTileCounts = nullptr)
 {
     const int keyBits = 8 * (int)sizeof(K);
     if (startBit < 0 || endBit <= startBit) {
@@ -763,11 +725,8 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
                                                                              tiles, stream);
 
     // per-tile counts of the first digit (from the caller's histogram kernel
-    // or from k_hist here) let the first pass skip its look-back; the tile
-    // scan also reduces the histogram partials (histBlocks rows of `live`
-    // passes) unless they are already reduced
+    // or from k_hist here) let the first pass skip its look-back
     const uint32_t* tileCounts = MLBVH_PRESCAN ? readyTileCounts : nullptr;
-    int histBlocks = readyHistBlocks;
     if (!histReady) {
         int hg = hist_grid(ctx, n);
         if ((rc = ensure_partials(ctx, hg * live + (MLBVH_PRESCAN ? (int)tiles : 0), 1))) return rc;
@@ -777,17 +736,12 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
         k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials, nullptr,
                                                  (uint32_t)sort_tile_keys(ctx), tiles, counts);
         tileCounts = counts;
-        histBlocks = hg;
 #else
         k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials);
-        launch_hist_reduce(ctx, hg, live, stream);
 #endif
+        launch_hist_reduce(ctx, hg, live, stream);
     }
-    if (tileCounts)
-        k_tile_scan<<<256, 256, 0, stream>>>(tileCounts, tiles, ctx->status, ctx->ctrl, 0, ctx->partials, histBlocks,
-                                             histBlocks ? live : 0, ctx->hist);
-    else if (histBlocks)
-        launch_hist_reduce(ctx, histBlocks, live, stream);
+    if (tileCounts) k_tile_scan<<<256, 256, 0, stream>>>(tileCounts, tiles, ctx->status, ctx->ctrl, 0);
     K* src = keys;
     K* dst = ktmp;
     uint32_t* vsrc = vals;
@@ -822,10 +776,9 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
 }
 
 int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, int startBit, int endBit,
-                     bool histReady, hipStream_t stream, const uint32_t* tileCounts, int histBlocks)
+                     bool histReady, hipStream_t stream, const uint32_t* tileCounts)
 {
-    return sort_impl<uint64_t>(ctx, keys, nullptr, tmp, nullptr, n, startBit, endBit, histReady, stream, tileCounts,
-                               histBlocks);
+    return sort_impl<uint64_t>(ctx, keys, nullptr, tmp, nullptr, n, startBit, endBit, histReady, stream, tileCounts);
 }
 
 int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,
