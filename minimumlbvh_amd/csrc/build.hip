@@ -12,9 +12,10 @@
 //    sorted leaves are built each on its own from the deltas (ranges by
 //    nearest-greater searches, boxes by range unions in LDS), then the same
 //    over blocks of 1024 of the resulting subtrees; the rest (about 0.05% of
-//    the nodes of a surface mesh) climbs through device-scope arrival words,
-//    one u32 per internal node; `context` is written once, with the
-//    reference's final value;
+//    the nodes of a surface mesh) climbs inside one workgroup with its
+//    hand-offs in LDS (k_top_tree), or, when there are more of them than it
+//    holds, through device-scope arrival words; `context` is written once,
+//    with the reference's final value;
 //  * one thread per leaf for all n leaves (the reference launches
 //    ceil((n-1)/256) blocks and drops the last leaf when (n-1) % 256 == 0).
 #include <algorithm>
@@ -254,13 +255,17 @@ __global__ void __launch_bounds__(kBuildBlock)
 // j and j + 1 lies internal node hi_j with delta d[hi_j], and the nodes above
 // level 1 are exactly the tree these splits form under the same rule, so
 // level 2 runs the same kernel over the super-leaves in blocks of kLeaves2.
-// What crosses a level-2 block (a few thousand nodes for surface meshes)
-// climbs on through the device-scope arrival words (k_climb_upper,
-// climb_from).  The result is the tree of one thread per leaf climbing
-// through device memory all the way.  A node whose parent is built by a
-// later level gets its parent word from its own level (the rule above gives
-// it) and again, identically, from climb_from.  The blocks' barriers order
-// LDS only, so their global stores are never waited for.
+// Level 1's frontier stretches are laid out as the super-leaf list by
+// k_frontier_compact.  What crosses a level-2 block (a few thousand nodes for
+// surface meshes) is built by k_top_tree from level 2's frontier records, by
+// the same rule applied to the climb of one thread per entry through LDS; if
+// there are more than kTopMax entries, k_climb_upper climbs them through the
+// device-scope arrival words (climb_from) instead.  The result is the tree of
+// one thread per leaf climbing through device memory all the way.  A node
+// whose parent is built by a later level gets its parent word from its own
+// level (the rule above gives it) and again, identically, from the climb.
+// The blocks' barriers order LDS only, so their global stores are never
+// waited for.
 #ifndef MLBVH_LEVEL1_LEAVES
 #define MLBVH_LEVEL1_LEAVES 256
 #endif
