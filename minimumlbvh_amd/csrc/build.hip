@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(kBuildBlock)
 // k_frontier_compact.  What crosses a level-2 block (a few thousand nodes for
 // surface meshes) is built by k_top_tree from level 2's frontier records, by
 // the same rule applied to the climb of one thread per entry through LDS; if
-// there are more than kTopMax entries, k_climb_upper climbs them through the
+// there are more than kTopMax entries, the same launch climbs them through the
 // device-scope arrival words (climb_from) instead.  The result is the tree of
 // one thread per leaf climbing through device memory all the way.  A node
 // whose parent is built by a later level gets its parent word from its own
@@ -799,6 +799,42 @@ __global__ void __launch_bounds__(kCompactThreads)
     }
 }
 
+// The device-scope climb of level 2's frontier entries, for top lists longer
+// than kTopMax: wave w of nWaves takes level-2 blocks w, w + nWaves, ..., and
+// each of its lanes climbs one entry of the block's stretch (up to its end
+// mark) through the device-scope arrival words (climb_from).
+template <class TS>
+__device__ inline void climb_upper_wave(uint32_t w, uint32_t nWaves, TS tris, uint32_t n,
+                                        const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags,
+                                        const uint32_t* __restrict__ frontier, uint32_t count, uint32_t* root)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t blk = w; (size_t)blk * kLeaves2 < count; blk += nWaves) {
+        const uint32_t b0 = blk * kLeaves2;
+        const uint32_t m = count - b0 < (uint32_t)kLeaves2 ? count - b0 : (uint32_t)kLeaves2;
+        for (uint32_t base = 0; base < m; base += 64) {
+            const uint32_t k = base + lane;
+            const size_t e = 3 * (size_t)(b0 + k);
+            const uint32_t node = k < m ? frontier[e] : kFrontierEnd;
+            const uint64_t endMask = __ballot(node == kFrontierEnd);
+            const uint32_t valid = endMask ? (uint32_t)__ffsll((unsigned long long)endMask) - 1u : 64u;
+            if (lane < valid) {
+                const uint32_t lo = frontier[e + 1], hi = frontier[e + 2];
+                Box box;
+                if (node & kLeafBit) {
+                    box = tri_box(tris[node & ~kLeafBit]);
+                } else {
+                    const Node& nd = nodes[node];
+                    box.lo = umin3(nd.box[0].lo, nd.box[1].lo);
+                    box.hi = umax3(nd.box[0].hi, nd.box[1].hi);
+                }
+                climb_from(n, lo, hi, node, box, deltas, nodes, flags, root);
+            }
+            if (endMask) break;
+        }
+    }
+}
+
 // The nodes above level 2, in one workgroup.  Level 2's frontier entries, in
 // leaf order in the top list (node, last leaf, delta of the split after it,
 // subtree box), are the elements of a last sequence: entry j covers the
@@ -809,8 +845,10 @@ __global__ void __launch_bounds__(kCompactThreads)
 // Each node's fields are written by the threads that fill them (child and box
 // slots by the arriving child, the context by the second arriver, the parent
 // by the next step up), as in the device-scope climb.  With more than
-// kTopMax entries level 2 has set ctrl[kCtrlTopFallback] instead, and
-// k_climb_upper climbs them through device memory.
+// kTopMax entries level 2 has set ctrl[kCtrlTopFallback] instead, and the
+// launch's other blocks climb them through device memory (climb_upper_wave);
+// they return at once otherwise.  (One launch for both: a launch costs about
+// 4.5 us even when it returns at once.)
 //
 // The kernel also returns the build's tickets to zero and advances its
 // look-back tag, for the next build on this context.
@@ -818,10 +856,20 @@ constexpr int kTopThreads = 1024;
 constexpr int kTopPer = (int)kTopMax / kTopThreads;  // entries per thread
 constexpr uint32_t kTopReady = 0x80000000u;           // the first arriver's deposit is in place
 constexpr uint32_t kTopNoSplit = 0x100u;              // a missing split's delta: above every delta
+template <class TS>
 __global__ void __launch_bounds__(kTopThreads)
     k_top_tree(uint32_t n, Node* __restrict__ nodes, const uint32_t* __restrict__ topList, uint32_t* ctrl,
-               uint32_t* root, uint64_t* sceneBox)
+               uint32_t* root, uint64_t* sceneBox, TS tris, const uint8_t* __restrict__ deltas, uint32_t* flags,
+               const uint32_t* __restrict__ frontier)
 {
+    if (blockIdx.x > 0) {  // the device-scope climb, only for a long top list (flag set by level 2)
+        const uint32_t count = ctrl[kCtrlSuperCount];
+        if (count == 0 || ctrl[kCtrlTopFallback] == 0) return;
+        const uint32_t wavesPer = kTopThreads / 64;
+        climb_upper_wave(((blockIdx.x - 1) * wavesPer) + (threadIdx.x >> 6), (gridDim.x - 1) * wavesPer, tris, n,
+                         deltas, nodes, flags, frontier, count, root);
+        return;
+    }
     __shared__ Box s_box[kTopMax];       // per split: the first arriver's subtree box
     __shared__ uint32_t s_ctl[kTopMax];  // per split: arrivals (bits 0..7) | far end << 8 | kTopReady
     // entry j: its last leaf hi_j (the node index of split j) | the delta of split j << 32, read as
@@ -855,7 +903,7 @@ __global__ void __launch_bounds__(kTopThreads)
         if (tag == 0xFFFFFFFFu && t < kSceneBoxWords) sceneBox[t] = 0ull;
         if (t == 0) ctrl[kCtrlBuildTag] = tag == 0xFFFFFFFFu ? 0u : tag;
     }
-    if (F == 0 || F > kTopMax) return;  // the root was built below, or k_climb_upper climbs
+    if (F == 0 || F > kTopMax) return;  // the root was built below, or the device-scope climb runs
     uint32_t eNode[kTopPer];
     Box eBox[kTopPer];
 #pragma unroll
@@ -958,42 +1006,6 @@ __global__ void __launch_bounds__(kTopThreads)
     }
 }
 
-// One wave per level-2 block (looping over them), over that block's stretch
-// of the frontier list up to its end mark: each entry climbs through the
-// device-scope arrival words.  Runs only when k_top_tree found more entries
-// than it holds.
-template <class TS>
-__global__ void __launch_bounds__(64)
-    k_climb_upper(TS tris, uint32_t n, const uint8_t* __restrict__ deltas, Node* nodes, uint32_t* flags,
-                  const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ ctrl, uint32_t* root)
-{
-    if (ctrl[kCtrlTopFallback] == 0) return;
-    const uint32_t count = ctrl[kCtrlSuperCount];
-    for (uint32_t blk = blockIdx.x; (size_t)blk * kLeaves2 < count; blk += gridDim.x) {
-        const uint32_t b0 = blk * kLeaves2;
-        const uint32_t m = count - b0 < (uint32_t)kLeaves2 ? count - b0 : (uint32_t)kLeaves2;
-        for (uint32_t base = 0; base < m; base += 64) {
-            const uint32_t k = base + threadIdx.x;
-            const size_t e = 3 * (size_t)(b0 + k);
-            const uint32_t node = k < m ? frontier[e] : kFrontierEnd;
-            const uint64_t endMask = __ballot(node == kFrontierEnd);
-            const uint32_t valid = endMask ? (uint32_t)__ffsll((unsigned long long)endMask) - 1u : 64u;
-            if (threadIdx.x < valid) {
-                const uint32_t lo = frontier[e + 1], hi = frontier[e + 2];
-                Box box;
-                if (node & kLeafBit) {
-                    box = tri_box(tris[node & ~kLeafBit]);
-                } else {
-                    const Node& nd = nodes[node];
-                    box.lo = umin3(nd.box[0].lo, nd.box[1].lo);
-                    box.hi = umax3(nd.box[0].hi, nd.box[1].hi);
-                }
-                climb_from(n, lo, hi, node, box, deltas, nodes, flags, root);
-            }
-            if (endMask) break;
-        }
-    }
-}
 
 // fuseDeltas: compute the deltas from `rec` in level 1 and write them to
 // `deltas` (the 32-bit build); else read them from `deltas`.
@@ -1037,10 +1049,11 @@ static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uin
     k_local_nodes<TS, kLeaves2, kSuperLeaves><<<grid2, kLeaves2, 0, stream>>>(
         tris, (uint32_t)n, nullptr, deltas, nullptr, ctx->superList, ctx->ctrl, nodes, ctx->frontier, counts2,
         ctx->flags, root, ctx->l1Status + compactTiles, ctx->topList);
-    k_top_tree<<<1, kTopThreads, 0, stream>>>((uint32_t)n, nodes, ctx->topList, ctx->ctrl, root, ctx->sceneBox);
-    const unsigned gridUp = (unsigned)std::min<size_t>(blocks2Max, (size_t)ctx->cuCount * 8);
-    k_climb_upper<<<gridUp, 64, 0, stream>>>(tris, (uint32_t)n, deltas, nodes, ctx->flags, ctx->frontier, ctx->ctrl,
-                                             root);
+    // block 0: the top tree; the others: the device-scope climb when the top list overflowed
+    const unsigned climbBlocks = (unsigned)std::min<size_t>((blocks2Max + 15) / 16, (size_t)ctx->cuCount);
+    k_top_tree<TS><<<1 + climbBlocks, kTopThreads, 0, stream>>>((uint32_t)n, nodes, ctx->topList, ctx->ctrl, root,
+                                                                 ctx->sceneBox, tris, deltas, ctx->flags,
+                                                                 ctx->frontier);
     return MLBVH_OK;
 }
 
@@ -1096,8 +1109,7 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         uint32_t* tileCounts = ctx->partials + (size_t)gm * 4 * 256;
         k_morton<<<gm, kMortonBlock, 0, stream>>>(tris, n, ctx->sceneBox, opt, ctx->recA, ctx->partials,
                                                  (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
-        launch_hist_reduce(ctx, (int)gm, 4, stream);
-        rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts);
+        rc = sort_records_u64(ctx, ctx->recA, ctx->recB, (uint32_t)n, 0, 32, true, stream, tileCounts, (int)gm);
         if (rc) return rc;
     }
     // 32-bit keys: the local climb computes the deltas from the records

@@ -161,13 +161,15 @@ int set_device(mlbvh_ctx* ctx);
 
 // Radix sort passes shared by mlbvh_sort_* and the builders.  When histReady
 // is true the caller has already run sort_state_init and filled
-// ctx->hist[0..passes) (the builders fuse both into their own kernels) and
-// has called ensure_status(ctx, sort_tiles(n)).
+// ctx->hist[0..passes) or the partials below (the builders fuse both into
+// their own kernels) and has called ensure_status(ctx, sort_tiles(n)).
 // With tileCounts (digit-major per-tile counts of the first digit, over
 // sort_tiles(n) tiles of sort_tile_keys keys) the first pass reads prefixes
-// instead of looking back.
+// instead of looking back.  With histBlocks > 0 the histograms are summed
+// here from ctx->partials (histBlocks rows of [pass][256]) into the zeroed
+// ctx->hist, in the same launch as the prefix scan.
 int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, int startBit, int endBit,
-                     bool histReady, hipStream_t stream, const uint32_t* tileCounts = nullptr);
+                     bool histReady, hipStream_t stream, const uint32_t* tileCounts = nullptr, int histBlocks = 0);
 // Key-value sort (u64 keys, u32 values) with its own histograms, as
 // mlbvh_sort_u64.
 int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,

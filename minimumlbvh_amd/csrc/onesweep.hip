@@ -5,10 +5,12 @@
 // startBit) / 8) passes, result left in the caller's keys/values.
 //
 // Structure (one upfront histogram, then one kernel per digit pass):
-//   k_sort_init    hand every pass a fresh look-back tag, zero tickets
 //   k_hist         all digit histograms of the input in one read, written as
-//                  per-block partials (no global atomics)
-//   k_hist_reduce  partials -> per-pass histograms
+//                  per-block partials (no global atomics), and the first
+//                  pass's per-tile digit counts; its block 0 hands every pass
+//                  a fresh look-back tag and zeroes the tickets
+//   k_scan_reduce  partials -> per-pass histograms, and the first pass's
+//                  per-tile prefixes from the counts (its tiles do not look back)
 //   k_pass         per tile: wave64 ballot multisplit ranking, per-digit
 //                  decoupled look-back over a [tile][digit] status table,
 //                  LDS-staged local scatter, writes of contiguous digit runs
@@ -37,9 +39,6 @@ struct SortCfg {
 constexpr SortCfg kSortCfgs[kNumSortCfgs] = {{256, 16}, {256, 32}, {512, 16}, {512, 24}, {1024, 16}};
 constexpr int kHistBlock = 256;
 
-#ifndef MLBVH_PRESCAN
-#define MLBVH_PRESCAN 1
-#endif
 constexpr uint64_t kFlagA = 1;  // tile aggregate published
 constexpr uint64_t kFlagP = 2;  // inclusive prefix published
 
@@ -73,23 +72,29 @@ __device__ inline uint32_t scan256(uint32_t v, uint32_t* s_tmp)
     return base + x - v;
 }
 
-// nPass pass slots (tags, tickets); nHist <= 8 histograms zeroed.
-__global__ void k_sort_init(uint32_t* ctrl, int nPass, int nHist, uint32_t* hist, uint64_t* status, size_t statusWords)
-{
-    sort_state_init(ctrl, nPass, hist, nHist, status, statusWords);
-}
-
 // With tileCounts, blocks take whole sort tiles of tileKeys keys (tile-
 // strided) and also write each tile's counts of the first digit,
-// tileCounts[digit * nTiles + tile], for k_tile_scan.
+// tileCounts[digit * nTiles + tile], for k_scan_reduce.  With init, block 0
+// first sets up the sort's state (sort_state_init: init->passes pass slots,
+// init->hists histograms), which no kernel before the next one reads.
+struct SortInit {
+    uint32_t* ctrl;
+    int passes, hists;
+    uint32_t* hist;
+    uint64_t* status;
+    size_t statusWords;
+};
 template <class K>
 __global__ void __launch_bounds__(kHistBlock) k_hist(const K* __restrict__ keys, uint32_t n, int startBit,
                                                      int nPass, uint32_t* __restrict__ partials,
                                                      const uint32_t* __restrict__ guard = nullptr,
                                                      uint32_t tileKeys = 0, uint32_t nTiles = 0,
-                                                     uint32_t* __restrict__ tileCounts = nullptr)
+                                                     uint32_t* __restrict__ tileCounts = nullptr,
+                                                     SortInit init = SortInit{})
 {
     if (guard && *guard == 0) return;
+    if (init.ctrl && blockIdx.x == 0)
+        sort_state_init(init.ctrl, init.passes, init.hist, init.hists, init.status, init.statusWords);
     __shared__ uint32_t s_h[8][256];
     for (int p = 0; p < 8; p++) s_h[p][threadIdx.x] = 0;
     const int keyBits = 8 * (int)sizeof(K);
@@ -169,15 +174,33 @@ __global__ void __launch_bounds__(256) k_hist_reduce(const uint32_t* __restrict_
     if (s) atomicAdd(&hist[p * 256 + threadIdx.x], s);
 }
 
-// The first digit pass's inclusive prefixes filled in advance from per-tile
-// counts: status[d * nTiles + tile] = (that pass's tag, prefix flag, count
-// of digit d in tiles 0..tile).  Digit-major, so that each lane writes whole
-// lines; the other passes' look-back words ([tile][digit]) carry other tags,
-// so sharing the table is safe.  One block per digit.
-__global__ void __launch_bounds__(256) k_tile_scan(const uint32_t* __restrict__ tileCounts, uint32_t nTiles,
-                                                   uint64_t* __restrict__ status, const uint32_t* __restrict__ ctrl,
-                                                   int pass)
+// Two independent jobs in one launch (a launch costs about 4.5 us even when
+// it does little):
+//  - blocks 0 .. 255 (with tileCounts): the first digit pass's inclusive
+//    prefixes filled in advance from per-tile counts: status[d * nTiles +
+//    tile] = (that pass's tag, prefix flag, count of digit d in tiles
+//    0..tile).  Digit-major, so that each lane writes whole lines; the other
+//    passes' look-back words ([tile][digit]) carry other tags, so sharing the
+//    table is safe.  One block per digit.
+//  - the blocks after them: hist[p][d] += partials of nBlocks blocks, as
+//    k_hist_reduce (block r: pass r % nPass, chunk r / nPass).
+__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ tileCounts, uint32_t nTiles,
+                                                     uint64_t* __restrict__ status, const uint32_t* __restrict__ ctrl,
+                                                     int pass, const uint32_t* __restrict__ partials, int nBlocks,
+                                                     int nPass, uint32_t* __restrict__ hist)
 {
+    const uint32_t scanBlocks = tileCounts ? 256u : 0u;
+    if (blockIdx.x >= scanBlocks) {
+        const int r = (int)(blockIdx.x - scanBlocks);
+        const int p = r % nPass;
+        const int b0 = (r / nPass) * kReduceChunk;
+        const int b1 = min(b0 + kReduceChunk, nBlocks);
+        uint32_t s = 0;
+#pragma unroll 8
+        for (int b = b0; b < b1; b++) s += partials[((size_t)b * nPass + p) * 256 + threadIdx.x];
+        if (s) atomicAdd(&hist[p * 256 + threadIdx.x], s);
+        return;
+    }
     __shared__ uint32_t s_tmp[4];
     const uint32_t d = blockIdx.x;
     const uint32_t epoch = ctrl[kCtrlEpochs + pass];
@@ -193,6 +216,18 @@ __global__ void __launch_bounds__(256) k_tile_scan(const uint32_t* __restrict__ 
         run += col[j];
         out[j] = lb_pack(epoch, kFlagP, run);
     }
+}
+
+// k_scan_reduce for the first pass's tile counts (or none) and histBlocks
+// rows of nPass partial histograms (or none) into hist.
+static void launch_scan_reduce(mlbvh_ctx* ctx, const uint32_t* tileCounts, uint32_t tiles, int histBlocks, int nPass,
+                               uint32_t* hist, hipStream_t stream)
+{
+    const unsigned reduceBlocks = histBlocks > 0 ? (unsigned)(nPass * ((histBlocks + kReduceChunk - 1) / kReduceChunk)) : 0u;
+    const unsigned grid = (tileCounts ? 256u : 0u) + reduceBlocks;
+    if (grid)
+        k_scan_reduce<<<grid, 256, 0, stream>>>(tileCounts, tiles, ctx->status, ctx->ctrl, 0, ctx->partials,
+                                                histBlocks, nPass > 0 ? nPass : 1, hist);
 }
 
 template <class K, bool kVals, int kBlock, int kItems>
@@ -297,7 +332,7 @@ __global__ void __launch_bounds__(kBlock)
         // Decoupled look-back for digit t.
         uint64_t* my = status + (uint64_t)tile * 256 + t;
         uint32_t excl = 0;
-        if (preScanned) {  // k_tile_scan has published every tile's inclusive prefix
+        if (preScanned) {  // k_scan_reduce has published every tile's inclusive prefix
             if (tile != 0) excl = (uint32_t)status[(uint64_t)t * gridDim.x + tile - 1];
         } else if (tile != 0) {  // the aggregate (tot) went out before the ranking
             int64_t j = (int64_t)tile - 1;
@@ -620,28 +655,22 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
     const size_t partialRows = (size_t)hg * 3 + tiles > (size_t)hg * live ? (size_t)hg * 3 + tiles : (size_t)hg * live;
     if ((rc = ensure_partials(ctx, (int)partialRows, 1))) return rc;
     uint32_t* tileCounts = ctx->partials + (size_t)hg * 3 * 256;
-    // pass slots: 0..2 = the three top digits; 3 .. 3+live-1 = the fallback sequence
-    k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, 3 + live, live, ctx->hist, ctx->status, ctx->statusWords);
-    // the three top digits' histograms now (the rest only if the fallback
-    // runs), with the first pass's per-tile counts: its look-back table is
-    // then filled in advance and its tiles never wait on each other
-#if MLBVH_PRESCAN
+    // pass slots: 0..2 = the three top digits; 3 .. 3+live-1 = the fallback
+    // sequence (set up by k_hist's block 0).  The three top digits'
+    // histograms now (the rest only if the fallback runs), with the first
+    // pass's per-tile counts: its look-back table is then filled in advance
+    // and its tiles never wait on each other
+    const SortInit init{ctx->ctrl, 3 + live, live, ctx->hist, ctx->status, ctx->statusWords};
     k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - 3), 3, ctx->partials, nullptr,
-                                             (uint32_t)sort_tile_keys(ctx), tiles, tileCounts);
-    launch_hist_reduce(ctx, hg, 3, stream, live - 3);
-    k_tile_scan<<<256, 256, 0, stream>>>(tileCounts, tiles, ctx->status, ctx->ctrl, 0);
-#else
-    (void)tileCounts;
-    k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - 3), 3, ctx->partials);
-    launch_hist_reduce(ctx, hg, 3, stream, live - 3);
-#endif
+                                             (uint32_t)sort_tile_keys(ctx), tiles, tileCounts, init);
+    launch_scan_reduce(ctx, tileCounts, tiles, hg, 3, ctx->hist + 256 * (live - 3), stream);
     const uint32_t* guard = ctx->ctrl + kCtrlSortFallback;
     K* bufs[2] = {keys, ktmp};
     uint32_t* vbufs[2] = {vals, vtmp};
     for (int q = 0; q < 3; q++) {
         const int p = live - 3 + q;
         const int shift = startBit + 8 * p;
-        const int pre = MLBVH_PRESCAN && q == 0;
+        const int pre = q == 0;
         if (vals)
             dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, bufs[q & 1], bufs[(q + 1) & 1], vbufs[q & 1],
                                    vbufs[(q + 1) & 1], n, shift, ctx->hist + 256 * p, ctx->status, ctx->ctrl, q,
@@ -696,7 +725,8 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
 
 template <class K>
 static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, uint32_t n, int startBit,
-                     int endBit, bool histReady, hipStream_t stream, const uint32_t* readyTileCounts = nullptr)
+                     int endBit, bool histReady, hipStream_t stream, const uint32_t* readyTileCounts = nullptr,
+                     int readyHistBlocks = 0)
 {
     const int keyBits = 8 * (int)sizeof(K);
     if (startBit < 0 || endBit <= startBit) {
@@ -725,23 +755,21 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
                                                                              tiles, stream);
 
     // per-tile counts of the first digit (from the caller's histogram kernel
-    // or from k_hist here) let the first pass skip its look-back
-    const uint32_t* tileCounts = MLBVH_PRESCAN ? readyTileCounts : nullptr;
+    // or from k_hist here) let the first pass skip its look-back; partial
+    // histograms not reduced yet (histBlocks rows) are summed in the same launch
+    const uint32_t* tileCounts = readyTileCounts;
+    int histBlocks = readyHistBlocks;
     if (!histReady) {
         int hg = hist_grid(ctx, n);
-        if ((rc = ensure_partials(ctx, hg * live + (MLBVH_PRESCAN ? (int)tiles : 0), 1))) return rc;
-        k_sort_init<<<1, 256, 0, stream>>>(ctx->ctrl, live, live, ctx->hist, ctx->status, ctx->statusWords);
-#if MLBVH_PRESCAN
+        if ((rc = ensure_partials(ctx, hg * live + (int)tiles, 1))) return rc;
         uint32_t* counts = ctx->partials + (size_t)hg * live * 256;
+        const SortInit init{ctx->ctrl, live, live, ctx->hist, ctx->status, ctx->statusWords};
         k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials, nullptr,
-                                                 (uint32_t)sort_tile_keys(ctx), tiles, counts);
+                                                 (uint32_t)sort_tile_keys(ctx), tiles, counts, init);
         tileCounts = counts;
-#else
-        k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit, live, ctx->partials);
-#endif
-        launch_hist_reduce(ctx, hg, live, stream);
+        histBlocks = hg;
     }
-    if (tileCounts) k_tile_scan<<<256, 256, 0, stream>>>(tileCounts, tiles, ctx->status, ctx->ctrl, 0);
+    launch_scan_reduce(ctx, tileCounts, tiles, histBlocks, live, ctx->hist, stream);
     K* src = keys;
     K* dst = ktmp;
     uint32_t* vsrc = vals;
@@ -776,9 +804,10 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
 }
 
 int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, int startBit, int endBit,
-                     bool histReady, hipStream_t stream, const uint32_t* tileCounts)
+                     bool histReady, hipStream_t stream, const uint32_t* tileCounts, int histBlocks)
 {
-    return sort_impl<uint64_t>(ctx, keys, nullptr, tmp, nullptr, n, startBit, endBit, histReady, stream, tileCounts);
+    return sort_impl<uint64_t>(ctx, keys, nullptr, tmp, nullptr, n, startBit, endBit, histReady, stream, tileCounts,
+                               histBlocks);
 }
 
 int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,
