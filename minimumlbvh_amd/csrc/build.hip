@@ -32,7 +32,8 @@ constexpr int kBuildBlock = 256;
 // upper bounds and ~f2ord(lo) for the lower ones, tag = the build's tag
 // (kCtrlBuildTag + 1).  A newer tag wins over whatever an earlier build left,
 // so the slots are never cleared (zeroed slots carry tag 0, which no build
-// uses; k_top_tree zeroes them when the tag would wrap).
+// uses; level 1 advances the tag for the next build, and zeroes the slots
+// when it would wrap).
 #ifndef MLBVH_BOX_COPIES
 #define MLBVH_BOX_COPIES 32
 #endif
@@ -404,7 +405,7 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
                   uint8_t* __restrict__ deltasOut, const uint32_t* __restrict__ elems, uint32_t* __restrict__ ctrl,
                   Node* __restrict__ nodes, uint32_t* __restrict__ frontier, uint32_t* __restrict__ blockCounts,
                   uint32_t* __restrict__ flags, uint32_t* __restrict__ root, uint64_t* __restrict__ status2,
-                  uint32_t* __restrict__ topList)
+                  uint32_t* __restrict__ topList, uint64_t* __restrict__ sceneBox)
 {
     constexpr int kLift = log2_exact(kLeaves) + 1;  // windows of 1 .. kLeaves deltas
     static_assert(kLeaves == 1 << (kLift - 1) && kLeaves >= 256 && kLeaves <= 1024,
@@ -435,6 +436,24 @@ __global__ void __launch_bounds__(kLeaves) __attribute__((amdgpu_waves_per_eu(kL
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     constexpr bool kLevel1 = kMode != kSuperLeaves;
+    if (kLevel1 && blockIdx.x == 0) {
+        // This call's later kernels start from ticket 0 with a fresh look-back
+        // tag (their words carry tag + 1 != 0), and the next build's scene box
+        // slots get a fresh tag (on wrap the slots restart from zero).  No
+        // kernel before this one in the call reads these words, and none after
+        // it reads them before it ends.
+        const uint32_t boxTag = ctrl[kCtrlBuildTag] + 1u;
+        if (t == 0) {
+            ctrl[kCtrlCompactTicket] = 0u;
+            ctrl[kCtrlL2Ticket] = 0u;
+            const uint32_t tag = ctrl[kCtrlCompactTag] + 1u;
+            ctrl[kCtrlCompactTag] = tag == 0xFFFFFFFFu ? 0u : tag;
+        }
+        if (boxTag == 0xFFFFFFFFu)
+            for (int k = t; k < kSceneBoxWords; k += kLeaves) sceneBox[k] = 0ull;
+        lds_barrier();  // every thread has read the box tag
+        if (t == 0) ctrl[kCtrlBuildTag] = boxTag == 0xFFFFFFFFu ? 0u : boxTag;
+    }
     const uint32_t count = kLevel1 ? n : ctrl[kCtrlSuperCount];
     const uint32_t nInternals = n - 1;
     // Level 1 has one block per kLeaves leaves; level 2 takes its blocks by
@@ -850,8 +869,6 @@ __device__ inline void climb_upper_wave(uint32_t w, uint32_t nWaves, TS tris, ui
 // they return at once otherwise.  (One launch for both: a launch costs about
 // 4.5 us even when it returns at once.)
 //
-// The kernel also returns the build's tickets to zero and advances its
-// look-back tag, for the next build on this context.
 constexpr int kTopThreads = 1024;
 constexpr int kTopPer = (int)kTopMax / kTopThreads;  // entries per thread
 constexpr uint32_t kTopReady = 0x80000000u;           // the first arriver's deposit is in place
@@ -859,7 +876,7 @@ constexpr uint32_t kTopNoSplit = 0x100u;              // a missing split's delta
 template <class TS>
 __global__ void __launch_bounds__(kTopThreads)
     k_top_tree(uint32_t n, Node* __restrict__ nodes, const uint32_t* __restrict__ topList, uint32_t* ctrl,
-               uint32_t* root, uint64_t* sceneBox, TS tris, const uint8_t* __restrict__ deltas, uint32_t* flags,
+               uint32_t* root, TS tris, const uint8_t* __restrict__ deltas, uint32_t* flags,
                const uint32_t* __restrict__ frontier)
 {
     if (blockIdx.x > 0) {  // the device-scope climb, only for a long top list (flag set by level 2)
@@ -890,19 +907,7 @@ __global__ void __launch_bounds__(kTopThreads)
     // no super-leaves: level 1 built the root, level 2 ran no block and wrote no count
     const bool none = ctrl[kCtrlSuperCount] == 0;
     const uint32_t F = none ? 0u : ctrl[kCtrlTopCount];
-    if (t == 0) {  // the next build starts from ticket 0 with fresh tags
-        if (none) ctrl[kCtrlTopFallback] = 0u;
-        ctrl[kCtrlCompactTicket] = 0u;
-        ctrl[kCtrlL2Ticket] = 0u;
-        const uint32_t tag = ctrl[kCtrlCompactTag] + 1u;
-        ctrl[kCtrlCompactTag] = tag == 0xFFFFFFFFu ? 0u : tag;  // the next build's words carry tag + 1 != 0
-    }
-    {  // the scene box slots' tag likewise; on wrap the slots restart from zero
-        const uint32_t tag = ctrl[kCtrlBuildTag] + 1u;
-        __syncthreads();
-        if (tag == 0xFFFFFFFFu && t < kSceneBoxWords) sceneBox[t] = 0ull;
-        if (t == 0) ctrl[kCtrlBuildTag] = tag == 0xFFFFFFFFu ? 0u : tag;
-    }
+    if (t == 0 && none) ctrl[kCtrlTopFallback] = 0u;
     if (F == 0 || F > kTopMax) return;  // the root was built below, or the device-scope climb runs
     uint32_t eNode[kTopPer];
     Box eBox[kTopPer];
@@ -1029,18 +1034,18 @@ static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uin
     }
     if (!rc) rc = ensure(ctx, &ctx->topList, &ctx->topListCap, (size_t)kTopMax * kTopWords, "hipMalloc(top list)");
     if (rc) return rc;
-    // Every kernel below is launched on every call: k_top_tree returns the
-    // compaction's ticket to zero and advances its tag for the next build.
+    // Level 1's block 0 resets the tickets and advances the look-back tag of
+    // the kernels after it, so each call starts clean.
     uint32_t* counts1 = ctx->blockCounts;            // level-1 frontier counts
     uint32_t* counts2 = ctx->blockCounts + blocks1;  // level-2 frontier counts
     if (fuseDeltas)
         k_local_nodes<TS, kLeaves1, kLeavesFusedDeltas><<<blocks1, kLeaves1, 0, stream>>>(
             tris, (uint32_t)n, rec, nullptr, deltas, nullptr, ctx->ctrl, nodes, ctx->frontier, counts1, nullptr,
-            root, nullptr, nullptr);
+            root, nullptr, nullptr, ctx->sceneBox);
     else
         k_local_nodes<TS, kLeaves1, kLeavesGivenDeltas><<<blocks1, kLeaves1, 0, stream>>>(
             tris, (uint32_t)n, rec, deltas, nullptr, nullptr, ctx->ctrl, nodes, ctx->frontier, counts1, nullptr,
-            root, nullptr, nullptr);
+            root, nullptr, nullptr, ctx->sceneBox);
     k_frontier_compact<<<compactTiles, kCompactThreads, 0, stream>>>(counts1, blocks1, ctx->frontier,
                                                                     ctx->superList, kLeaves1, ctx->l1Status, ctx->ctrl);
     // level 2 and the upper climb loop over however many level-2 blocks the
@@ -1048,12 +1053,11 @@ static int launch_climb(mlbvh_ctx* ctx, TS tris, int n, const uint64_t* rec, uin
     const unsigned grid2 = (unsigned)std::min<size_t>(blocks2Max, (size_t)ctx->cuCount * 2);
     k_local_nodes<TS, kLeaves2, kSuperLeaves><<<grid2, kLeaves2, 0, stream>>>(
         tris, (uint32_t)n, nullptr, deltas, nullptr, ctx->superList, ctx->ctrl, nodes, ctx->frontier, counts2,
-        ctx->flags, root, ctx->l1Status + compactTiles, ctx->topList);
+        ctx->flags, root, ctx->l1Status + compactTiles, ctx->topList, nullptr);
     // block 0: the top tree; the others: the device-scope climb when the top list overflowed
     const unsigned climbBlocks = (unsigned)std::min<size_t>((blocks2Max + 15) / 16, (size_t)ctx->cuCount);
     k_top_tree<TS><<<1 + climbBlocks, kTopThreads, 0, stream>>>((uint32_t)n, nodes, ctx->topList, ctx->ctrl, root,
-                                                                 ctx->sceneBox, tris, deltas, ctx->flags,
-                                                                 ctx->frontier);
+                                                                 tris, deltas, ctx->flags, ctx->frontier);
     return MLBVH_OK;
 }
 

@@ -70,11 +70,12 @@ constexpr int kCtrlSuperCount = 51;    // super-leaves of the build's second blo
 constexpr int kCtrlAoTiles = 52;       // next pixel tile of the persistent AO kernel
 constexpr int kCtrlPtTiles = 53;       // next pixel tile of the persistent path-tracing kernel
 constexpr int kCtrlTopFallback = 54;   // set when the nodes above level 2 climb through device memory
-constexpr int kCtrlCompactTicket = 55;  // next workgroup of the level-1 frontier compaction (reset after each build)
-constexpr int kCtrlCompactTag = 56;     // the build's look-back words carry this + 1 (advanced after each build)
-constexpr int kCtrlL2Ticket = 57;       // next block of the build's second block-local level (reset after each build)
+// The words below are reset or advanced by level 1's block 0 at the start of each build or refit climb.
+constexpr int kCtrlCompactTicket = 55;  // next workgroup of the level-1 frontier compaction
+constexpr int kCtrlCompactTag = 56;     // the build's look-back words carry this + 1
+constexpr int kCtrlL2Ticket = 57;       // next block of the build's second block-local level
 constexpr int kCtrlTopCount = 58;       // entries of the top-tree list (level 2's frontier)
-constexpr int kCtrlBuildTag = 59;       // the scene box slots carry this + 1 (advanced after each build)
+constexpr int kCtrlBuildTag = 59;       // the scene box slots carry this + 1 (advanced for the next build)
 constexpr int kCtrlWords = 64;
 // The build's top-tree list (mlbvh_ctx::topList): at most kTopMax records of
 // kTopWords words, one per level-2 frontier entry, in leaf order: node, last

@@ -12,9 +12,9 @@
 // the top-tree list (the super-leaf list reuses the sort's pong buffers).  Every kernel
 // of the build clears or overwrites what it reads before reading it
 // (histograms, sort tickets, arrival words, frontier counts), and the
-// build's last kernel returns the build's tickets to zero and advances the
-// tags of its look-back words and scene-box slots, so a call does not depend
-// on other contents a previous call left.  All calls on one GPU share
+// build's first block-local level returns the build's tickets to zero and
+// advances the tags of its look-back words and scene-box slots, so a call
+// does not depend on other contents a previous call left.  All calls on one GPU share
 // that GPU's array: calls that overlap in time on one GPU race and give
 // undefined results; order them on one stream.
 //
