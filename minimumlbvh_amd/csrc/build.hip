@@ -894,30 +894,32 @@ __global__ void __launch_bounds__(kTopThreads)
     __shared__ uint64_t s_pk[kTopMax];
 
     const int t = threadIdx.x;
-    // Entries kTopPer * t .. kTopPer * t + kTopPer - 1 (consecutive entries
-    // are often siblings, so a thread's climbs tend to meet), loaded before
-    // their number is known (the list holds kTopMax records).
-    uint4 rec[kTopPer][3];
-#pragma unroll
-    for (int q = 0; q < kTopPer; q++) {
-        const uint4* src = reinterpret_cast<const uint4*>(topList + (size_t)(kTopPer * t + q) * kTopWords);
-#pragma unroll
-        for (int k = 0; k < 3; k++) rec[q][k] = src[k];
-    }
     // no super-leaves: level 1 built the root, level 2 ran no block and wrote no count
     const bool none = ctrl[kCtrlSuperCount] == 0;
     const uint32_t F = none ? 0u : ctrl[kCtrlTopCount];
     if (t == 0 && none) ctrl[kCtrlTopFallback] = 0u;
     if (F == 0 || F > kTopMax) return;  // the root was built below, or the device-scope climb runs
+    // Entries i0 .. i0 + per - 1 of this thread, per = ceil(F / kTopThreads):
+    // consecutive entries are often siblings, so a thread's climbs tend to meet.
+    const uint32_t per = (F + kTopThreads - 1) / kTopThreads;
+    const uint32_t i0 = (uint32_t)t * per;
+    uint4 rec[kTopPer][3];
+#pragma unroll
+    for (int q = 0; q < kTopPer; q++) {
+        const uint32_t i = i0 + (uint32_t)q;
+        const uint4* src = reinterpret_cast<const uint4*>(topList + (size_t)(i < F ? i : 0u) * kTopWords);
+#pragma unroll
+        for (int k = 0; k < 3; k++) rec[q][k] = (uint32_t)q < per && i < F ? src[k] : make_uint4(0u, 0u, 0u, 0u);
+    }
     uint32_t eNode[kTopPer];
     Box eBox[kTopPer];
 #pragma unroll
     for (int q = 0; q < kTopPer; q++) {
-        const uint32_t i = (uint32_t)(kTopPer * t + q);
+        const uint32_t i = i0 + (uint32_t)q;
         eNode[q] = rec[q][0].x;
         eBox[q].lo = mk(__uint_as_float(rec[q][1].x), __uint_as_float(rec[q][1].y), __uint_as_float(rec[q][1].z));
         eBox[q].hi = mk(__uint_as_float(rec[q][1].w), __uint_as_float(rec[q][2].x), __uint_as_float(rec[q][2].y));
-        if (i < F) {
+        if ((uint32_t)q < per && i < F) {
             const uint32_t d = rec[q][0].z == kDeltaInf ? kTopNoSplit : rec[q][0].z;
             s_pk[i] = (uint64_t)rec[q][0].y | ((uint64_t)d << 32);
             s_ctl[i] = 0u;
@@ -934,7 +936,6 @@ __global__ void __launch_bounds__(kTopThreads)
     // operation, keeps the loop from being rewritten into such a wait.  A lane
     // whose entry stops (first arriver, or root) takes its next entry.
     const uint32_t nInternals = n - 1;
-    const uint32_t i0 = (uint32_t)(kTopPer * t);
     int q = 0;                // the entry being climbed: i0 + q
     bool active = i0 < F;     // an entry of this lane is climbing
     bool waiting = false;     // arrived second at split p, the sibling's deposit not yet read
@@ -998,7 +999,7 @@ __global__ void __launch_bounds__(kTopThreads)
         if (done) {  // the next entry of this lane, if any
             q++;
             const uint32_t i = i0 + (uint32_t)q;
-            active = q < kTopPer && i < F;
+            active = (uint32_t)q < per && i < F;
             a = b = i;
 #pragma unroll
             for (int k = 1; k < kTopPer; k++)
