@@ -859,20 +859,34 @@ __device__ inline void climb_upper_wave(uint32_t w, uint32_t nWaves, TS tris, ui
 // subtree box), are the elements of a last sequence: entry j covers the
 // leaves [lo_j, hi_j], and the split between entries j and j + 1 is internal
 // node hi_j with delta d[hi_j].  The climb runs over entry indices with its
-// hand-offs in LDS: at each split the first of the two arrivers leaves its
-// subtree box and far end there and stops, the second merges and goes on.
-// Each node's fields are written by the threads that fill them (child and box
-// slots by the arriving child, the context by the second arriver, the parent
-// by the next step up), as in the device-scope climb.  With more than
-// kTopMax entries level 2 has set ctrl[kCtrlTopFallback] instead, and the
-// launch's other blocks climb them through device memory (climb_upper_wave);
-// they return at once otherwise.  (One launch for both: a launch costs about
-// 4.5 us even when it returns at once.)
+// hand-offs in LDS.  A node arriving at split p leaves its subtree box and
+// node index in slot p if it is p's left child (its range ends at entry p) or
+// in slot p + 1 if it is the right child (its range starts there), then adds
+// one arrival and its far end to p's word in a single atomic: the first
+// arriver stops, the second finds its sibling's far end in the value the
+// atomic returns and its box and index in the sibling's slot, and goes on
+// with p.  Slot q holds entry q to start with and is written later by at most
+// one ancestor of entry q, which climbs only after entry q's parent is
+// complete, so no deposit is overwritten before it is read, and no arriver
+// waits.  A completed node is written whole (64 bytes) with the next step,
+// which finds its parent and its context; an entry only gets its parent.
+// With more than kTopMax entries level 2 has set ctrl[kCtrlTopFallback]
+// instead, and the launch's other blocks climb them through device memory
+// (climb_upper_wave); they return at once otherwise.  (One launch for both: a
+// launch costs about 4.5 us even when it returns at once.)
 //
+// The loop below is bound by instruction issue: every wave with an active
+// lane runs the whole step, so kTopClimbers (fewer than the block's threads,
+// which all load the list) trades a few more steps per lane for fewer waves
+// per SIMD.
 constexpr int kTopThreads = 1024;
-constexpr int kTopPer = (int)kTopMax / kTopThreads;  // entries per thread
-constexpr uint32_t kTopReady = 0x80000000u;           // the first arriver's deposit is in place
-constexpr uint32_t kTopNoSplit = 0x100u;              // a missing split's delta: above every delta
+#ifndef MLBVH_TOP_CLIMBERS
+#define MLBVH_TOP_CLIMBERS 512
+#endif
+constexpr int kTopClimbers = MLBVH_TOP_CLIMBERS;
+static_assert(kTopClimbers % 64 == 0 && kTopClimbers <= kTopThreads, "whole waves of the block climb");
+constexpr uint32_t kTopNoSplit = 0x100u;  // a missing split's delta: above every delta
+constexpr int kTopArrive = 41;            // split words: last leaf | delta << 32 | (arrivals + 2 far ends) << 41
 template <class TS>
 __global__ void __launch_bounds__(kTopThreads)
     k_top_tree(uint32_t n, Node* __restrict__ nodes, const uint32_t* __restrict__ topList, uint32_t* ctrl,
@@ -887,10 +901,10 @@ __global__ void __launch_bounds__(kTopThreads)
                          deltas, nodes, flags, frontier, count, root);
         return;
     }
-    __shared__ Box s_box[kTopMax];       // per split: the first arriver's subtree box
-    __shared__ uint32_t s_ctl[kTopMax];  // per split: arrivals (bits 0..7) | far end << 8 | kTopReady
-    // entry j: its last leaf hi_j (the node index of split j) | the delta of split j << 32, read as
-    // above every delta (kTopNoSplit) for the last entry, which has no split after it
+    __shared__ Box s_box[kTopMax];       // deposit slots: subtree box
+    __shared__ uint32_t s_nd[kTopMax];   // deposit slots: node index
+    // split j: the last leaf of entry j (its node index) | the delta of split j << 32 (kTopNoSplit
+    // for the last entry, which has no split after it) | its arrival count and far ends << kTopArrive
     __shared__ uint64_t s_pk[kTopMax];
 
     const int t = threadIdx.x;
@@ -899,116 +913,110 @@ __global__ void __launch_bounds__(kTopThreads)
     const uint32_t F = none ? 0u : ctrl[kCtrlTopCount];
     if (t == 0 && none) ctrl[kCtrlTopFallback] = 0u;
     if (F == 0 || F > kTopMax) return;  // the root was built below, or the device-scope climb runs
-    // Entries i0 .. i0 + per - 1 of this thread, per = ceil(F / kTopThreads):
-    // consecutive entries are often siblings, so a thread's climbs tend to meet.
-    const uint32_t per = (F + kTopThreads - 1) / kTopThreads;
-    const uint32_t i0 = (uint32_t)t * per;
-    uint4 rec[kTopPer][3];
-#pragma unroll
-    for (int q = 0; q < kTopPer; q++) {
-        const uint32_t i = i0 + (uint32_t)q;
-        const uint4* src = reinterpret_cast<const uint4*>(topList + (size_t)(i < F ? i : 0u) * kTopWords);
-#pragma unroll
-        for (int k = 0; k < 3; k++) rec[q][k] = (uint32_t)q < per && i < F ? src[k] : make_uint4(0u, 0u, 0u, 0u);
-    }
-    uint32_t eNode[kTopPer];
-    Box eBox[kTopPer];
-#pragma unroll
-    for (int q = 0; q < kTopPer; q++) {
-        const uint32_t i = i0 + (uint32_t)q;
-        eNode[q] = rec[q][0].x;
-        eBox[q].lo = mk(__uint_as_float(rec[q][1].x), __uint_as_float(rec[q][1].y), __uint_as_float(rec[q][1].z));
-        eBox[q].hi = mk(__uint_as_float(rec[q][1].w), __uint_as_float(rec[q][2].x), __uint_as_float(rec[q][2].y));
-        if ((uint32_t)q < per && i < F) {
-            const uint32_t d = rec[q][0].z == kDeltaInf ? kTopNoSplit : rec[q][0].z;
-            s_pk[i] = (uint64_t)rec[q][0].y | ((uint64_t)d << 32);
-            s_ctl[i] = 0u;
-        }
+    for (uint32_t i = (uint32_t)t; i < F; i += kTopThreads) {
+        const uint4* src = reinterpret_cast<const uint4*>(topList + (size_t)i * kTopWords);
+        const uint4 r0 = src[0], r1 = src[1], r2 = src[2];
+        const uint32_t d = r0.z == kDeltaInf ? kTopNoSplit : r0.z;
+        s_pk[i] = (uint64_t)r0.y | ((uint64_t)d << 32);
+        s_nd[i] = r0.x;
+        Box bx;
+        bx.lo = mk(__uint_as_float(r1.x), __uint_as_float(r1.y), __uint_as_float(r1.z));
+        bx.hi = mk(__uint_as_float(r1.w), __uint_as_float(r2.x), __uint_as_float(r2.y));
+        s_box[i] = bx;
     }
     lds_barrier();
+    if (t >= kTopClimbers) return;
 
-    // The climb is a loop of steps that the whole wave runs together (its
-    // condition is a ballot), and no lane blocks inside a step: a second
-    // arriver whose sibling's deposit is not in place yet checks again in the
-    // next step.  A wait loop inside the step would hang when both arrivers
-    // share a wave, since the compiler may run the waiting lanes' side of the
-    // branch before the depositing lanes' side; the ballot, a convergent
-    // operation, keeps the loop from being rewritten into such a wait.  A lane
-    // whose entry stops (first arriver, or root) takes its next entry.
-    const uint32_t nInternals = n - 1;
-    int q = 0;                // the entry being climbed: i0 + q
-    bool active = i0 < F;     // an entry of this lane is climbing
-    bool waiting = false;     // arrived second at split p, the sibling's deposit not yet read
-    bool pendingCtx = false;  // `node` was just completed: its context is written with the next step
-    uint32_t a = i0, b = i0, node = eNode[0], p = 0, pNode = 0;
-    Box box = eBox[0];
-    auto merge = [&](uint32_t w) {  // the sibling's far end (in w) and box: `node` becomes p
-        const uint32_t other = (w & ~kTopReady) >> 8;
-        const Box ob = s_box[p];
-        a = a < other ? a : other;
-        b = b > other ? b : other;
-        box.lo = umin3(box.lo, ob.lo);
-        box.hi = umax3(box.hi, ob.hi);
-        node = pNode;
-        pendingCtx = true;
-        waiting = false;
+    // Entries i0 .. i0 + per - 1 of this lane, per = ceil(F / kTopClimbers):
+    // consecutive entries are often siblings, so a lane's climbs tend to meet.
+    // The loop's condition is a ballot, so the wave runs it together; no step
+    // waits on another lane.  R and L are the words of the splits at the
+    // range's two ends (b, and a - 1), read ahead.
+    constexpr uint64_t kNoSplitWord = (uint64_t)kTopNoSplit << 32;
+    auto split_word = [&](uint32_t j) {
+        return __hip_atomic_load(&s_pk[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
+    auto delta_of = [](uint64_t w) { return (uint32_t)(w >> 32) & 0x1FFu; };
+    const uint32_t nInternals = n - 1;
+    const uint32_t per = (F + kTopClimbers - 1) / kTopClimbers;
+    const uint32_t i0 = (uint32_t)t * per;
+    uint32_t q = 0;           // the entry being climbed: i0 + q
+    bool active = i0 < F;     // an entry of this lane is climbing
+    bool pending = false;     // `node` was just completed: it is written with the next step
+    uint32_t a = i0, b = i0, node = 0, c0 = 0, c1 = 0;
+    Box box = {}, box0 = {}, box1 = {};  // the subtree box; a completed node's child boxes
+    uint64_t R = 0, L = kNoSplitWord;
+    if (active) {
+        node = s_nd[i0];
+        box = s_box[i0];
+        R = split_word(i0);
+        if (i0 > 0) L = split_word(i0 - 1);
+    }
     while (__ballot(active) != 0) {
-        bool progressed = false, done = false;
-        if (active && !waiting) {
-            progressed = true;
-            const uint64_t R = s_pk[b];
-            const uint64_t L = a > 0 ? s_pk[a - 1] : ((uint64_t)kTopNoSplit << 32);
-            const uint32_t myHi = (uint32_t)R;  // the last leaf of [a, b]
-            if (pendingCtx) nodes[node].context = myHi < nInternals ? myHi : 0xFFFFFFFFu;
-            pendingCtx = false;
-            if (a == 0 && b == F - 1) {
-                *root = node;
-                if (!(node & kLeafBit)) nodes[node].parent = kInvalidNode;
-                done = true;
-            } else {
-                const uint32_t goLeft = (uint32_t)(L >> 32) < (uint32_t)(R >> 32) ? 1u : 0u;  // parent: split a - 1 or b
-                p = goLeft ? a - 1 : b;
-                pNode = goLeft ? (uint32_t)L : myHi;
-                Node* pn = nodes + pNode;
-                pn->child[goLeft] = node;
-                pn->box[goLeft] = box;
-                if (!(node & kLeafBit)) nodes[node].parent = pNode;
-                const uint32_t prior =
-                    __hip_atomic_fetch_add(&s_ctl[p], 1u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if ((prior & 0xFFu) == 0) {  // first: deposit the box and the far end, and stop
-                    s_box[p] = box;
-                    const uint32_t far = goLeft ? b : a;
-                    __hip_atomic_fetch_or(&s_ctl[p], kTopReady | (far << 8), __ATOMIC_RELEASE,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-                    done = true;
-                } else if (prior & kTopReady) {  // second, and the deposit is in place
-                    merge(prior);
-                } else {
-                    waiting = true;
-                }
-            }
+        if (!active) continue;
+        const uint32_t myHi = (uint32_t)R;  // the last leaf of [a, b]
+        const bool isRoot = a == 0 && b == F - 1;
+        const uint32_t goLeft = delta_of(L) < delta_of(R) ? 1u : 0u;  // parent: split a - 1 or b
+        const uint32_t pNode = isRoot ? kInvalidNode : goLeft ? (uint32_t)L : myHi;
+        if (pending) {
+            const uint32_t context = myHi < nInternals ? myHi : 0xFFFFFFFFu;
+            uint4* dst = reinterpret_cast<uint4*>(nodes + node);
+            dst[0] = make_uint4(c0, c1, __float_as_uint(box0.lo.x), __float_as_uint(box0.lo.y));
+            dst[1] = make_uint4(__float_as_uint(box0.lo.z), __float_as_uint(box0.hi.x), __float_as_uint(box0.hi.y),
+                                __float_as_uint(box0.hi.z));
+            dst[2] = make_uint4(__float_as_uint(box1.lo.x), __float_as_uint(box1.lo.y), __float_as_uint(box1.lo.z),
+                                __float_as_uint(box1.hi.x));
+            dst[3] = make_uint4(__float_as_uint(box1.hi.y), __float_as_uint(box1.hi.z), pNode, context);
+        } else if (!(node & kLeafBit)) {
+            nodes[node].parent = pNode;
         }
-        if (active && waiting) {
-            const uint32_t w = __hip_atomic_load(&s_ctl[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (w & kTopReady) {
-                merge(w);
-                progressed = true;
+        pending = false;
+        bool done = true;
+        if (isRoot) {
+            *root = node;
+        } else {
+            const uint32_t p = goLeft ? a - 1 : b;
+            if (a != b) {  // an entry is in its slot already
+                const uint32_t slot = goLeft ? a : b;
+                s_box[slot] = box;
+                s_nd[slot] = node;
+            }
+            const uint32_t far = goLeft ? b : a;
+            const uint64_t prior = __hip_atomic_fetch_add(&s_pk[p], (uint64_t)(1u | (far << 1)) << kTopArrive,
+                                                          __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t arr = (uint32_t)(prior >> kTopArrive);
+            if (arr & 1u) {  // second: the sibling's far end, box and index are in place
+                const uint32_t other = arr >> 1;
+                const uint32_t sib = goLeft ? a - 1 : b + 1;
+                const Box ob = s_box[sib];
+                const uint32_t on = s_nd[sib];
+                c0 = goLeft ? on : node;
+                c1 = goLeft ? node : on;
+                box0 = goLeft ? ob : box;
+                box1 = goLeft ? box : ob;
+                a = a < other ? a : other;
+                b = b > other ? b : other;
+                R = split_word(b);
+                L = a > 0 ? split_word(a - 1) : kNoSplitWord;
+                box.lo = umin3(box.lo, ob.lo);
+                box.hi = umax3(box.hi, ob.hi);
+                node = pNode;
+                pending = true;
+                done = false;
             }
         }
         if (done) {  // the next entry of this lane, if any
             q++;
-            const uint32_t i = i0 + (uint32_t)q;
-            active = (uint32_t)q < per && i < F;
-            a = b = i;
-#pragma unroll
-            for (int k = 1; k < kTopPer; k++)
-                if (q == k) {
-                    node = eNode[k];
-                    box = eBox[k];
-                }
+            const uint32_t i = i0 + q;
+            active = q < per && i < F;
+            if (active) {
+                a = b = i;
+                node = s_nd[i];
+                box = s_box[i];
+                R = split_word(i);
+                L = split_word(i - 1);  // i > i0 >= 0
+            }
         }
-        if (__ballot(progressed) == 0) __builtin_amdgcn_s_sleep(1);
     }
 }
 
