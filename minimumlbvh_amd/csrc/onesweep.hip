@@ -50,6 +50,29 @@ __device__ inline uint64_t lb_pack(uint32_t epoch, uint64_t flag, uint32_t value
 __device__ inline uint32_t digit_of(uint32_t k, int shift) { return (k >> shift) & 0xFFu; }
 __device__ inline uint32_t digit_of(uint64_t k, int shift) { return (uint32_t)(k >> shift) & 0xFFu; }
 
+// The lanes of the wave (among those with `valid`) whose 8-bit digit equals
+// this lane's, as the two 32-bit halves of a lane mask: for each bit, the
+// ballot of the lanes that have it set, or its complement where this lane has
+// it clear.  nb is this lane's bit inverted and sign-extended (0 when set, ~0
+// when clear), so an XOR picks the ballot or its complement; the XOR and the
+// AND are one three-input bit operation per half (kAndXor: its truth table for
+// x & (y ^ z), from the operand patterns 0xF0, 0xCC and 0xAA).
+constexpr unsigned kAndXor = 0xF0u & (0xCCu ^ 0xAAu);
+__device__ inline void digit_peers(uint32_t d, bool valid, uint32_t& lo, uint32_t& hi)
+{
+    const uint64_t v = __ballot(valid);
+    lo = (uint32_t)v;
+    hi = (uint32_t)(v >> 32);
+    const uint32_t nd = ~d;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const uint32_t nb = (uint32_t)__builtin_amdgcn_sbfe((int)nd, (unsigned)b, 1u);
+        const uint64_t m = __ballot(nb == 0u);
+        lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)m, nb, kAndXor);
+        hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(m >> 32), nb, kAndXor);
+    }
+}
+
 // Exclusive scan of one value per thread over threads 0..255 (threads past
 // 255 pass 0 and get an unspecified result); every thread of the block calls.
 __device__ inline uint32_t scan256(uint32_t v, uint32_t* s_tmp)
@@ -292,22 +315,16 @@ __global__ void __launch_bounds__(kBlock)
     }
 
     // Wave64 multisplit: lanes whose 8-bit digits agree are found with eight
-    // ballots; a lane's rank is the number of peers below it.
-    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // ballots (digit_peers); a lane's rank is the number of peers below it.
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
         uint64_t idx = waveBase + (uint64_t)k * 64 + lane;
         bool valid = idx < n;
         uint32_t d = digit_of(key[k], shift);
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            bool bit = (d >> b) & 1u;
-            uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        uint32_t rank = (uint32_t)__popcll(peers & ltMask);
-        uint32_t cnt = (uint32_t)__popcll(peers);
+        uint32_t plo, phi;
+        digit_peers(d, valid, plo, phi);
+        uint32_t rank = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+        uint32_t cnt = (uint32_t)(__popc(plo) + __popc(phi));
         uint32_t prior = valid ? s_wcnt[wave][d] : 0u;
         off[k] = prior + rank;
         if (valid && rank == 0) s_wcnt[wave][d] = prior + cnt;
