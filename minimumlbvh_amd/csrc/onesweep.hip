@@ -39,6 +39,10 @@ struct SortCfg {
 constexpr SortCfg kSortCfgs[kNumSortCfgs] = {{256, 16}, {256, 32}, {512, 16}, {512, 24}, {1024, 16}};
 constexpr int kHistBlock = 256;
 
+#ifndef MLBVH_LOOKBACK_WINDOW
+#define MLBVH_LOOKBACK_WINDOW 4
+#endif
+constexpr int kLookWindow = MLBVH_LOOKBACK_WINDOW;  // predecessors read per look-back round
 constexpr uint64_t kFlagA = 1;  // tile aggregate published
 constexpr uint64_t kFlagP = 2;  // inclusive prefix published
 
@@ -352,18 +356,37 @@ __global__ void __launch_bounds__(kBlock)
         if (preScanned) {  // k_scan_reduce has published every tile's inclusive prefix
             if (tile != 0) excl = (uint32_t)status[(uint64_t)t * gridDim.x + tile - 1];
         } else if (tile != 0) {  // the aggregate (tot) went out before the ranking
+            // kLookWindow predecessors per round, loaded together: the words
+            // are summed from the nearest up to the first published prefix;
+            // an unpublished word ends the round, which resumes there.  The
+            // walk is what costs: at 2^25 keys a tile's predecessors have
+            // nearly always published their aggregates by the time it looks
+            // back, and the nearest prefix is 18-28 tiles back, so a window
+            // divides the dependent round trips (one word per round: 8 us per
+            // tile; 4 or 8 words: 6 us, the rounds slower under the added
+            // traffic; 16 words: slower overall).
             int64_t j = (int64_t)tile - 1;
             while (true) {
-                const uint64_t* src = status + (uint64_t)j * 256 + t;
-                uint64_t w;
-                while (true) {
-                    w = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if ((uint32_t)(w >> 34) == epoch && ((w >> 32) & 3u) != 0) break;
-                    __builtin_amdgcn_s_sleep(1);
+                uint64_t w[kLookWindow];
+#pragma unroll
+                for (int q = 0; q < kLookWindow; q++)
+                    w[q] = j - q >= 0 ? __hip_atomic_load(status + (uint64_t)(j - q) * 256 + t, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : lb_pack(epoch, kFlagP, 0u);
+                int used = 0;
+                bool done = false;
+#pragma unroll
+                for (int q = 0; q < kLookWindow; q++) {
+                    const bool pub = (uint32_t)(w[q] >> 34) == epoch && ((w[q] >> 32) & 3u) != 0;
+                    if (!done && used == q && pub) {
+                        excl += (uint32_t)w[q];
+                        used = q + 1;
+                        done = ((w[q] >> 32) & 3u) == kFlagP;
+                    }
                 }
-                excl += (uint32_t)w;
-                if (((w >> 32) & 3u) == kFlagP) break;
-                j--;
+                if (done) break;
+                j -= used;
+                if (used < kLookWindow) __builtin_amdgcn_s_sleep(1);  // word j is not published yet
             }
             __hip_atomic_store(my, lb_pack(epoch, kFlagP, excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
