@@ -15,8 +15,8 @@
 //                  decoupled look-back over a [tile][digit] status table,
 //                  LDS-staged local scatter, writes of contiguous digit runs
 //
-// Tiles are handed out by an atomic ticket, so a block only waits on tiles
-// that are already running; look-back words carry a pass tag, so the status
+// Tiles of the look-back passes are handed out by an atomic ticket, so a
+// block only waits on tiles that are already running; look-back words carry a pass tag, so the status
 // table is never cleared between passes.  The tile shape (threads x keys per
 // thread) is one of kSortCfgs, chosen per context (mlbvh_set_tuning).
 //
@@ -280,11 +280,14 @@ __global__ void __launch_bounds__(kBlock)
     uint32_t* ticket = ctrl + kCtrlTickets + pass;
     const uint32_t epoch = ctrl[kCtrlEpochs + pass];
 
-    if (t == 0) s_tile = atomicAdd(ticket, 1u);
+    // A pre-scanned pass never waits on another tile, so its tiles need no
+    // ticket (whose device-scope atomic every tile would wait for before its
+    // loads): tile = block index.
+    if (t == 0 && !preScanned) s_tile = atomicAdd(ticket, 1u);
     for (int i = t; i < kWaves * 256; i += kBlock) (&s_wcnt[0][0])[i] = 0;
     if (t < 256) s_hist[t] = 0;
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = preScanned ? blockIdx.x : s_tile;
     const uint64_t tileBase = (uint64_t)tile * kTile;
     const uint64_t waveBase = tileBase + (uint64_t)wave * 64 * kItems;
 
