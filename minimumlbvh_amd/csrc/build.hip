@@ -1110,8 +1110,12 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         if ((rc = ensure(ctx, &ctx->idxA, &ctx->idxCap, nn, "hipMalloc(indices)"))) return rc;
         if ((rc = ensure(ctx, &ctx->idxB, &ctx->idxBCap, nn, "hipMalloc(indices tmp)"))) return rc;
         k_morton64<<<gMorton, kBuildBlock, 0, stream>>>(tris, n, ctx->sceneBox, ctx->recB, ctx->idxA);
-        // keys in recB (tmp: recA), records written to recA afterwards
-        rc = sort_pairs_u64(ctx, ctx->recB, ctx->idxA, ctx->recA, ctx->idxB, (uint32_t)n, 0, 63, stream);
+        // keys in recB (tmp: recA), records written to recA afterwards.  Large
+        // sorts take the route of four top-digit passes and the segment
+        // finish: on a surface mesh the 63-bit codes' runs of equal 24-bit
+        // prefix are often longer than the finish handles (then the three-pass
+        // route re-ran every pass), their 32-bit prefixes far less often.
+        rc = sort_pairs_u64(ctx, ctx->recB, ctx->idxA, ctx->recA, ctx->idxB, (uint32_t)n, 0, 63, stream, 4);
         if (rc) return rc;
         k_deltas64<<<g, kBuildBlock, 0, stream>>>(ctx->recB, ctx->idxA, n, ctx->deltas, ctx->recA);
     } else {

@@ -439,8 +439,9 @@ __global__ void __launch_bounds__(kBlock)
 
 
 // ---------------------------------------------------------------------------
-// Hybrid finish: after three top-digit passes the array is stably ordered by
-// a 24-bit prefix, and for large uniform inputs the runs of equal prefix
+// Hybrid finish: after the top-digit passes (three, or four) the array is
+// stably ordered by a 24-bit (32-bit) prefix, and for large uniform inputs
+// (or spatially spread 63-bit Morton codes) the runs of equal prefix
 // ("segments") hold a few keys each.  Every key finds its segment by looking
 // at its neighbours' prefixes and its place in it by counting the segment's
 // keys that order before it (sort bits compared, ties by position): the
@@ -458,16 +459,18 @@ constexpr int kFinishHalo = 64;                        // longest segment handle
 constexpr int kFinishNear = 8;                         // segments up to 9 keys take the fixed-count path
 constexpr uint32_t kHybridMinKeys = 1u << 18;
 
+// The prefix of the top-digit passes: bits [shift, shift + 8 * top digits),
+// as masked by prefixMask.
 template <class K>
-__device__ inline uint32_t prefix24(K k, int shift)
+__device__ inline uint32_t prefix_of(K k, int shift, uint32_t prefixMask)
 {
-    return shift < (int)(8 * sizeof(K)) ? (uint32_t)(k >> shift) & 0xFFFFFFu : 0u;
+    return shift < (int)(8 * sizeof(K)) ? (uint32_t)(k >> shift) & prefixMask : 0u;
 }
 
-// The order of a key among keys sharing its 24-bit prefix: its sort bits
-// below the prefix, [startBit, prefixShift), moved down to bit 0 (at most
-// 8 * (8 - 3) = 40 bits), above its window position (12 bits).  The word then
-// stays below bit 52, where the segment start goes.
+// The order of a key among keys sharing its prefix: its sort bits below the
+// prefix, [startBit, prefixShift), moved down to bit 0 (at most 8 * (8 - 3) =
+// 40 bits), above its window position (12 bits).  The word then stays below
+// bit 52, where the segment start goes.
 template <class K>
 __device__ inline uint64_t order_word(K k, K lowMask, int startBit, int pos)
 {
@@ -491,8 +494,8 @@ constexpr uint64_t kFinishFar = 0xFFFull;
 template <class K, bool kVals>
 __global__ void __launch_bounds__(kFinishBlock)
     k_segment_finish(const K* __restrict__ src, K* __restrict__ dst, const uint32_t* __restrict__ vsrc,
-                     uint32_t* __restrict__ vdst, uint32_t n, int startBit, int prefixShift, K sortMask,
-                     uint32_t* ctrl)
+                     uint32_t* __restrict__ vdst, uint32_t n, int startBit, int prefixShift, uint32_t prefixMask,
+                     K sortMask, uint32_t* ctrl)
 {
     constexpr int kWin = kFinishChunk + 2 * kFinishHalo;
     constexpr int kRows = kWin / 64;
@@ -524,7 +527,7 @@ __global__ void __launch_bounds__(kFinishBlock)
 #pragma unroll
     for (int u = 0; u < kLoads; u++) {
         const int i = t + u * kFinishBlock;
-        if (i < kWin) s_pf[i] = prefix24(ld[u], prefixShift);
+        if (i < kWin) s_pf[i] = prefix_of(ld[u], prefixShift, prefixMask);
     }
     __syncthreads();
     const int wLo = (int)((lo < 0 ? 0 : lo) - lo);                                   // first valid window index
@@ -613,13 +616,15 @@ __global__ void __launch_bounds__(kFinishBlock)
     if (overflow) __hip_atomic_store(&ctrl[kCtrlSortFallback], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// dst = src when the fallback ran an odd number of passes (device-side test).
-template <class K>
+// dst = src when the guard word is set (kWhenSet) or clear (device-side
+// test): the fallback's result when it ran an odd number of passes, or the
+// finish's when it is not in the caller's buffers and the fallback did not run.
+template <class K, bool kWhenSet = true>
 __global__ void __launch_bounds__(256) k_copy_guarded(const K* __restrict__ src, K* __restrict__ dst,
                                                       const uint32_t* __restrict__ vsrc, uint32_t* __restrict__ vdst,
                                                       uint32_t n, const uint32_t* __restrict__ guard)
 {
-    if (*guard == 0) return;
+    if ((*guard != 0) != kWhenSet) return;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         dst[i] = src[i];
         if (vsrc) vdst[i] = vsrc[i];
@@ -683,35 +688,40 @@ static void dispatch_pass(int cfg, hipStream_t s, uint32_t tiles, const K* src, 
     }
 }
 
-// Three top-digit passes (keys -> tmp -> keys -> tmp), the segment finish
-// (tmp -> keys), then the full pass sequence from tmp guarded by the
-// fallback flag (launched always, returning at once when not needed).
+// `top` top-digit passes (3 or 4; keys -> tmp -> ...), the segment finish
+// from the top-pass state into the other buffer, then the full pass
+// sequence from that state, guarded by the fallback flag (launched always,
+// returning at once when not needed).  The result ends in the caller's
+// buffers: with 4 top passes the state is in them, so the finish writes the
+// temporaries and a copy guarded on the flag being clear brings it back, and
+// the fallback (8 passes, or an even number) ends where it started.
 template <class K>
 static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, uint32_t n, int startBit,
-                       int live, uint32_t tiles, hipStream_t stream)
+                       int live, uint32_t tiles, hipStream_t stream, int top)
 {
     int rc;
     const int keyBits = 8 * (int)sizeof(K);
     const int hg = hist_grid(ctx, n);
-    // partials: hg blocks x 3 histograms, then the first pass's per-tile
+    // partials: hg blocks x `top` histograms, then the first pass's per-tile
     // counts (the fallback's histograms later reuse the space)
-    const size_t partialRows = (size_t)hg * 3 + tiles > (size_t)hg * live ? (size_t)hg * 3 + tiles : (size_t)hg * live;
+    const size_t topRows = (size_t)hg * top + tiles;
+    const size_t partialRows = topRows > (size_t)hg * live ? topRows : (size_t)hg * live;
     if ((rc = ensure_partials(ctx, (int)partialRows, 1))) return rc;
-    uint32_t* tileCounts = ctx->partials + (size_t)hg * 3 * 256;
-    // pass slots: 0..2 = the three top digits; 3 .. 3+live-1 = the fallback
-    // sequence (set up by k_hist's block 0).  The three top digits'
+    uint32_t* tileCounts = ctx->partials + (size_t)hg * top * 256;
+    // pass slots: 0 .. top-1 = the top digits; top .. top+live-1 = the
+    // fallback sequence (set up by k_hist's block 0).  The top digits'
     // histograms now (the rest only if the fallback runs), with the first
     // pass's per-tile counts: its look-back table is then filled in advance
     // and its tiles never wait on each other
-    const SortInit init{ctx->ctrl, 3 + live, live, ctx->hist, ctx->status, ctx->statusWords};
-    k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - 3), 3, ctx->partials, nullptr,
+    const SortInit init{ctx->ctrl, top + live, live, ctx->hist, ctx->status, ctx->statusWords};
+    k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - top), top, ctx->partials, nullptr,
                                              (uint32_t)sort_tile_keys(ctx), tiles, tileCounts, init);
-    launch_scan_reduce(ctx, tileCounts, tiles, hg, 3, ctx->hist + 256 * (live - 3), stream);
+    launch_scan_reduce(ctx, tileCounts, tiles, hg, top, ctx->hist + 256 * (live - top), stream);
     const uint32_t* guard = ctx->ctrl + kCtrlSortFallback;
     K* bufs[2] = {keys, ktmp};
     uint32_t* vbufs[2] = {vals, vtmp};
-    for (int q = 0; q < 3; q++) {
-        const int p = live - 3 + q;
+    for (int q = 0; q < top; q++) {
+        const int p = live - top + q;
         const int shift = startBit + 8 * p;
         const int pre = q == 0;
         if (vals)
@@ -722,34 +732,42 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
             dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, bufs[q & 1], bufs[(q + 1) & 1], nullptr, nullptr, n,
                                     shift, ctx->hist + 256 * p, ctx->status, ctx->ctrl, q, nullptr, pre);
     }
+    K* state = bufs[top & 1];
+    K* other = bufs[(top + 1) & 1];
+    uint32_t* vstate = vbufs[top & 1];
+    uint32_t* vother = vbufs[(top + 1) & 1];
     // sort key = bits [startBit, min(startBit + 8 * live, keyBits))
     const int topBit = startBit + 8 * live < keyBits ? startBit + 8 * live : keyBits;
     const K high = topBit == keyBits ? ~(K)0 : (((K)1 << topBit) - 1);
     const K sortMask = high & ~(((K)1 << startBit) - 1);
-    const int prefixShift = startBit + 8 * (live - 3);
+    const int prefixShift = startBit + 8 * (live - top);
+    const uint32_t prefixMask = top >= 4 ? 0xFFFFFFFFu : (1u << (8 * top)) - 1u;
     const unsigned fb = (unsigned)((n + kFinishChunk - 1) / kFinishChunk);
     if (vals)
-        k_segment_finish<K, true><<<fb, kFinishBlock, 0, stream>>>(ktmp, keys, vtmp, vals, n, startBit, prefixShift,
-                                                                   sortMask, ctx->ctrl);
+        k_segment_finish<K, true><<<fb, kFinishBlock, 0, stream>>>(state, other, vstate, vother, n, startBit,
+                                                                   prefixShift, prefixMask, sortMask, ctx->ctrl);
     else
-        k_segment_finish<K, false><<<fb, kFinishBlock, 0, stream>>>(ktmp, keys, nullptr, nullptr, n, startBit,
-                                                                    prefixShift, sortMask, ctx->ctrl);
-    // fallback: histograms of the lower digits (of the three-pass state, which
+        k_segment_finish<K, false><<<fb, kFinishBlock, 0, stream>>>(state, other, nullptr, nullptr, n, startBit,
+                                                                    prefixShift, prefixMask, sortMask, ctx->ctrl);
+    const unsigned gc = (unsigned)(((size_t)n + 255) / 256), capc = (unsigned)ctx->cuCount * 8;
+    if (other != keys)  // the finish's result, unless the fallback runs (which keeps the state intact)
+        k_copy_guarded<K, false><<<gc < capc ? gc : capc, 256, 0, stream>>>(other, keys, vother, vals, n, guard);
+    // fallback: histograms of the lower digits (of the top-pass state, which
     // the finish does not modify), then every pass from that state
-    k_hist<K><<<hg, kHistBlock, 0, stream>>>(ktmp, n, startBit, live - 3, ctx->partials, guard);
-    launch_hist_reduce(ctx, hg, live - 3, stream, 0, guard);
-    K* src = ktmp;
-    K* dst = keys;
-    uint32_t* vsrc = vtmp;
-    uint32_t* vdst = vals;
+    k_hist<K><<<hg, kHistBlock, 0, stream>>>(state, n, startBit, live - top, ctx->partials, guard);
+    launch_hist_reduce(ctx, hg, live - top, stream, 0, guard);
+    K* src = state;
+    K* dst = other;
+    uint32_t* vsrc = vstate;
+    uint32_t* vdst = vother;
     for (int p = 0; p < live; p++) {
         const int shift = startBit + 8 * p;
         if (vals)
             dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, src, dst, vsrc, vdst, n, shift, ctx->hist + 256 * p,
-                                   ctx->status, ctx->ctrl, 3 + p, guard);
+                                   ctx->status, ctx->ctrl, top + p, guard);
         else
             dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, src, dst, nullptr, nullptr, n, shift,
-                                    ctx->hist + 256 * p, ctx->status, ctx->ctrl, 3 + p, guard);
+                                    ctx->hist + 256 * p, ctx->status, ctx->ctrl, top + p, guard);
         K* tk = src;
         src = dst;
         dst = tk;
@@ -757,11 +775,8 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
         vsrc = vdst;
         vdst = tv;
     }
-    if (src != keys) {
-        unsigned g = (unsigned)(((size_t)n + 255) / 256);
-        unsigned cap = (unsigned)ctx->cuCount * 8;
-        k_copy_guarded<K><<<g < cap ? g : cap, 256, 0, stream>>>(src, keys, vsrc, vals, n, guard);
-    }
+    if (src != keys)
+        k_copy_guarded<K, true><<<gc < capc ? gc : capc, 256, 0, stream>>>(src, keys, vsrc, vals, n, guard);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? MLBVH_OK : hip_status(e, "sort kernel launch");
 }
@@ -769,7 +784,7 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
 template <class K>
 static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, uint32_t n, int startBit,
                      int endBit, bool histReady, hipStream_t stream, const uint32_t* readyTileCounts = nullptr,
-                     int readyHistBlocks = 0)
+                     int readyHistBlocks = 0, int hybridTop = 3)
 {
     const int keyBits = 8 * (int)sizeof(K);
     if (startBit < 0 || endBit <= startBit) {
@@ -794,8 +809,8 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
     uint32_t tiles = sort_tiles(ctx, n);
     if ((rc = ensure_status(ctx, tiles))) return rc;
 
-    if (!histReady && n >= kHybridMinKeys && live >= 4) return sort_hybrid(ctx, keys, vals, ktmp, vtmp, n, startBit, live,
-                                                                             tiles, stream);
+    if (!histReady && n >= kHybridMinKeys && live > hybridTop)
+        return sort_hybrid(ctx, keys, vals, ktmp, vtmp, n, startBit, live, tiles, stream, hybridTop);
 
     // per-tile counts of the first digit (from the caller's histogram kernel
     // or from k_hist here) let the first pass skip its look-back; partial
@@ -854,9 +869,13 @@ int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, 
 }
 
 int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,
-                   int startBit, int endBit, hipStream_t stream)
+                   int startBit, int endBit, hipStream_t stream, int hybridTop)
 {
-    return sort_impl<uint64_t>(ctx, keys, vals, ktmp, vtmp, n, startBit, endBit, false, stream);
+    if (hybridTop < 3 || hybridTop > 4) {
+        set_error("sort: %d top-digit passes (3 or 4)", hybridTop);
+        return MLBVH_E_INVALID_ARG;
+    }
+    return sort_impl<uint64_t>(ctx, keys, vals, ktmp, vtmp, n, startBit, endBit, false, stream, nullptr, 0, hybridTop);
 }
 
 void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream, int firstHist,
