@@ -76,6 +76,10 @@ constexpr int kCtrlCompactTag = 56;     // the build's look-back words carry thi
 constexpr int kCtrlL2Ticket = 57;       // next block of the build's second block-local level
 constexpr int kCtrlTopCount = 58;       // entries of the top-tree list (level 2's frontier)
 constexpr int kCtrlBuildTag = 59;       // the scene box slots carry this + 1 (advanced for the next build)
+// Device-wide barriers of the sort's one-launch fallback (onesweep.hip, grid_barrier):
+constexpr int kCtrlGridArrive = 60;   // arrivals, only ever incremented
+constexpr int kCtrlGridBase = 61;     // kCtrlGridArrive's value when the next such launch starts
+constexpr int kCtrlGridTimeout = 62;  // set when a barrier wait gave up (blocks not all resident)
 constexpr int kCtrlWords = 64;
 // The build's top-tree list (mlbvh_ctx::topList): at most kTopMax records of
 // kTopWords words, one per level-2 frontier entry, in leaf order: node, last

@@ -617,8 +617,8 @@ __global__ void __launch_bounds__(kFinishBlock)
 }
 
 // dst = src when the guard word is set (kWhenSet) or clear (device-side
-// test): the fallback's result when it ran an odd number of passes, or the
-// finish's when it is not in the caller's buffers and the fallback did not run.
+// test): the finish's result, when it is not in the caller's buffers and the
+// fallback did not run.
 template <class K, bool kWhenSet = true>
 __global__ void __launch_bounds__(256) k_copy_guarded(const K* __restrict__ src, K* __restrict__ dst,
                                                       const uint32_t* __restrict__ vsrc, uint32_t* __restrict__ vdst,
@@ -629,6 +629,185 @@ __global__ void __launch_bounds__(256) k_copy_guarded(const K* __restrict__ src,
         dst[i] = src[i];
         if (vsrc) vdst[i] = vsrc[i];
     }
+}
+
+// ---------------------------------------------------------------------------
+// The hybrid route's fallback in one launch.  When the finish found a run of
+// equal prefix longer than its halo, every digit pass of the LSD sequence
+// runs from the top-pass state here, inside one kernel: one 1024-thread
+// block per CU (all resident), each a contiguous chunk of the keys, and three
+// device-wide barriers per pass (chunk counts; per-digit scans over the
+// blocks; the stable scatter of each chunk in 1024-key steps, ranked as in
+// k_pass).  With the flag clear it returns at once: one launch of about 4 us
+// in place of the eleven guarded ones (histograms, their reduction, every
+// pass and a copy) that each cost as much while returning at once.
+// ---------------------------------------------------------------------------
+constexpr int kFbThreads = 1024;
+constexpr int kFbWaves = kFbThreads / 64;
+constexpr int kFbMaxBlocks = kFbThreads;  // the per-digit scan has one thread per block
+constexpr uint32_t kGridPollLimit = 1u << 22;
+
+// Device-wide barrier of a launch whose blocks are all resident (one per
+// CU).  ctrl[kCtrlGridArrive] only grows: every block adds 1 per barrier and
+// waits until it reaches base + k * gridDim.x at its k-th barrier, base being
+// ctrl[kCtrlGridBase] as read at the start of the launch (block 0 stores the
+// final value there at the end, after every block has read it).  Every
+// wave's stores have completed at the block barrier; thread 0 then releases
+// at agent scope (L2 written back) before it arrives and acquires (caches
+// invalidated) after the wait.  The wait gives up after 2^22 polls
+// (seconds) and sets ctrl[kCtrlGridTimeout], so a launch whose blocks were
+// not all resident ends (with wrong results) instead of hanging.
+__device__ inline void grid_barrier(uint32_t* ctrl, uint32_t target)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(ctrl + kCtrlGridArrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t polls = 0;
+        while ((int32_t)(__hip_atomic_load(ctrl + kCtrlGridArrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                         target) < 0) {
+            if (++polls == kGridPollLimit) {
+                __hip_atomic_store(ctrl + kCtrlGridTimeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+// Exclusive scan over the 1024 threads of the block (all of them call).
+__device__ inline uint32_t scan1024(uint32_t v, uint32_t* s_tmp /* kFbWaves + 1 words */, uint32_t* total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kFbWaves; w++) {
+        const uint32_t c = s_tmp[w];
+        before += w < wave ? c : 0u;
+        all += c;
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// counts: (2 * gridDim.x + 1) rows of 256 words: the blocks' chunk counts,
+// their per-digit exclusive prefixes over the blocks, the digit totals.
+template <class K, bool kVals>
+__global__ void __launch_bounds__(kFbThreads)
+    k_fallback_sort(K* state, K* other, uint32_t* vstate, uint32_t* vother, K* keys, uint32_t* vals, uint32_t n,
+                    int startBit, int live, uint32_t* counts, uint32_t* ctrl)
+{
+    if (ctrl[kCtrlSortFallback] == 0) return;  // the finish placed every key
+    __shared__ uint32_t s_cnt[256];
+    __shared__ uint32_t s_off[256];
+    __shared__ uint32_t s_wcnt[kFbWaves][256];
+    __shared__ uint32_t s_tmp[kFbWaves + 1];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t G = gridDim.x, blk = blockIdx.x;
+    const uint32_t base =
+        t == 0 ? __hip_atomic_load(ctrl + kCtrlGridBase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    uint32_t barriers = 0;
+    const uint32_t chunk = (n + G - 1) / G;
+    const uint32_t lo = blk * chunk < n ? blk * chunk : n;
+    const uint32_t hi = lo + chunk < n ? lo + chunk : n;
+    const int keyBits = 8 * (int)sizeof(K);
+    K* src = state;
+    K* dst = other;
+    uint32_t* vsrc = vstate;
+    uint32_t* vdst = vother;
+    uint32_t* rowPrefix = counts + (size_t)G * 256;
+    uint32_t* rowTotal = counts + (size_t)2 * G * 256;
+    for (int p = 0; p < live; p++) {
+        const int shift = startBit + 8 * p;
+        auto dig = [&](K k) -> uint32_t { return shift < keyBits ? digit_of(k, shift) : 0u; };
+        // 1. this block's counts
+        if (t < 256) s_cnt[t] = 0;
+        __syncthreads();
+        for (uint32_t i = lo + (uint32_t)t; i < hi; i += kFbThreads) atomicAdd(&s_cnt[dig(src[i])], 1u);
+        __syncthreads();
+        if (t < 256) counts[(size_t)blk * 256 + t] = s_cnt[t];
+        grid_barrier(ctrl, base + (++barriers) * G);
+        // 2. per digit: exclusive prefix over the blocks, and the total
+        for (uint32_t d = blk; d < 256; d += G) {
+            const uint32_t v = (uint32_t)t < G ? counts[(size_t)t * 256 + d] : 0u;
+            uint32_t all;
+            const uint32_t ex = scan1024(v, s_tmp, &all);
+            if ((uint32_t)t < G) rowPrefix[(size_t)t * 256 + d] = ex;
+            if (t == 0) rowTotal[d] = all;
+        }
+        grid_barrier(ctrl, base + (++barriers) * G);
+        // 3. the block's first position for each digit, then its chunk in order
+        {
+            uint32_t all;
+            const uint32_t tot = t < 256 ? rowTotal[t] : 0u;
+            const uint32_t start = scan1024(tot, s_tmp, &all);
+            if (t < 256) s_off[t] = start + rowPrefix[(size_t)blk * 256 + t];
+        }
+        for (int w = wave; w < kFbWaves; w += kFbWaves)
+            for (int d = lane; d < 256; d += 64) s_wcnt[w][d] = 0;
+        __syncthreads();
+        for (uint32_t b0 = lo; b0 < hi; b0 += kFbThreads) {
+            const uint32_t i = b0 + (uint32_t)t;
+            const bool valid = i < hi;
+            const K key = valid ? src[i] : (K)0;
+            const uint32_t val = kVals && valid ? vsrc[i] : 0u;
+            const uint32_t d = dig(key);
+            uint32_t plo, phi;
+            digit_peers(d, valid, plo, phi);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+            const uint32_t cnt = (uint32_t)(__popc(plo) + __popc(phi));
+            if (valid && rank == 0) s_wcnt[wave][d] = cnt;
+            __syncthreads();
+            uint32_t tot = 0;
+            if (t < 256) {  // digit t: the waves' offsets, and the step's total
+#pragma unroll
+                for (int w = 0; w < kFbWaves; w++) {
+                    const uint32_t c = s_wcnt[w][t];
+                    s_wcnt[w][t] = tot;
+                    tot += c;
+                }
+            }
+            __syncthreads();
+            if (valid) {
+                const uint32_t pos = s_off[d] + s_wcnt[wave][d] + rank;
+                dst[pos] = key;
+                if (kVals) vdst[pos] = val;
+            }
+            __syncthreads();
+            if (t < 256) {
+                s_off[t] += tot;
+#pragma unroll
+                for (int w = 0; w < kFbWaves; w++) s_wcnt[w][t] = 0;
+            }
+            __syncthreads();
+        }
+        grid_barrier(ctrl, base + (++barriers) * G);
+        K* tk = src;
+        src = dst;
+        dst = tk;
+        uint32_t* tv = vsrc;
+        vsrc = vdst;
+        vdst = tv;
+    }
+    if (src != keys) {  // the result into the caller's buffers
+        for (size_t i = (size_t)blk * kFbThreads + t; i < n; i += (size_t)G * kFbThreads) {
+            keys[i] = src[i];
+            if (kVals) vals[i] = vsrc[i];
+        }
+    }
+    if (blk == 0 && t == 0)
+        __hip_atomic_store(ctrl + kCtrlGridBase, base + barriers * G, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 static int hist_grid(mlbvh_ctx* ctx, uint32_t n)
@@ -690,11 +869,11 @@ static void dispatch_pass(int cfg, hipStream_t s, uint32_t tiles, const K* src, 
 
 // `top` top-digit passes (3 or 4; keys -> tmp -> ...), the segment finish
 // from the top-pass state into the other buffer, then the full pass
-// sequence from that state, guarded by the fallback flag (launched always,
-// returning at once when not needed).  The result ends in the caller's
-// buffers: with 4 top passes the state is in them, so the finish writes the
-// temporaries and a copy guarded on the flag being clear brings it back, and
-// the fallback (8 passes, or an even number) ends where it started.
+// sequence from that state in k_fallback_sort (launched always, returning at
+// once when the finish did not set the fallback flag).  The result ends in
+// the caller's buffers: with 4 top passes the state is in them, so the
+// finish writes the temporaries and a copy guarded on the flag being clear
+// brings it back; the fallback copies its result there itself.
 template <class K>
 static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, uint32_t n, int startBit,
                        int live, uint32_t tiles, hipStream_t stream, int top)
@@ -703,9 +882,10 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
     const int keyBits = 8 * (int)sizeof(K);
     const int hg = hist_grid(ctx, n);
     // partials: hg blocks x `top` histograms, then the first pass's per-tile
-    // counts (the fallback's histograms later reuse the space)
+    // counts (the fallback launch later reuses them: 2 rows per block and one of totals)
     const size_t topRows = (size_t)hg * top + tiles;
-    const size_t partialRows = topRows > (size_t)hg * live ? topRows : (size_t)hg * live;
+    const size_t fbRows = 2 * (size_t)(ctx->cuCount < kFbMaxBlocks ? ctx->cuCount : kFbMaxBlocks) + 1;
+    const size_t partialRows = topRows > fbRows ? topRows : fbRows;
     if ((rc = ensure_partials(ctx, (int)partialRows, 1))) return rc;
     uint32_t* tileCounts = ctx->partials + (size_t)hg * top * 256;
     // pass slots: 0 .. top-1 = the top digits; top .. top+live-1 = the
@@ -752,31 +932,15 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
     const unsigned gc = (unsigned)(((size_t)n + 255) / 256), capc = (unsigned)ctx->cuCount * 8;
     if (other != keys)  // the finish's result, unless the fallback runs (which keeps the state intact)
         k_copy_guarded<K, false><<<gc < capc ? gc : capc, 256, 0, stream>>>(other, keys, vother, vals, n, guard);
-    // fallback: histograms of the lower digits (of the top-pass state, which
-    // the finish does not modify), then every pass from that state
-    k_hist<K><<<hg, kHistBlock, 0, stream>>>(state, n, startBit, live - top, ctx->partials, guard);
-    launch_hist_reduce(ctx, hg, live - top, stream, 0, guard);
-    K* src = state;
-    K* dst = other;
-    uint32_t* vsrc = vstate;
-    uint32_t* vdst = vother;
-    for (int p = 0; p < live; p++) {
-        const int shift = startBit + 8 * p;
-        if (vals)
-            dispatch_pass<K, true>(ctx->sortConfig, stream, tiles, src, dst, vsrc, vdst, n, shift, ctx->hist + 256 * p,
-                                   ctx->status, ctx->ctrl, top + p, guard);
-        else
-            dispatch_pass<K, false>(ctx->sortConfig, stream, tiles, src, dst, nullptr, nullptr, n, shift,
-                                    ctx->hist + 256 * p, ctx->status, ctx->ctrl, top + p, guard);
-        K* tk = src;
-        src = dst;
-        dst = tk;
-        uint32_t* tv = vsrc;
-        vsrc = vdst;
-        vdst = tv;
-    }
-    if (src != keys)
-        k_copy_guarded<K, true><<<gc < capc ? gc : capc, 256, 0, stream>>>(src, keys, vsrc, vals, n, guard);
+    // fallback: every pass from the top-pass state (which the finish does not
+    // modify), in one launch that returns at once unless the finish set the flag
+    const unsigned G = (unsigned)(ctx->cuCount < kFbMaxBlocks ? ctx->cuCount : kFbMaxBlocks);
+    if (vals)
+        k_fallback_sort<K, true><<<G, kFbThreads, 0, stream>>>(state, other, vstate, vother, keys, vals, n, startBit,
+                                                               live, ctx->partials, ctx->ctrl);
+    else
+        k_fallback_sort<K, false><<<G, kFbThreads, 0, stream>>>(state, other, nullptr, nullptr, keys, nullptr, n,
+                                                                startBit, live, ctx->partials, ctx->ctrl);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? MLBVH_OK : hip_status(e, "sort kernel launch");
 }
