@@ -1115,7 +1115,7 @@ static int build_impl(mlbvh_ctx* ctx, TS tris, int n, uint32_t opt, void* d_node
         // finish: on a surface mesh the 63-bit codes' runs of equal 24-bit
         // prefix are often longer than the finish handles (then the three-pass
         // route re-ran every pass), their 32-bit prefixes far less often.
-        rc = sort_pairs_u64(ctx, ctx->recB, ctx->idxA, ctx->recA, ctx->idxB, (uint32_t)n, 0, 63, stream, 4);
+        rc = sort_pairs_u64(ctx, ctx->recB, ctx->idxA, ctx->recA, ctx->idxB, (uint32_t)n, 0, 63, stream, 4, true);
         if (rc) return rc;
         k_deltas64<<<g, kBuildBlock, 0, stream>>>(ctx->recB, ctx->idxA, n, ctx->deltas, ctx->recA);
     } else {

@@ -179,9 +179,10 @@ int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, 
 // mlbvh_sort_u64.  hybridTop: the top-digit passes (3 or 4) of the route
 // large sorts take before the segment finish; 4 (a 32-bit prefix) for keys
 // whose 24-bit prefixes repeat often, such as the 63-bit Morton codes of a
-// surface mesh.
+// surface mesh.  sharedDigits: count the histograms with one LDS atomic per
+// wave where its lanes share a digit (Morton codes), not per key.
 int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,
-                   int startBit, int endBit, hipStream_t stream, int hybridTop = 3);
+                   int startBit, int endBit, hipStream_t stream, int hybridTop = 3, bool sharedDigits = false);
 uint32_t sort_tiles(mlbvh_ctx* ctx, uint32_t n);
 int sort_tile_keys(mlbvh_ctx* ctx);
 int ensure_status(mlbvh_ctx* ctx, uint32_t tiles);

@@ -77,6 +77,20 @@ __device__ inline void digit_peers(uint32_t d, bool valid, uint32_t& lo, uint32_
     }
 }
 
+// One LDS histogram add per wave when its active lanes share the digit (as
+// whole runs of equal keys do), else one per lane.
+__device__ inline void lds_count(uint32_t* h, uint32_t d)
+{
+    const uint64_t active = __ballot(1);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__ballot(d == d0) == active) {
+        if ((uint32_t)__lane_id() == (uint32_t)__ffsll((unsigned long long)active) - 1u)
+            atomicAdd(&h[d0], (uint32_t)__popcll(active));
+    } else {
+        atomicAdd(&h[d], 1u);
+    }
+}
+
 // Exclusive scan of one value per thread over threads 0..255 (threads past
 // 255 pass 0 and get an unspecified result); every thread of the block calls.
 __device__ inline uint32_t scan256(uint32_t v, uint32_t* s_tmp)
@@ -111,7 +125,10 @@ struct SortInit {
     uint64_t* status;
     size_t statusWords;
 };
-template <class K>
+// kShared: count through lds_count (keys whose digits are often shared by a
+// whole wave, such as Morton codes of a mesh; on uniform keys the check costs
+// more than it saves).
+template <class K, bool kShared = false>
 __global__ void __launch_bounds__(kHistBlock) k_hist(const K* __restrict__ keys, uint32_t n, int startBit,
                                                      int nPass, uint32_t* __restrict__ partials,
                                                      const uint32_t* __restrict__ guard = nullptr,
@@ -129,11 +146,17 @@ __global__ void __launch_bounds__(kHistBlock) k_hist(const K* __restrict__ keys,
         __shared__ uint32_t s_t[256];  // this tile's counts of the first digit
         s_t[threadIdx.x] = 0;
         __syncthreads();
+        auto add = [&](uint32_t* h, uint32_t d) {
+            if (kShared)
+                lds_count(h, d);
+            else
+                atomicAdd(&h[d], 1u);
+        };
         auto count = [&](K k) {
-            atomicAdd(&s_t[startBit < keyBits ? digit_of(k, startBit) : 0u], 1u);
+            add(s_t, startBit < keyBits ? digit_of(k, startBit) : 0u);
             for (int p = 1; p < nPass; p++) {
                 int sh = startBit + 8 * p;
-                atomicAdd(&s_h[p][sh < keyBits ? digit_of(k, sh) : 0u], 1u);
+                add(s_h[p], sh < keyBits ? digit_of(k, sh) : 0u);
             }
         };
         for (uint32_t tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
@@ -510,6 +533,7 @@ __global__ void __launch_bounds__(kFinishBlock)
     __shared__ uint64_t s_ow[kWin];
     __shared__ int s_rowLast[kRows];   // the row's last segment start, or -1
     __shared__ int s_rowFirst[kRows];  // one past the row's first segment end, or kNone
+    __shared__ uint32_t s_overflow;    // a wave of the block has stored the fallback flag
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t lo = (int64_t)blockIdx.x * kFinishChunk - kFinishHalo;  // window start (may be < 0)
     // all loads in flight before the first LDS store
@@ -529,6 +553,7 @@ __global__ void __launch_bounds__(kFinishBlock)
         const int i = t + u * kFinishBlock;
         if (i < kWin) s_pf[i] = prefix_of(ld[u], prefixShift, prefixMask);
     }
+    if (t == 0) s_overflow = 0;
     __syncthreads();
     const int wLo = (int)((lo < 0 ? 0 : lo) - lo);                                   // first valid window index
     const int wHi = (int)((lo + kWin < (int64_t)n ? lo + kWin : (int64_t)n) - lo);   // one past the last
@@ -613,7 +638,12 @@ __global__ void __launch_bounds__(kFinishBlock)
         dst[lo + a + rank] = ld[u];
         if (kVals) vdst[lo + a + rank] = val[u];
     }
-    if (overflow) __hip_atomic_store(&ctrl[kCtrlSortFallback], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // one store per block, by the first wave that found a long run: with every
+    // key of a long run storing, the stores to the one word queued for
+    // milliseconds, and one per wave still for 2 ms at 2^25 keys (a block-wide
+    // OR instead cost a barrier in every block of the common case)
+    if (__ballot(overflow) != 0 && lane == 0 && atomicExch(&s_overflow, 1u) == 0u)
+        __hip_atomic_store(&ctrl[kCtrlSortFallback], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // dst = src when the guard word is set (kWhenSet) or clear (device-side
@@ -637,7 +667,7 @@ __global__ void __launch_bounds__(256) k_copy_guarded(const K* __restrict__ src,
 // runs from the top-pass state here, inside one kernel: one 1024-thread
 // block per CU (all resident), each a contiguous chunk of the keys, and three
 // device-wide barriers per pass (chunk counts; per-digit scans over the
-// blocks; the stable scatter of each chunk in 1024-key steps, ranked as in
+// blocks; the stable scatter of each chunk in 8192-key steps, ranked as in
 // k_pass).  With the flag clear it returns at once: one launch of about 4 us
 // in place of the eleven guarded ones (histograms, their reduction, every
 // pass and a copy) that each cost as much while returning at once.
@@ -645,6 +675,8 @@ __global__ void __launch_bounds__(256) k_copy_guarded(const K* __restrict__ src,
 constexpr int kFbThreads = 1024;
 constexpr int kFbWaves = kFbThreads / 64;
 constexpr int kFbMaxBlocks = kFbThreads;  // the per-digit scan has one thread per block
+constexpr int kFbItems = 8;                // keys per thread in one scatter step
+constexpr uint32_t kFbStep = kFbThreads * kFbItems;
 constexpr uint32_t kGridPollLimit = 1u << 22;
 
 // Device-wide barrier of a launch whose blocks are all resident (one per
@@ -731,10 +763,26 @@ __global__ void __launch_bounds__(kFbThreads)
     for (int p = 0; p < live; p++) {
         const int shift = startBit + 8 * p;
         auto dig = [&](K k) -> uint32_t { return shift < keyBits ? digit_of(k, shift) : 0u; };
-        // 1. this block's counts
+        // 1. this block's counts: one LDS atomic per distinct digit of a wave
         if (t < 256) s_cnt[t] = 0;
         __syncthreads();
-        for (uint32_t i = lo + (uint32_t)t; i < hi; i += kFbThreads) atomicAdd(&s_cnt[dig(src[i])], 1u);
+        for (uint32_t b0 = lo; b0 < hi; b0 += kFbStep) {  // the same steps in every lane (ballots)
+            K kk[kFbItems];
+#pragma unroll
+            for (int k = 0; k < kFbItems; k++) {  // all loads in flight first
+                const uint32_t i = b0 + (uint32_t)(k * kFbThreads + t);
+                kk[k] = i < hi ? src[i] : (K)0;
+            }
+#pragma unroll
+            for (int k = 0; k < kFbItems; k++) {
+                const bool valid = b0 + (uint32_t)(k * kFbThreads + t) < hi;
+                const uint32_t d = dig(kk[k]);
+                uint32_t plo, phi;
+                digit_peers(d, valid, plo, phi);
+                if (valid && __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u)) == 0)
+                    atomicAdd(&s_cnt[d], (uint32_t)(__popc(plo) + __popc(phi)));
+            }
+        }
         __syncthreads();
         if (t < 256) counts[(size_t)blk * 256 + t] = s_cnt[t];
         grid_barrier(ctrl, base + (++barriers) * G);
@@ -754,20 +802,34 @@ __global__ void __launch_bounds__(kFbThreads)
             const uint32_t start = scan1024(tot, s_tmp, &all);
             if (t < 256) s_off[t] = start + rowPrefix[(size_t)blk * 256 + t];
         }
-        for (int w = wave; w < kFbWaves; w += kFbWaves)
-            for (int d = lane; d < 256; d += 64) s_wcnt[w][d] = 0;
+        for (int d = lane; d < 256; d += 64) s_wcnt[wave][d] = 0;
         __syncthreads();
-        for (uint32_t b0 = lo; b0 < hi; b0 += kFbThreads) {
-            const uint32_t i = b0 + (uint32_t)t;
-            const bool valid = i < hi;
-            const K key = valid ? src[i] : (K)0;
-            const uint32_t val = kVals && valid ? vsrc[i] : 0u;
-            const uint32_t d = dig(key);
-            uint32_t plo, phi;
-            digit_peers(d, valid, plo, phi);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
-            const uint32_t cnt = (uint32_t)(__popc(plo) + __popc(phi));
-            if (valid && rank == 0) s_wcnt[wave][d] = cnt;
+        // kFbItems keys per thread per step; wave w's keys are the step's
+        // positions [w * 64 * kFbItems, (w + 1) * 64 * kFbItems), item k
+        // covering 64 consecutive ones, ranked as in k_pass
+        for (uint32_t b0 = lo; b0 < hi; b0 += kFbStep) {
+            K key[kFbItems];
+            uint32_t val[kVals ? kFbItems : 1], off[kFbItems], d[kFbItems];
+#pragma unroll
+            for (int k = 0; k < kFbItems; k++) {
+                const uint32_t i = b0 + (uint32_t)(wave * 64 * kFbItems + k * 64 + lane);
+                const bool valid = i < hi;
+                key[k] = valid ? src[i] : (K)0;
+                if (kVals) val[k] = valid ? vsrc[i] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < kFbItems; k++) {
+                const uint32_t i = b0 + (uint32_t)(wave * 64 * kFbItems + k * 64 + lane);
+                const bool valid = i < hi;
+                d[k] = dig(key[k]);
+                uint32_t plo, phi;
+                digit_peers(d[k], valid, plo, phi);
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+                const uint32_t cnt = (uint32_t)(__popc(plo) + __popc(phi));
+                const uint32_t prior = valid ? s_wcnt[wave][d[k]] : 0u;
+                off[k] = prior + rank;
+                if (valid && rank == 0) s_wcnt[wave][d[k]] = prior + cnt;
+            }
             __syncthreads();
             uint32_t tot = 0;
             if (t < 256) {  // digit t: the waves' offsets, and the step's total
@@ -779,10 +841,14 @@ __global__ void __launch_bounds__(kFbThreads)
                 }
             }
             __syncthreads();
-            if (valid) {
-                const uint32_t pos = s_off[d] + s_wcnt[wave][d] + rank;
-                dst[pos] = key;
-                if (kVals) vdst[pos] = val;
+#pragma unroll
+            for (int k = 0; k < kFbItems; k++) {
+                const uint32_t i = b0 + (uint32_t)(wave * 64 * kFbItems + k * 64 + lane);
+                if (i < hi) {
+                    const uint32_t pos = s_off[d[k]] + s_wcnt[wave][d[k]] + off[k];
+                    dst[pos] = key[k];
+                    if (kVals) vdst[pos] = val[k];
+                }
             }
             __syncthreads();
             if (t < 256) {
@@ -876,7 +942,7 @@ static void dispatch_pass(int cfg, hipStream_t s, uint32_t tiles, const K* src, 
 // brings it back; the fallback copies its result there itself.
 template <class K>
 static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, uint32_t n, int startBit,
-                       int live, uint32_t tiles, hipStream_t stream, int top)
+                       int live, uint32_t tiles, hipStream_t stream, int top, bool sharedDigits)
 {
     int rc;
     const int keyBits = 8 * (int)sizeof(K);
@@ -894,8 +960,12 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
     // pass's per-tile counts: its look-back table is then filled in advance
     // and its tiles never wait on each other
     const SortInit init{ctx->ctrl, top + live, live, ctx->hist, ctx->status, ctx->statusWords};
-    k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - top), top, ctx->partials, nullptr,
-                                             (uint32_t)sort_tile_keys(ctx), tiles, tileCounts, init);
+    if (sharedDigits)
+        k_hist<K, true><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - top), top, ctx->partials,
+                                                       nullptr, (uint32_t)sort_tile_keys(ctx), tiles, tileCounts, init);
+    else
+        k_hist<K><<<hg, kHistBlock, 0, stream>>>(keys, n, startBit + 8 * (live - top), top, ctx->partials, nullptr,
+                                                 (uint32_t)sort_tile_keys(ctx), tiles, tileCounts, init);
     launch_scan_reduce(ctx, tileCounts, tiles, hg, top, ctx->hist + 256 * (live - top), stream);
     const uint32_t* guard = ctx->ctrl + kCtrlSortFallback;
     K* bufs[2] = {keys, ktmp};
@@ -948,7 +1018,7 @@ static int sort_hybrid(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_
 template <class K>
 static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t* vtmp, uint32_t n, int startBit,
                      int endBit, bool histReady, hipStream_t stream, const uint32_t* readyTileCounts = nullptr,
-                     int readyHistBlocks = 0, int hybridTop = 3)
+                     int readyHistBlocks = 0, int hybridTop = 3, bool sharedDigits = false)
 {
     const int keyBits = 8 * (int)sizeof(K);
     if (startBit < 0 || endBit <= startBit) {
@@ -974,7 +1044,7 @@ static int sort_impl(mlbvh_ctx* ctx, K* keys, uint32_t* vals, K* ktmp, uint32_t*
     if ((rc = ensure_status(ctx, tiles))) return rc;
 
     if (!histReady && n >= kHybridMinKeys && live > hybridTop)
-        return sort_hybrid(ctx, keys, vals, ktmp, vtmp, n, startBit, live, tiles, stream, hybridTop);
+        return sort_hybrid(ctx, keys, vals, ktmp, vtmp, n, startBit, live, tiles, stream, hybridTop, sharedDigits);
 
     // per-tile counts of the first digit (from the caller's histogram kernel
     // or from k_hist here) let the first pass skip its look-back; partial
@@ -1033,13 +1103,14 @@ int sort_records_u64(mlbvh_ctx* ctx, uint64_t* keys, uint64_t* tmp, uint32_t n, 
 }
 
 int sort_pairs_u64(mlbvh_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* ktmp, uint32_t* vtmp, uint32_t n,
-                   int startBit, int endBit, hipStream_t stream, int hybridTop)
+                   int startBit, int endBit, hipStream_t stream, int hybridTop, bool sharedDigits)
 {
     if (hybridTop < 3 || hybridTop > 4) {
         set_error("sort: %d top-digit passes (3 or 4)", hybridTop);
         return MLBVH_E_INVALID_ARG;
     }
-    return sort_impl<uint64_t>(ctx, keys, vals, ktmp, vtmp, n, startBit, endBit, false, stream, nullptr, 0, hybridTop);
+    return sort_impl<uint64_t>(ctx, keys, vals, ktmp, vtmp, n, startBit, endBit, false, stream, nullptr, 0, hybridTop,
+                               sharedDigits);
 }
 
 void launch_hist_reduce(mlbvh_ctx* ctx, int blocks, int nPass, hipStream_t stream, int firstHist,
